@@ -1,0 +1,363 @@
+"""Python view of the MI355X block-based triangle counter (libbbtc.so).
+
+A thin ctypes layer over the C-ABI in include/bbtc.h. It mirrors the
+reference's objects for this path (paths relative to the GT-TDAlab/bbTC tree):
+
+    Graph.load / Graph.rmat / Graph.gnm   sarma::Matrix(path) (.bbtc input)
+    Graph.partition(kind, p)              utils.hpp:277-288 (uni | n1d | pal)
+    Layout(graph, cuts)                   BlockCSR(cuts, x_adj, adj)
+    Layout.tasks(order)                   tasks::TaskList + tasks::SortTasks
+    Layout.count_cpu(tasks)               BBTc::NoOfTriangles, CPU build
+    Device / DeviceGraph.count(tasks)     BBTc::NoOfTriangles, GPU build
+
+All counting happens in the native library; if libbbtc.so is missing every
+entry point raises instead of computing anything in Python.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbbtc.so")
+CLI_PATH = os.path.join(_HERE, "bbtc")
+
+PART = {"uni": 0, "n1d": 1, "pal": 2}
+ORDER = {"lex": 0, "desc": 1, "asc": 2}
+KERNEL = {"auto": 0, "merge": 1, "bitmap": 2}
+
+
+class BbtcError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"bbtc status {status}: {msg}")
+        self.status = status
+
+
+class bbtc_task(C.Structure):
+    _fields_ = [("i", C.c_uint32), ("j", C.c_uint32), ("k", C.c_uint32)]
+
+
+class bbtc_tile_info(C.Structure):
+    _fields_ = [("nnz", C.c_uint64), ("height", C.c_uint32), ("width", C.c_uint32),
+                ("nz_rows", C.c_uint32), ("avg_degree", C.c_float)]
+
+
+class bbtc_opts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64)]
+
+
+TASK_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("k", np.uint32)])
+
+_lib = None
+
+
+def _declare(lib):
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    pvp = C.POINTER(C.c_void_p)
+    st = C.c_int
+    sig = {
+        "bbtc_strerror": (C.c_char_p, [i32]),
+        "bbtc_last_error": (C.c_char_p, []),
+        "bbtc_hgraph_load": (st, [C.c_char_p, pvp]),
+        "bbtc_hgraph_save": (st, [vp, C.c_char_p]),
+        "bbtc_hgraph_from_csr": (st, [u32, u32, u64, vp, vp, pvp]),
+        "bbtc_hgraph_from_edges": (st, [u32, u64, vp, vp, pvp]),
+        "bbtc_hgraph_gen_rmat": (st, [u32, u32, C.c_double, C.c_double, C.c_double, u64, pvp]),
+        "bbtc_hgraph_gen_gnm": (st, [u32, u64, u64, pvp]),
+        "bbtc_hgraph_info": (None, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64)]),
+        "bbtc_hgraph_offsets": (vp, [vp]),
+        "bbtc_hgraph_cols": (vp, [vp]),
+        "bbtc_hgraph_free": (None, [vp]),
+        "bbtc_partition": (st, [vp, i32, u32, vp]),
+        "bbtc_layout_build": (st, [vp, vp, u32, pvp]),
+        "bbtc_layout_ncut": (u32, [vp]),
+        "bbtc_layout_cuts": (vp, [vp]),
+        "bbtc_layout_tile": (st, [vp, u32, u32, C.POINTER(bbtc_tile_info)]),
+        "bbtc_layout_device_bytes": (u64, [vp]),
+        "bbtc_layout_free": (None, [vp]),
+        "bbtc_task_count": (u64, [u32]),
+        "bbtc_tasks_build": (st, [vp, i32, vp, u64, C.POINTER(u64)]),
+        "bbtc_tasks_bytes": (st, [vp, vp, u64, u32, vp, C.POINTER(u64), C.POINTER(u64)]),
+        "bbtc_count_cpu": (st, [vp, vp, u64, i32, vp, C.POINTER(u64)]),
+        "bbtc_ctx_create": (st, [i32, C.POINTER(bbtc_opts), pvp]),
+        "bbtc_ctx_destroy": (None, [vp]),
+        "bbtc_ctx_bytes_in_use": (u64, [vp]),
+        "bbtc_graph_upload": (st, [vp, vp, pvp]),
+        "bbtc_graph_bytes": (u64, [vp]),
+        "bbtc_graph_release": (None, [vp]),
+        "bbtc_plan_create": (st, [vp, vp, vp, u64, i32, pvp]),
+        "bbtc_plan_launch": (st, [vp, vp]),
+        "bbtc_plan_results": (st, [vp, vp, C.POINTER(u64)]),
+        "bbtc_plan_time": (st, [vp, i32, vp]),
+        "bbtc_plan_bitmap_tasks": (u64, [vp]),
+        "bbtc_plan_destroy": (None, [vp]),
+        "bbtc_count": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def lib():
+    """Load libbbtc.so (built by __graft_entry__.build / make); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        _lib = _declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _check(status: int):
+    if status != 0:
+        L = lib()
+        raise BbtcError(status, f"{L.bbtc_strerror(status).decode()}: {L.bbtc_last_error().decode()}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def tasks_array(tasks) -> np.ndarray:
+    """(n,3) integers or a TASK_DTYPE array -> contiguous TASK_DTYPE array."""
+    if isinstance(tasks, np.ndarray) and tasks.dtype == TASK_DTYPE:
+        return np.ascontiguousarray(tasks)
+    a = np.asarray(tasks, dtype=np.uint32).reshape(-1, 3)
+    out = np.empty(len(a), dtype=TASK_DTYPE)
+    out["i"], out["j"], out["k"] = a[:, 0], a[:, 1], a[:, 2]
+    return out
+
+
+class Graph:
+    """Degree-ordered upper-triangular CSR held by the native library."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def _make(fn, *args):
+        h = C.c_void_p()
+        _check(fn(*args, C.byref(h)))
+        return Graph(h)
+
+    @classmethod
+    def load(cls, path: str) -> "Graph":
+        return cls._make(lib().bbtc_hgraph_load, path.encode())
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1) -> "Graph":
+        return cls._make(lib().bbtc_hgraph_gen_rmat, scale, edge_factor, a, b, c, seed)
+
+    @classmethod
+    def gnm(cls, n, m, seed) -> "Graph":
+        return cls._make(lib().bbtc_hgraph_gen_gnm, n, m, seed)
+
+    @classmethod
+    def from_csr(cls, offsets, cols, n=None, ncols=None) -> "Graph":
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        col = np.ascontiguousarray(cols, dtype=np.uint32)
+        n = len(off) - 1 if n is None else n
+        ncols = n if ncols is None else ncols
+        return cls._make(lib().bbtc_hgraph_from_csr, n, ncols, int(off[-1]) if len(off) else 0,
+                         _ptr(off), _ptr(col))
+
+    @classmethod
+    def from_edges(cls, n, src, dst) -> "Graph":
+        s = np.ascontiguousarray(src, dtype=np.uint32)
+        d = np.ascontiguousarray(dst, dtype=np.uint32)
+        return cls._make(lib().bbtc_hgraph_from_edges, n, len(s), _ptr(s), _ptr(d))
+
+    def save(self, path: str):
+        _check(lib().bbtc_hgraph_save(self._h, path.encode()))
+
+    @property
+    def info(self):
+        n, nc, m = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        lib().bbtc_hgraph_info(self._h, C.byref(n), C.byref(nc), C.byref(m))
+        return n.value, nc.value, m.value
+
+    @property
+    def n(self):
+        return self.info[0]
+
+    @property
+    def m(self):
+        return self.info[2]
+
+    def offsets(self) -> np.ndarray:
+        n = self.n
+        p = lib().bbtc_hgraph_offsets(self._h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint64)), shape=(n + 1,)).copy()
+
+    def cols(self) -> np.ndarray:
+        m = self.m
+        if m == 0:
+            return np.zeros(0, dtype=np.uint32)
+        p = lib().bbtc_hgraph_cols(self._h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(m,)).copy()
+
+    def partition(self, kind: str, p: int) -> np.ndarray:
+        cuts = np.zeros(p + 1, dtype=np.uint32)
+        _check(lib().bbtc_partition(self._h, PART[kind], p, _ptr(cuts)))
+        return cuts
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_hgraph_free(self._h)
+            self._h = None
+
+
+@dataclass
+class TileInfo:
+    nnz: int
+    height: int
+    width: int
+    nz_rows: int
+    avg_degree: float
+
+
+class Layout:
+    """Block CSR of a graph for given cuts (BlockCSR)."""
+
+    def __init__(self, graph: Graph, cuts):
+        self.graph = graph
+        c = np.ascontiguousarray(cuts, dtype=np.uint32)
+        h = C.c_void_p()
+        _check(lib().bbtc_layout_build(graph._h, _ptr(c), len(c) - 1, C.byref(h)))
+        self._h = h
+        self.p = len(c) - 1
+        self.cuts = c.copy()
+
+    def tile(self, i, k) -> TileInfo:
+        t = bbtc_tile_info()
+        _check(lib().bbtc_layout_tile(self._h, i, k, C.byref(t)))
+        return TileInfo(t.nnz, t.height, t.width, t.nz_rows, t.avg_degree)
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().bbtc_layout_device_bytes(self._h)
+
+    def tasks(self, order: str = "lex") -> np.ndarray:
+        n = C.c_uint64()
+        cap = lib().bbtc_task_count(self.p)
+        out = np.empty(cap, dtype=TASK_DTYPE)
+        _check(lib().bbtc_tasks_build(self._h, ORDER[order], _ptr(out), cap, C.byref(n)))
+        return out[: n.value]
+
+    def task_bytes(self, tasks, offset_bytes: int):
+        t = tasks_array(tasks)
+        per = np.zeros(len(t), dtype=np.uint64)
+        tot, wed = C.c_uint64(), C.c_uint64()
+        _check(lib().bbtc_tasks_bytes(self._h, _ptr(t), len(t), offset_bytes, _ptr(per),
+                                      C.byref(tot), C.byref(wed)))
+        return per, tot.value, wed.value
+
+    def count_cpu(self, tasks, nthreads: int = 0):
+        t = tasks_array(tasks)
+        per = np.zeros(len(t), dtype=np.uint64)
+        tot = C.c_uint64()
+        _check(lib().bbtc_count_cpu(self._h, _ptr(t), len(t), nthreads, _ptr(per), C.byref(tot)))
+        return per, tot.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_layout_free(self._h)
+            self._h = None
+
+
+class Device:
+    """A bbtc context on one GPU (stream, counters, memory accounting)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        o = bbtc_opts(hbm_budget_bytes)
+        h = C.c_void_p()
+        _check(lib().bbtc_ctx_create(device, C.byref(o), C.byref(h)))
+        self._h = h
+
+    @property
+    def bytes_in_use(self) -> int:
+        return lib().bbtc_ctx_bytes_in_use(self._h)
+
+    def upload(self, layout: Layout) -> "DeviceGraph":
+        return DeviceGraph(self, layout)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceGraph:
+    def __init__(self, dev: Device, layout: Layout):
+        self.dev = dev
+        self.layout = layout
+        h = C.c_void_p()
+        _check(lib().bbtc_graph_upload(dev._h, layout._h, C.byref(h)))
+        self._h = h
+
+    @property
+    def bytes(self) -> int:
+        return lib().bbtc_graph_bytes(self._h)
+
+    def plan(self, tasks, kernel: str = "auto") -> "Plan":
+        return Plan(self, tasks, kernel)
+
+    def count(self, tasks, kernel: str = "auto"):
+        t = tasks_array(tasks)
+        per = np.zeros(len(t), dtype=np.uint64)
+        tot = C.c_uint64()
+        _check(lib().bbtc_count(self.dev._h, self._h, _ptr(t), len(t), KERNEL[kernel], _ptr(per),
+                                C.byref(tot), None))
+        return per, tot.value
+
+    def release(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_graph_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.release()
+
+
+class Plan:
+    def __init__(self, g: DeviceGraph, tasks, kernel: str = "auto"):
+        self.g = g
+        self.tasks = tasks_array(tasks)
+        h = C.c_void_p()
+        _check(lib().bbtc_plan_create(g.dev._h, g._h, _ptr(self.tasks), len(self.tasks),
+                                      KERNEL[kernel], C.byref(h)))
+        self._h = h
+
+    @property
+    def bitmap_tasks(self) -> int:
+        return lib().bbtc_plan_bitmap_tasks(self._h)
+
+    def launch(self):
+        _check(lib().bbtc_plan_launch(self._h, None))
+
+    def results(self):
+        per = np.zeros(len(self.tasks), dtype=np.uint64)
+        tot = C.c_uint64()
+        _check(lib().bbtc_plan_results(self._h, _ptr(per), C.byref(tot)))
+        return per, tot.value
+
+    def time(self, iters: int) -> np.ndarray:
+        ms = np.zeros(iters, dtype=np.float32)
+        _check(lib().bbtc_plan_time(self._h, iters, _ptr(ms)))
+        return ms
+
+    def destroy(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.destroy()

@@ -1,0 +1,79 @@
+// Internal C++ interfaces of the host library (graph I/O, generators,
+// partitioners, block layout, tasks, CPU count).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace bbtc {
+
+// ---- CSR ------------------------------------------------------------------
+// Throws Error(BBTC_ERR_FORMAT) unless offsets start at 0, never decrease,
+// end at m, and every row is strictly increasing with all columns > row and
+// < max(n, ncols).
+void validate_csr(const Csr& g);
+Csr load_bbtc(const std::string& path);
+void save_bbtc(const Csr& g, const std::string& path);
+Csr copy_csr(uint32_t n, uint32_t ncols, uint64_t m, const uint64_t* off, const uint32_t* col);
+
+// ---- generators -----------------------------------------------------------
+// Canonical input from an undirected pair list: symmetrise, drop self loops
+// and duplicates, rank by (degree, id) ascending, keep the upper triangle.
+Csr canonical_from_edges(uint32_t n, uint64_t nedges, const uint32_t* src, const uint32_t* dst);
+Csr gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed);
+Csr gen_gnm(uint32_t n, uint64_t m, uint64_t seed);
+
+// ---- partitioners ---------------------------------------------------------
+std::vector<uint32_t> partition_uni(const Csr& g, uint32_t p);
+std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t p);
+std::vector<uint32_t> partition_pal(const Csr& g, uint32_t p);
+// Dispatch + cuts[p] = n + 1, as the reference driver does.
+std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p);
+
+// ---- block layout -----------------------------------------------------------
+// Tile (i,k), i <= k, of a P x P grid over the degree-ordered upper triangle.
+// Stored per upper tile t (dense P*P indexing, lower tiles unused):
+//   rowptr[rp_base[t] + r]   tile-local offset of local row r, r in [0, h_i]
+//   cols[col_base[t] + e]    tile-local column id (global col - cuts[k])
+//   nzrows[nz_base[t] + q]   q-th non-empty local row
+// col_base is a multiple of 32 entries (128 bytes) so every tile segment
+// starts on a cache line.
+struct Layout {
+  uint32_t P = 0;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  std::vector<uint32_t> cuts;      // P+1, cuts[P] == n + 1
+  std::vector<uint64_t> rp_base;   // P*P
+  std::vector<uint64_t> col_base;  // P*P
+  std::vector<uint64_t> nz_base;   // P*P
+  std::vector<uint64_t> nnz;       // P*P
+  std::vector<uint32_t> nz_rows;   // P*P
+  std::vector<float> deg;          // P*P, (float)nnz / (float)h
+  Buf<uint32_t> rowptr;
+  Buf<uint32_t> cols;
+  Buf<uint32_t> nzrows;
+
+  uint32_t height(uint32_t b) const { return cuts[b + 1] - cuts[b]; }
+  size_t tile(uint32_t i, uint32_t k) const { return size_t(i) * P + k; }
+  uint64_t device_bytes() const {
+    return rowptr.size() * 4 + cols.size() * 4 + nzrows.size() * 4 + cuts.size() * 4 +
+           3 * rp_base.size() * 8 + nnz.size() * 8;
+  }
+};
+Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts);
+
+// ---- tasks ----------------------------------------------------------------
+std::vector<bbtc_task> task_list(uint32_t p);
+void sort_tasks(std::vector<bbtc_task>& t, const Layout& L, bbtc_order order);
+// Algorithmic bytes (see bbtc_tasks_bytes).
+void task_bytes(const Layout& L, const bbtc_task* t, uint64_t n, uint32_t o, uint64_t* per_task,
+                uint64_t* total, uint64_t* wedges);
+// True when the reference would skip the task: a tile with average degree 0.
+bool task_skipped(const Layout& L, const bbtc_task& t);
+
+// ---- CPU count ------------------------------------------------------------
+uint64_t count_cpu(const Layout& L, const bbtc_task* t, uint64_t n, int nthreads,
+                   uint64_t* per_task);
+
+}  // namespace bbtc

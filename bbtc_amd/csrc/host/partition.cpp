@@ -1,0 +1,265 @@
+// 2D partitioners behind the reference's `--partitioner uni|n1d|pal`.
+//
+// Each one reproduces the reference's cut vector bit for bit, including its
+// integer widths (everything is 32-bit unsigned, as Ordinal/Value are in
+// definitions.h:9-10) and its random sample (std::mt19937_64 seeded with
+// 1091467301 feeding std::uniform_real_distribution<double>).
+#include <omp.h>
+
+#include <cmath>
+#include <limits>
+#include <random>
+#include <vector>
+
+#include "graph.hpp"
+
+namespace bbtc {
+
+// ---- uni: sarma::uniform::partition(N, P) (uniform.hpp:20-27) ---------------
+std::vector<uint32_t> partition_uni(const Csr& g, uint32_t p) {
+  std::vector<uint32_t> c(size_t(p) + 1);
+  for (uint32_t i = 0; i <= p; ++i) c[i] = uint32_t((long long)i * (long long)g.n / p);
+  return c;
+}
+
+// ---- n1d: sarma::nicol1d::partition(indptr, P) (nicol1d.hpp:30-216) ---------
+namespace {
+
+using u32 = uint32_t;
+
+// Probe of nicol1d.hpp:30-65 for one prefix array (Q = 1): greedy cuts under
+// load bound L, each cut as far right as possible but not beyond UB.
+std::vector<u32> n1d_probe(const std::vector<u32>& prefix, u32 L, const std::vector<u32>& UB) {
+  std::vector<u32> p;
+  p.reserve(UB.size());
+  p.push_back(0);
+  while (p.size() < UB.size()) {
+    u32 r = UB[p.size()];
+    u32 l = p.back();
+    const u32 base = prefix[p.back()];
+    while (l < r) {
+      const u32 m = (l + 1) + (r - (l + 1)) / 2;
+      const u32 cl = prefix[m] - base;
+      if (cl <= L) l = m;
+      else r = m - 1;
+    }
+    p.push_back(r);
+    if (p.back() == p[p.size() - 2]) {
+      while (p.size() < UB.size()) p.push_back(p.back());
+      break;
+    }
+  }
+  return p;
+}
+
+}  // namespace
+
+std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t P) {
+  // The weight of element i is indptr[i] itself (the reference passes the
+  // offset array, utils.hpp:281) and the prefix is accumulated in 32 bits,
+  // wrapping (nicol1d.hpp:211-214).
+  const size_t asz = size_t(g.n) + 1;
+  // One trailing zero: partition_indices reads prefix[l + 1] with l == N at
+  // the last cut (nicol1d.hpp:144), one element past the array; the zero
+  // filled pages a large allocation sits in supply 0 there.
+  std::vector<u32> prefix(asz + 2, 0);
+  for (size_t i = 0; i < asz; ++i) prefix[i + 1] = prefix[i] + u32(g.off[i]);
+  const u32 N = u32(asz);  // prefix.size() - 1 in the reference
+  std::vector<u32> UB(size_t(P) + 1, N);
+  u32 lower = prefix[N] / P, upper = prefix[N];
+  auto cub = [&](u32 l, u32 r) { return u32(prefix[r] - prefix[l]); };
+  u32 pback = 0;
+  for (u32 i = 1; i <= P; ++i) {
+    u32 l = pback, r = UB[i];
+    while (l < r) {
+      const u32 m = (l + 1) + (r - (l + 1)) / 2;
+      const u32 b = cub(pback, m);
+      if (b > upper) r = m - 1;
+      else if (b < lower) l = m;
+      else {
+        auto pr = n1d_probe(prefix, b, UB);
+        if (pr.back() < N) { l = m; lower = b; }
+        else { r = m - 1; UB = pr; upper = b; }
+      }
+    }
+    const u32 b = cub(pback, l + 1);
+    if (lower < b && b < upper) {
+      auto pr = n1d_probe(prefix, b, UB);
+      if (pr.back() >= N) { UB = pr; upper = b; }
+      else lower = b;
+    }
+    pback = l;
+  }
+  return n1d_probe(prefix, upper, UB);
+}
+
+// ---- pal: sparsify + sarma::probe_a_load::partition ----------------------------
+namespace {
+
+// Bernoulli sample of the nonzeros, as sarma::utils::sparse_mask
+// (tools/utils.hpp:81-106) + Matrix::sparsify (csr_matrix.hpp:435-454).
+// Returns the sampled entries as (row offsets, cols).
+void sparsify(const Csr& g, double keep, int seed, std::vector<u32>& sp_off, std::vector<u32>& sp_col) {
+  const size_t N = size_t(u32(g.m));  // the reference's nnz is a 32-bit Ordinal
+  std::vector<bool> mask(N, false);
+  std::mt19937_64 gen(seed);
+  std::uniform_real_distribution<double> rnd;
+  if (keep >= 1.0) {
+    mask.assign(N, true);
+  } else if (keep > 1.0 / 3) {
+    for (size_t i = 0; i < N; ++i) mask[i] = rnd(gen) <= keep;
+  } else {
+    const double Q = std::log(1 - keep);
+    for (size_t i = std::numeric_limits<size_t>::max();;) {
+      const size_t fc = size_t(std::log(1 - rnd(gen)) / Q);
+      i += std::min(fc, mask.size());
+      if (++i < mask.size()) mask[i] = true;
+      else break;
+    }
+  }
+  sp_off.assign(size_t(g.n) + 1, 0);
+  sp_col.clear();
+  for (u32 r = 0; r < g.n; ++r) {
+    sp_off[r + 1] = sp_off[r];
+    for (uint64_t x = g.off[r]; x < g.off[r + 1]; ++x)
+      if (x < N && mask[x]) { sp_col.push_back(g.col[x]); sp_off[r + 1]++; }
+  }
+}
+
+// Exact dominance counts over the sample: query(i, j) = #samples with
+// row < i and col < j, the value sarma::sparse_prefix_sum::query returns
+// (sparse_prefix_sum.hpp:318-322). A Fenwick tree over sample positions
+// (row-major order) with a sorted column list per node.
+class PrefixCounter {
+ public:
+  PrefixCounter(const std::vector<u32>& off, const std::vector<u32>& col) : off_(off) {
+    S_ = col.size();
+    node_start_.assign(S_ + 2, 0);
+    for (size_t k = 1; k <= S_; ++k) node_start_[k + 1] = node_start_[k] + (k & (~k + 1));
+    data_.resize(node_start_[S_ + 1]);
+    for (size_t k = 1; k <= S_; ++k) {
+      const size_t len = k & (~k + 1);
+      u32* dst = data_.data() + node_start_[k];
+      for (size_t q = 0; q < len; ++q) dst[q] = col[k - len + q];
+      std::sort(dst, dst + len);
+    }
+  }
+  u32 total() const { return u32(S_); }
+  // Number of samples with row < i and col < j.
+  u32 query(u32 i, u32 j) const {
+    const size_t rows = off_.size() - 1;
+    size_t r = off_[std::min<size_t>(i, rows)];
+    u32 s = 0;
+    for (; r > 0; r &= r - 1) {
+      const u32* b = data_.data() + node_start_[r];
+      const u32* e = data_.data() + node_start_[r + 1];
+      s += u32(std::lower_bound(b, e, j) - b);
+    }
+    return s;
+  }
+
+ private:
+  const std::vector<u32>& off_;
+  size_t S_;
+  std::vector<size_t> node_start_;
+  std::vector<u32> data_;
+};
+
+// Max tile load over one row band (col == false) or one column band
+// (col == true) of the cut vector p (sparse_prefix_sum.hpp:351-387).
+u32 band_maxload(const PrefixCounter& pc, const std::vector<u32>& p, size_t idx, bool col) {
+  u32 best = 0;
+  if (col) {
+    const u32 q0 = p[idx - 1], q1 = p[idx];
+    u32 prev0 = pc.query(p[0], q0), prev1 = pc.query(p[0], q1);
+    for (size_t a = 1; a < p.size(); ++a) {
+      const u32 c0 = pc.query(p[a], q0), c1 = pc.query(p[a], q1);
+      const u32 load = c1 - prev1 + prev0 - c0;
+      best = std::max(best, load);
+      prev0 = c0;
+      prev1 = c1;
+    }
+  } else {
+    const u32 r0 = p[idx - 1], r1 = p[idx];
+    u32 prev0 = pc.query(r0, p[0]), prev1 = pc.query(r1, p[0]);
+    for (size_t b = 1; b < p.size(); ++b) {
+      const u32 c0 = pc.query(r0, p[b]), c1 = pc.query(r1, p[b]);
+      const u32 load = c1 - c0 + prev0 - prev1;
+      best = std::max(best, load);
+      prev0 = c0;
+      prev1 = c1;
+    }
+  }
+  return best;
+}
+
+// probe_a_load::probe (probe_a_load.hpp:29-57) with an upper-bound vector.
+std::vector<u32> pal_probe(const PrefixCounter& pc, u32 L, const std::vector<u32>& UB) {
+  std::vector<u32> p;
+  p.reserve(UB.size() + 1);
+  p.push_back(0);
+  while (p.size() < UB.size()) {
+    u32 l = p.back(), r = UB[p.size()];
+    while (l < r) {
+      const u32 m = (l + 1) + (r - (l + 1)) / 2;
+      p.push_back(m);
+      const size_t idx = p.size() - 1;
+      const u32 cl = std::max(band_maxload(pc, p, idx, true), band_maxload(pc, p, idx, false));
+      if (cl <= L) l = m;
+      else r = m - 1;
+      p.pop_back();
+    }
+    p.push_back(l);
+    if (p.back() == p[p.size() - 2]) {
+      while (p.size() < UB.size()) p.push_back(p.back());
+      break;
+    }
+  }
+  return p;
+}
+
+}  // namespace
+
+std::vector<uint32_t> partition_pal(const Csr& g, uint32_t P) {
+  // keep probability get_prob(NNZ, p, p, 10.) (tools/utils.hpp:20-25), seed
+  // 1091467301 (utils.hpp:271-273). The reference's generator is a
+  // thread_local seeded on first use; its driver samples once per process, so
+  // every call here starts a fresh stream.
+  const u32 M32 = u32(g.m);
+  const double keep = std::min(1.0, (P * 1.0 * 10.0) * (P * 1.0 * 10.0) / M32);
+  std::vector<u32> sp_off, sp_col;
+  sparsify(g, keep, 1091467301, sp_off, sp_col);
+  PrefixCounter pc(sp_off, sp_col);
+  // probe_a_load::partition (probe_a_load.hpp:70-89); N = sps.size() =
+  // max(rows, ncols) (sparse_prefix_sum.hpp:257-259).
+  const u32 N = std::max(g.n, g.ncols);
+  const u32 M = pc.total();
+  u32 l = (M - 1 + P * P) / P / P;
+  u32 r = std::min(M, (M + P - 1) / P + N);
+  std::vector<u32> UB(size_t(P) + 1, N);
+  while (l + .5 < r) {
+    const u32 m = l + (r - l) / 2;
+    auto pr = pal_probe(pc, m, UB);
+    if (pr.back() < N) l = m + 1;
+    else { r = m; UB = pr; }
+  }
+  return pal_probe(pc, r, UB);
+}
+
+std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p) {
+  if (p < 1) fail(BBTC_ERR_INVALID, "number of cuts must be >= 1");
+  std::vector<uint32_t> c;
+  switch (kind) {
+    case BBTC_PART_UNI: c = partition_uni(g, p); break;
+    case BBTC_PART_N1D: c = partition_n1d(g, p); break;
+    case BBTC_PART_PAL: c = partition_pal(g, p); break;
+    default: fail(BBTC_ERR_INVALID, "unknown partitioner");
+  }
+  if (c.size() != size_t(p) + 1) fail(BBTC_ERR_INTERNAL, "partitioner returned a wrong cut count");
+  c[p] = g.n + 1;  // utils.hpp:288
+  for (uint32_t b = 0; b < p; ++b)
+    if (c[b] > c[b + 1]) fail(BBTC_ERR_INTERNAL, "partitioner returned decreasing cuts");
+  return c;
+}
+
+}  // namespace bbtc

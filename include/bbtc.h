@@ -1,0 +1,224 @@
+/*
+ * bbtc.h - C-ABI of the MI355X block-based triangle counter.
+ *
+ * Plain C: opaque handles, fixed-width integers, pointers and sizes. No HIP,
+ * torch or C++ types cross this boundary (streams are passed as void*, which
+ * is a hipStream_t; NULL means the library's own stream).
+ *
+ * Every entry point returns a bbtc_status; nothing calls exit(). On failure
+ * bbtc_last_error() returns a message for the calling thread.
+ *
+ * Each declaration cites the reference interface it replaces (paths are
+ * relative to the GT-TDAlab/bbTC source tree).
+ */
+#ifndef BBTC_H_
+#define BBTC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bbtc_status {
+  BBTC_OK = 0,
+  BBTC_ERR_INVALID = 1,   /* bad argument or unsupported parameter */
+  BBTC_ERR_IO = 2,        /* file could not be opened, read or written */
+  BBTC_ERR_FORMAT = 3,    /* malformed .bbtc file or CSR content */
+  BBTC_ERR_NOMEM = 4,     /* host allocation failed */
+  BBTC_ERR_HIP = 5,       /* a HIP runtime call failed */
+  BBTC_ERR_NODEVICE = 6,  /* no usable GPU */
+  BBTC_ERR_BUDGET = 7,    /* device memory budget of the context exceeded */
+  BBTC_ERR_INTERNAL = 8
+} bbtc_status;
+
+/* Replaces nothing: the reference reports no errors (utils.hpp:79-85 exits
+ * with status 0; CUDA calls are unchecked). */
+const char* bbtc_strerror(bbtc_status s);
+const char* bbtc_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host graph: degree-ordered upper-triangular CSR (.bbtc content).          */
+/* ------------------------------------------------------------------------ */
+typedef struct bbtc_hgraph bbtc_hgraph;
+
+/* Replaces sarma::Matrix(path) -> pigo::CSR::read_bin_
+ * (SARMA csr_matrix.hpp:132-152; PIGO csr.impl.hpp:475-512). Accepts the
+ * PIGO-CSR-v2 binary with L=4 and O in {4,8}; rejects rows that are not
+ * strictly increasing or not above the diagonal. */
+bbtc_status bbtc_hgraph_load(const char* path, bbtc_hgraph** out);
+
+/* Replaces pigo::CSR::save (PIGO csr.impl.hpp:428-473) as used by
+ * utils.hpp:260-268. Writes L=4 and O=4 when m < 2^32, else O=8. */
+bbtc_status bbtc_hgraph_save(const bbtc_hgraph* g, const char* path);
+
+/* Copies a caller CSR (validated like bbtc_hgraph_load). */
+bbtc_status bbtc_hgraph_from_csr(uint32_t n, uint32_t ncols, uint64_t m,
+                                 const uint64_t* offsets, const uint32_t* cols,
+                                 bbtc_hgraph** out);
+
+/* Builds the canonical input from an undirected edge list: symmetrise, drop
+ * self loops and duplicates, rank vertices by (degree, id) ascending, keep
+ * the upper triangle, sort rows. The reference's text ingest
+ * (utils.hpp:171-268) does the same except deduplication. */
+bbtc_status bbtc_hgraph_from_edges(uint32_t n, uint64_t nedges, const uint32_t* src,
+                                   const uint32_t* dst, bbtc_hgraph** out);
+
+/* Seeded synthetic inputs (counter-based SplitMix64 streams; identical for any
+ * thread count). RMAT: edge_factor * 2^scale samples with quadrant
+ * probabilities a, b, c (d = 1-a-b-c). G(n,m): the first m distinct
+ * non-loop pairs of a uniform pair stream. */
+bbtc_status bbtc_hgraph_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b,
+                                 double c, uint64_t seed, bbtc_hgraph** out);
+bbtc_status bbtc_hgraph_gen_gnm(uint32_t n, uint64_t m, uint64_t seed, bbtc_hgraph** out);
+
+void bbtc_hgraph_info(const bbtc_hgraph* g, uint32_t* n, uint32_t* ncols, uint64_t* m);
+const uint64_t* bbtc_hgraph_offsets(const bbtc_hgraph* g);
+const uint32_t* bbtc_hgraph_cols(const bbtc_hgraph* g);
+void bbtc_hgraph_free(bbtc_hgraph* g);
+
+/* ------------------------------------------------------------------------ */
+/* Partitioners.                                                             */
+/* ------------------------------------------------------------------------ */
+typedef enum bbtc_part { BBTC_PART_UNI = 0, BBTC_PART_N1D = 1, BBTC_PART_PAL = 2 } bbtc_part;
+
+/* Replaces the partition dispatch of utils.hpp:271-288:
+ *   uni -> sarma::uniform::partition (uniform.hpp:20-27, 44-49)
+ *   n1d -> sarma::nicol1d::partition(indptr) (nicol1d.hpp:106-216)
+ *   pal -> sparsify (csr_matrix.hpp:435-454, tools/utils.hpp:20-25, 81-106)
+ *          then sarma::probe_a_load::partition (probe_a_load.hpp:29-89)
+ * Writes p+1 cuts into cuts_out exactly as the driver uses them, i.e. with
+ * cuts_out[p] overwritten by n+1 (utils.hpp:288). Requires p >= 1. */
+bbtc_status bbtc_partition(const bbtc_hgraph* g, bbtc_part kind, uint32_t p, uint32_t* cuts_out);
+
+/* ------------------------------------------------------------------------ */
+/* Block CSR layout (host copy; uploaded with bbtc_graph_upload).            */
+/* ------------------------------------------------------------------------ */
+typedef struct bbtc_layout bbtc_layout;
+
+typedef struct bbtc_tile_info {
+  uint64_t nnz;         /* entries in the tile */
+  uint32_t height;      /* rows of band i (cuts[i+1]-cuts[i]) */
+  uint32_t width;       /* columns of band k */
+  uint32_t nz_rows;     /* rows with at least one entry */
+  float avg_degree;     /* (float)nnz / height, as block_csr.cpp:278 (NaN when height is 0) */
+} bbtc_tile_info;
+
+/* Replaces BlockCSR(cuts, x_adj, adj) -> CSRtoBCSR + ComputeTileProps
+ * (block_csr.cpp:34-42, 257-376). Only tiles (i,k) with i <= k are stored;
+ * each keeps tile-local 32-bit row offsets, tile-local column ids in a
+ * 128-byte aligned segment, and a compact list of its non-empty rows. */
+bbtc_status bbtc_layout_build(const bbtc_hgraph* g, const uint32_t* cuts, uint32_t p,
+                              bbtc_layout** out);
+uint32_t bbtc_layout_ncut(const bbtc_layout* L);
+const uint32_t* bbtc_layout_cuts(const bbtc_layout* L);
+/* Replaces the BlockCSR getters GetNNZ/GetHeight/GetWidth/GetTileDeg
+ * (block_csr.cpp:87-186). Lower tiles (i > k) report zero entries. */
+bbtc_status bbtc_layout_tile(const bbtc_layout* L, uint32_t i, uint32_t k, bbtc_tile_info* out);
+/* Bytes the layout occupies once uploaded. */
+uint64_t bbtc_layout_device_bytes(const bbtc_layout* L);
+void bbtc_layout_free(bbtc_layout* L);
+
+/* ------------------------------------------------------------------------ */
+/* Tasks.                                                                    */
+/* ------------------------------------------------------------------------ */
+/* A task (i,j,k), i <= j <= k, counts triangles u<v<w with u in band i,
+ * v in band j, w in band k, over tiles ab=(i,j), bc=(j,k), ac=(i,k). */
+typedef struct bbtc_task { uint32_t i, j, k; } bbtc_task;
+
+typedef enum bbtc_order {
+  BBTC_ORDER_LEX = 0,   /* i, j, k lexicographic (tasks.hpp:7-17) */
+  BBTC_ORDER_DESC = 1,  /* by weight, descending: the GPU build (tasks.hpp:25-28) */
+  BBTC_ORDER_ASC = 2    /* by weight, ascending: the CPU build (tasks.hpp:29-32) */
+} bbtc_order;
+
+/* C(p+2,3). */
+uint64_t bbtc_task_count(uint32_t p);
+
+/* Replaces tasks::TaskList + tasks::SortTasks (tasks.hpp:7-34): weight is
+ * nnz(ab) * max(deg(bc), deg(ac)) in float, sorted with std::sort. */
+bbtc_status bbtc_tasks_build(const bbtc_layout* L, bbtc_order order, bbtc_task* out,
+                             uint64_t cap, uint64_t* n_out);
+
+/* Exact algorithmic bytes per task for the roofline:
+ *   B = sum over rows u of band i with nonempty ab(u) and ac(u) of
+ *       (2o + 4 d_ab(u)) + (2o + 4 d_ac(u)) + sum_{v in ab(u)} (2o + 4 d_bc(v))
+ * with o = offset_bytes. Also returns the wedge count sum d_ab(u). Tasks with
+ * an empty tile contribute nothing (bb_tc.cpp:86-91). Either output array may
+ * be NULL. */
+bbtc_status bbtc_tasks_bytes(const bbtc_layout* L, const bbtc_task* tasks, uint64_t n,
+                             uint32_t offset_bytes, uint64_t* per_task_bytes,
+                             uint64_t* total_bytes, uint64_t* total_wedges);
+
+/* ------------------------------------------------------------------------ */
+/* CPU count: the --ngpu 0 path and the host tail of --border.               */
+/* ------------------------------------------------------------------------ */
+/* Replaces BBTc::NoOfTriangles (CPU build, bb_tc.cpp:255-273) with
+ * RunCpuKernels / NoOfTrianglesInCpus / TrianglesIn{Sparse,Dense}Task
+ * (bb_tc.cpp:118-139, 184-197; cpu_sparse_kernel.cpp; cpu_dense_kernel.cpp):
+ * workers claim tasks from the back of the list. nthreads <= 0 means all
+ * OpenMP threads. per_task (may be NULL) receives one count per task. */
+bbtc_status bbtc_count_cpu(const bbtc_layout* L, const bbtc_task* tasks, uint64_t n,
+                           int nthreads, uint64_t* per_task, uint64_t* total);
+
+/* ------------------------------------------------------------------------ */
+/* Device side.                                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct bbtc_ctx bbtc_ctx;
+typedef struct bbtc_dgraph bbtc_dgraph;
+typedef struct bbtc_plan bbtc_plan;
+
+typedef enum bbtc_kernel {
+  BBTC_KERNEL_AUTO = 0,    /* per-task choice (bitmap where the ac band fits LDS and pays) */
+  BBTC_KERNEL_MERGE = 1,   /* wave-cooperative merge / binary-search kernel for every task */
+  BBTC_KERNEL_BITMAP = 2   /* LDS bitmap kernel wherever the ac band fits, merge elsewhere */
+} bbtc_kernel;
+
+typedef struct bbtc_opts {
+  uint64_t hbm_budget_bytes;  /* 0: no limit beyond the device's own */
+} bbtc_opts;
+
+/* Replaces BBTc::PrepareGpu (bb_tc_helper.cpp:53-73): one context per
+ * device holds the stream, counters and scratch. */
+bbtc_status bbtc_ctx_create(int device, const bbtc_opts* opts, bbtc_ctx** out);
+void bbtc_ctx_destroy(bbtc_ctx* ctx);
+/* Device bytes currently held by graphs and plans of this context. */
+uint64_t bbtc_ctx_bytes_in_use(const bbtc_ctx* ctx);
+
+/* Replaces CopyTilesToGpu / CopyTileToGpu (bb_tc_helper.cpp:75-172, the
+ * --copy-first path): the whole block CSR becomes device resident once. */
+bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* L, bbtc_dgraph** out);
+uint64_t bbtc_graph_bytes(const bbtc_dgraph* g);
+void bbtc_graph_release(bbtc_dgraph* g);
+
+/* A plan binds a graph to a task list and a kernel choice: it uploads task
+ * descriptors and the work-item table once, so repeated launches do no host
+ * work. Replaces the per-task launch loop of BBTc::RunGpuKernels /
+ * NoOfTrianglesInGPU (bb_tc.cpp:76-115, 142-181). */
+bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                             bbtc_kernel kernel, bbtc_plan** out);
+/* Zeroes the plan's per-task counters and enqueues the count on stream
+ * (NULL: the context stream). Asynchronous. */
+bbtc_status bbtc_plan_launch(bbtc_plan* plan, void* stream);
+/* Waits for the plan's last launch and copies per-task counts (may be NULL)
+ * and their sum. */
+bbtc_status bbtc_plan_results(bbtc_plan* plan, uint64_t* per_task, uint64_t* total);
+/* Launches the plan iters times on the context stream, recording a HIP event
+ * pair around each launch; ms_each[iters] receives the event times. */
+bbtc_status bbtc_plan_time(bbtc_plan* plan, int iters, float* ms_each);
+/* Number of tasks the plan routed to the bitmap kernel. */
+uint64_t bbtc_plan_bitmap_tasks(const bbtc_plan* plan);
+void bbtc_plan_destroy(bbtc_plan* plan);
+
+/* One-shot convenience: plan, launch, wait, copy, destroy. Replaces
+ * BBTc::NoOfTriangles (GPU build, bb_tc.cpp:200-252) for a fully resident
+ * graph; per_task may be NULL. */
+bbtc_status bbtc_count(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                       bbtc_kernel kernel, uint64_t* per_task, uint64_t* total, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBTC_H_ */
