@@ -1,19 +1,13 @@
-// Device side of the block-based triangle counter for gfx950 (CDNA4).
+// Device side of the block-based triangle counter for gfx950 (CDNA4): the
+// context, device-resident graphs, plans and their launches. The kernels are
+// in kernels.hpp.
 //
 // A graph's whole block CSR is resident in HBM. A plan turns a task list into
-// a flat table of work items (a task and a chunk of 64 non-empty rows of its
-// ab tile); one persistent launch drains that table through an atomic
-// head counter, so uneven tasks share the machine instead of being
-// serialised launch by launch as in bb_tc.cpp:142-181.
-//
-// Kernels:
-//   count_merge  - (d) wave-cooperative merge: the 64 lanes of a wave flatten
-//                  the (u, v) wedges of their 64 rows with a wave prefix sum
-//                  and each lane intersects ac(u) with bc(v); counts reduce
-//                  across the wave into one 64-bit atomic per item.
-//   count_bitmap - (c) LDS bitmap: a workgroup marks ac(u) of its rows in an
-//                  LDS bitmap over the k band and probes every bc(v) entry
-//                  with one bit test, counting with ballot + popcount.
+// a flat table of work items and chooses, per task, the LDS form of ac(u)
+// (sorted list for binary search, or bitmap); one persistent launch drains
+// every item through an atomic head counter, so uneven tasks share the
+// machine instead of being serialised launch by launch as in
+// bb_tc.cpp:142-181.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +15,7 @@
 #include <vector>
 
 #include "../host/handles.hpp"
+#include "kernels.hpp"
 
 namespace bbtc {
 
@@ -31,171 +26,6 @@ namespace bbtc {
       ::bbtc::fail(BBTC_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_));    \
   } while (0)
 
-// ---------------------------------------------------------------------------
-// device-side descriptors
-// ---------------------------------------------------------------------------
-struct TileDesc {
-  uint64_t rp;    // offset of the tile's row offsets in rowptr
-  uint64_t col;   // offset of the tile's columns in cols
-  uint64_t nz;    // offset of the tile's non-empty row list in nzrows
-  uint32_t nzr;   // number of non-empty rows
-  uint32_t h;     // band height
-};
-static_assert(sizeof(TileDesc) == 32, "TileDesc layout");
-
-struct TaskDesc {
-  uint32_t ab, bc, ac;  // tile ids (i*P+j, j*P+k, i*P+k)
-  uint32_t kind;        // 0 merge, 1 bitmap
-};
-
-struct CountArgs {
-  const uint32_t* __restrict__ rowptr;
-  const uint32_t* __restrict__ cols;
-  const uint32_t* __restrict__ nzrows;
-  const TileDesc* __restrict__ tiles;
-  const TaskDesc* __restrict__ tasks;
-  const uint64_t* __restrict__ item_prefix;  // ntasks + 1
-  uint32_t ntasks;
-  uint64_t nitems;
-  unsigned long long* per_task;
-  unsigned long long* head;  // work counter
-};
-
-constexpr int kWave = 64;
-constexpr int kRowsPerItem = 64;
-constexpr int kBlock = 256;
-
-// ---------------------------------------------------------------------------
-// small device helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off, kWave);
-    if (lane >= off) x += y;
-  }
-  return x;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
-  return x;
-}
-
-// |A ∩ B| for sorted, duplicate-free lists: merge with early exit once one
-// cursor passes the other list's last element.
-__device__ __forceinline__ uint32_t merge_count(const uint32_t* __restrict__ A, uint32_t na,
-                                                const uint32_t* __restrict__ B, uint32_t nb) {
-  if (na == 0 || nb == 0) return 0;
-  const uint32_t alast = A[na - 1], blast = B[nb - 1];
-  uint32_t i = 0, j = 0, c = 0;
-  uint32_t x = A[0], y = B[0];
-  for (;;) {
-    if (x == y) {
-      ++c;
-      if (++i == na || ++j == nb) break;
-      x = A[i];
-      y = B[j];
-    } else if (x < y) {
-      if (y > alast || ++i == na) break;
-      x = A[i];
-    } else {
-      if (x > blast || ++j == nb) break;
-      y = B[j];
-    }
-  }
-  return c;
-}
-
-// Task whose item range holds `item`: largest t with prefix[t] <= item.
-__device__ __forceinline__ uint32_t find_task(const uint64_t* __restrict__ prefix, uint32_t n,
-                                              uint64_t item) {
-  uint32_t lo = 0, hi = n;  // prefix[lo] <= item < prefix[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (prefix[mid] <= item) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// ---------------------------------------------------------------------------
-// kernels
-// ---------------------------------------------------------------------------
-__global__ void zero_counts(unsigned long long* __restrict__ p, uint64_t n,
-                            unsigned long long* __restrict__ head) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-  if (i == 0) *head = 0;
-}
-
-__global__ __launch_bounds__(kBlock) void count_merge(CountArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  uint32_t t_cur = 0;
-  uint64_t t_lo = 1, t_hi = 0;  // cached item range of t_cur (empty at start)
-  for (;;) {
-    unsigned long long item = 0;
-    if (lane == 0) item = atomicAdd(a.head, 1ull);
-    item = __shfl(item, 0, kWave);
-    if (item >= a.nitems) break;
-    if (!(item >= t_lo && item < t_hi)) {
-      t_cur = find_task(a.item_prefix, a.ntasks, item);
-      t_lo = a.item_prefix[t_cur];
-      t_hi = a.item_prefix[t_cur + 1];
-    }
-    const TaskDesc T = a.tasks[t_cur];
-    const TileDesc AB = a.tiles[T.ab];
-    const TileDesc BC = a.tiles[T.bc];
-    const TileDesc AC = a.tiles[T.ac];
-    const uint32_t* __restrict__ ab_rp = a.rowptr + AB.rp;
-    const uint32_t* __restrict__ ac_rp = a.rowptr + AC.rp;
-    const uint32_t* __restrict__ bc_rp = a.rowptr + BC.rp;
-    const uint32_t* __restrict__ ab_c = a.cols + AB.col;
-    const uint32_t* __restrict__ ac_c = a.cols + AC.col;
-    const uint32_t* __restrict__ bc_c = a.cols + BC.col;
-
-    const uint64_t z = (item - t_lo) * kRowsPerItem + lane;
-    const bool valid = z < AB.nzr;
-    uint32_t ab0 = 0, ac0 = 0, ac1 = 0, dab = 0;
-    if (valid) {
-      const uint32_t r = a.nzrows[AB.nz + z];
-      ab0 = ab_rp[r];
-      const uint32_t ab1 = ab_rp[r + 1];
-      ac0 = ac_rp[r];
-      ac1 = ac_rp[r + 1];
-      dab = (ac1 > ac0) ? ab1 - ab0 : 0u;
-    }
-    const uint32_t incl = wave_incl_scan(dab, lane);
-    const uint32_t excl = incl - dab;
-    const uint32_t W = __shfl(incl, kWave - 1, kWave);
-
-    unsigned long long cnt = 0;
-    for (uint32_t base = 0; base < W; base += kWave) {
-      const uint32_t w = base + lane;
-      // owner lane: number of lanes whose inclusive prefix is <= w
-      int q = 0;
-#pragma unroll
-      for (int s = kWave / 2; s >= 1; s >>= 1) {
-        const uint32_t pv = __shfl(incl, q + s - 1, kWave);
-        if (pv <= w) q += s;
-      }
-      q = q > kWave - 1 ? kWave - 1 : q;
-      const uint32_t q_ab0 = __shfl(ab0, q, kWave);
-      const uint32_t q_excl = __shfl(excl, q, kWave);
-      const uint32_t q_ac0 = __shfl(ac0, q, kWave);
-      const uint32_t q_ac1 = __shfl(ac1, q, kWave);
-      if (w < W) {
-        const uint32_t v = ab_c[q_ab0 + (w - q_excl)];
-        const uint32_t b0 = bc_rp[v], b1 = bc_rp[v + 1];
-        cnt += merge_count(ac_c + q_ac0, q_ac1 - q_ac0, bc_c + b0, b1 - b0);
-      }
-    }
-    cnt = wave_sum64(cnt);
-    if (lane == 0 && cnt) atomicAdd(a.per_task + t_cur, cnt);
-  }
-}
-
 }  // namespace bbtc
 
 // ---------------------------------------------------------------------------
@@ -204,6 +34,7 @@ __global__ __launch_bounds__(kBlock) void count_merge(CountArgs a) {
 struct bbtc_ctx {
   int device = 0;
   int num_cu = 0;
+  int blocks_per_cu = 1;  // resident blocks of count_probe per CU
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
   uint64_t in_use = 0;
@@ -300,10 +131,10 @@ void launch(bbtc_plan* p, hipStream_t s) {
   a.nitems = p->nitems;
   a.per_task = p->per_task;
   a.head = p->head;
-  const int waves_per_block = kBlock / kWave;
-  const uint64_t want_blocks = (p->nitems + waves_per_block - 1) / waves_per_block;
-  const unsigned blocks = unsigned(std::min<uint64_t>(want_blocks, uint64_t(p->ctx->num_cu) * 8));
-  hipLaunchKernelGGL(count_merge, dim3(blocks), dim3(kBlock), 0, s, a);
+  const uint64_t want_blocks = (p->nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint64_t resident = uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu);
+  const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident)));
+  hipLaunchKernelGGL(count_probe, dim3(blocks), dim3(kBlock), 0, s, a);
   HIP_OK(hipGetLastError());
   p->last_stream = s;
 }
@@ -351,6 +182,8 @@ bbtc_status bbtc_ctx_create(int device, const bbtc_opts* opts, bbtc_ctx** out) {
   try {
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->blocks_per_cu, bbtc::count_probe, bbtc::kBlock, 0));
+    if (c->blocks_per_cu < 1) c->blocks_per_cu = 1;
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   } catch (...) {
     delete c;
@@ -440,15 +273,27 @@ bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tas
     std::vector<uint64_t> pref(n + 1, 0);
     for (uint64_t q = 0; q < n; ++q) {
       const bbtc_task& x = tasks[q];
-      bbtc::TaskDesc d;
+      bbtc::TaskDesc d{};
       d.ab = x.i * P + x.j;
       d.bc = x.j * P + x.k;
       d.ac = x.i * P + x.k;
-      d.kind = 0;
+      // LDS form of ac(u): a bitmap over band k needs ceil(width/32) words per
+      // row; the wave slice holds kLdsWords words.
+      const uint32_t width = g->htiles[x.k * P + x.k].h;
+      const uint32_t wpr = (width + 31) / 32;
+      const uint32_t fit = wpr ? uint32_t(bbtc::kLdsWords) / wpr : uint32_t(bbtc::kWave);
+      bool use_bitmap = false;
+      if (kernel == BBTC_KERNEL_BITMAP) use_bitmap = fit >= 1;
+      else if (kernel == BBTC_KERNEL_AUTO) use_bitmap = fit >= 32;
+      d.kind = use_bitmap ? bbtc::kKindBitmap : bbtc::kKindSearch;
+      d.rows = use_bitmap ? std::min<uint32_t>(fit, bbtc::kWave) : uint32_t(bbtc::kWave);
+      d.wpr = use_bitmap ? wpr : 0;
       td[q] = d;
       uint64_t items = 0;
-      if (!bbtc::skipped(g, d.ab, d.bc, d.ac))
-        items = (uint64_t(g->htiles[d.ab].nzr) + bbtc::kRowsPerItem - 1) / bbtc::kRowsPerItem;
+      if (!bbtc::skipped(g, d.ab, d.bc, d.ac)) {
+        items = (uint64_t(g->htiles[d.ab].nzr) + d.rows - 1) / d.rows;
+        if (use_bitmap) p->bitmap_tasks++;
+      }
       pref[q + 1] = pref[q] + items;
     }
     p->nitems = pref[n];

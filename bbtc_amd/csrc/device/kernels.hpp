@@ -1,0 +1,251 @@
+// Device code: task descriptors and the counting kernels for gfx950.
+//
+// Work decomposition. A plan lists work items; item x of task t covers
+// rows [x*R, (x+1)*R) of the ab tile's compact non-empty row list (R <= 64,
+// per task). One wave owns one item at a time and draws the next from an
+// atomic head counter, so a single persistent launch drains every task.
+//
+// Inside an item the wave flattens its work twice with wave prefix sums:
+//   rows  -> wedges (u, v), v in ab(u), for rows whose ac(u) is non-empty
+//   wedges -> probes (u, v, w), w in bc(v)
+// so each lane handles one probe per step, lanes read bc(v) lists
+// contiguously, and a hub row's wedges spread over all 64 lanes instead of
+// serialising one thread as the reference kernels do
+// (gpu_sparse_kernel.cu:25-46, gpu_dense_kernel.cu:74-93).
+//
+// Membership of w in ac(u) is tested against ac(u) staged in the wave's LDS
+// slice in one of two forms, chosen per task:
+//   search (d): the sorted list itself; a probe is a binary search.
+//   bitmap (c): one bit per column of band k per row; a probe is one bit test.
+// Matches are counted with ballot + popcount into a wave-uniform 64-bit count
+// and added to the task's slot with one atomic per item.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bbtc {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kLdsWords = 2048;  // 8 KiB of LDS per wave
+constexpr uint32_t kKindSearch = 0;
+constexpr uint32_t kKindBitmap = 1;
+
+struct TileDesc {
+  uint64_t rp;   // offset of the tile's row offsets in rowptr
+  uint64_t col;  // offset of the tile's columns in cols
+  uint64_t nz;   // offset of the tile's non-empty row list in nzrows
+  uint32_t nzr;  // number of non-empty rows
+  uint32_t h;    // band height
+};
+static_assert(sizeof(TileDesc) == 32, "TileDesc layout");
+
+struct TaskDesc {
+  uint32_t ab, bc, ac;  // tile ids i*P+j, j*P+k, i*P+k
+  uint32_t kind;        // kKindSearch or kKindBitmap
+  uint32_t rows;        // rows of the ab non-empty list per item (1..64)
+  uint32_t wpr;         // bitmap words per row (bitmap kind)
+  uint32_t pad0, pad1;
+};
+static_assert(sizeof(TaskDesc) == 32, "TaskDesc layout");
+
+struct CountArgs {
+  const uint32_t* __restrict__ rowptr;
+  const uint32_t* __restrict__ cols;
+  const uint32_t* __restrict__ nzrows;
+  const TileDesc* __restrict__ tiles;
+  const TaskDesc* __restrict__ tasks;
+  const uint64_t* __restrict__ item_prefix;  // ntasks + 1 entries
+  uint32_t ntasks;
+  uint64_t nitems;
+  unsigned long long* per_task;
+  unsigned long long* head;
+};
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Lane owning flattened index x: the number of lanes whose inclusive prefix
+// is <= x (prefixes never decrease across lanes), clamped to the last lane.
+__device__ __forceinline__ int owner_of(uint32_t incl, uint32_t x) {
+  int q = 0;
+#pragma unroll
+  for (int s = kWave / 2; s >= 1; s >>= 1) {
+    const uint32_t pv = __shfl(incl, q + s - 1, kWave);
+    if (pv <= x) q += s;
+  }
+  return q > kWave - 1 ? kWave - 1 : q;
+}
+
+__device__ __forceinline__ bool sorted_contains(const uint32_t* s, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, len = n;
+  while (len) {
+    const uint32_t half = len >> 1;
+    if (s[lo + half] < x) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo < n && s[lo] == x;
+}
+
+// Largest t with prefix[t] <= item (prefix[n] > item).
+__device__ __forceinline__ uint32_t find_task(const uint64_t* __restrict__ prefix, uint32_t n,
+                                              uint64_t item) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void zero_counts(unsigned long long* __restrict__ p, uint64_t n,
+                            unsigned long long* __restrict__ head) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+  if (i == 0) *head = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
+  __shared__ uint32_t lds_all[kWavesPerBlock * kLdsWords];
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* lds = lds_all + (threadIdx.x / kWave) * kLdsWords;
+  // bitmap slices are all-zero between items
+  for (int x = lane; x < kLdsWords; x += kWave) lds[x] = 0;
+  wave_lds_sync();
+
+  uint32_t t_cur = 0;
+  uint64_t t_lo = 1, t_hi = 0;  // item range of t_cur (empty at start)
+  for (;;) {
+    unsigned long long item = 0;
+    if (lane == 0) item = atomicAdd(a.head, 1ull);
+    item = __shfl(item, 0, kWave);
+    if (item >= a.nitems) break;
+    if (!(item >= t_lo && item < t_hi)) {
+      t_cur = find_task(a.item_prefix, a.ntasks, item);
+      t_lo = a.item_prefix[t_cur];
+      t_hi = a.item_prefix[t_cur + 1];
+    }
+    const TaskDesc T = a.tasks[t_cur];
+    const TileDesc AB = a.tiles[T.ab];
+    const TileDesc BC = a.tiles[T.bc];
+    const TileDesc AC = a.tiles[T.ac];
+    const uint32_t* __restrict__ ab_rp = a.rowptr + AB.rp;
+    const uint32_t* __restrict__ ac_rp = a.rowptr + AC.rp;
+    const uint32_t* __restrict__ bc_rp = a.rowptr + BC.rp;
+    const uint32_t* __restrict__ ab_c = a.cols + AB.col;
+    const uint32_t* __restrict__ ac_c = a.cols + AC.col;
+    const uint32_t* __restrict__ bc_c = a.cols + BC.col;
+    const bool bitmap = T.kind == kKindBitmap;
+
+    // one row per lane
+    const uint64_t z = (item - t_lo) * T.rows + uint64_t(lane);
+    uint32_t ab0 = 0, dab = 0, ac0 = 0, dac = 0;
+    if (uint32_t(lane) < T.rows && z < AB.nzr) {
+      const uint32_t r = a.nzrows[AB.nz + z];
+      ab0 = ab_rp[r];
+      dab = ab_rp[r + 1] - ab0;
+      ac0 = ac_rp[r];
+      dac = ac_rp[r + 1] - ac0;
+      if (dac == 0) dab = 0;
+      if (dab == 0) dac = 0;
+    }
+
+    // stage ac(u) of the rows that have wedges
+    const uint32_t inclA = wave_incl_scan(dac, lane);
+    const uint32_t exclA = inclA - dac;
+    const uint32_t A = __shfl(inclA, kWave - 1, kWave);
+    const bool in_lds = bitmap || A <= uint32_t(kLdsWords);
+    if (in_lds) {
+      for (uint32_t s = 0; s < A; s += kWave) {
+        const uint32_t e = s + lane;
+        const int q = owner_of(inclA, e);
+        const uint32_t q_ac0 = __shfl(ac0, q, kWave);
+        const uint32_t q_ex = __shfl(exclA, q, kWave);
+        if (e < A) {
+          const uint32_t x = ac_c[q_ac0 + (e - q_ex)];
+          if (bitmap) atomicOr(&lds[uint32_t(q) * T.wpr + (x >> 5)], 1u << (x & 31));
+          else lds[e] = x;
+        }
+      }
+    }
+    wave_lds_sync();
+    // where row `lane` keeps its membership structure
+    const uint32_t seg = bitmap ? uint32_t(lane) * T.wpr : (in_lds ? exclA : ac0);
+    const uint32_t* __restrict__ search_base = in_lds ? lds : ac_c;
+
+    // rows -> wedges -> probes
+    const uint32_t inclW = wave_incl_scan(dab, lane);
+    const uint32_t exclW = inclW - dab;
+    const uint32_t W = __shfl(inclW, kWave - 1, kWave);
+    unsigned long long cnt = 0;
+    for (uint32_t wb = 0; wb < W; wb += kWave) {
+      const uint32_t w = wb + lane;
+      const int q = owner_of(inclW, w);
+      const uint32_t q_ab0 = __shfl(ab0, q, kWave);
+      const uint32_t q_ex = __shfl(exclW, q, kWave);
+      const uint32_t q_seg = __shfl(seg, q, kWave);
+      const uint32_t q_dac = __shfl(dac, q, kWave);
+      uint32_t b0 = 0, dbc = 0;
+      if (w < W) {
+        const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
+        b0 = bc_rp[v];
+        dbc = bc_rp[v + 1] - b0;
+      }
+      const uint32_t inclP = wave_incl_scan(dbc, lane);
+      const uint32_t exclP = inclP - dbc;
+      const uint32_t Pn = __shfl(inclP, kWave - 1, kWave);
+      for (uint32_t pb = 0; pb < Pn; pb += kWave) {
+        const uint32_t p = pb + lane;
+        const int q2 = owner_of(inclP, p);
+        const uint32_t p_b0 = __shfl(b0, q2, kWave);
+        const uint32_t p_ex = __shfl(exclP, q2, kWave);
+        const uint32_t p_seg = __shfl(q_seg, q2, kWave);
+        const uint32_t p_dac = __shfl(q_dac, q2, kWave);
+        bool found = false;
+        if (p < Pn) {
+          const uint32_t x = bc_c[p_b0 + (p - p_ex)];
+          if (bitmap) found = (lds[p_seg + (x >> 5)] >> (x & 31)) & 1u;
+          else found = sorted_contains(search_base + p_seg, p_dac, x);
+        }
+        cnt += __popcll(__ballot(found));
+      }
+    }
+
+    if (bitmap) {  // restore the all-zero slice
+      wave_lds_sync();
+      for (uint32_t s = 0; s < A; s += kWave) {
+        const uint32_t e = s + lane;
+        const int q = owner_of(inclA, e);
+        const uint32_t q_ac0 = __shfl(ac0, q, kWave);
+        const uint32_t q_ex = __shfl(exclA, q, kWave);
+        if (e < A) {
+          const uint32_t x = ac_c[q_ac0 + (e - q_ex)];
+          lds[uint32_t(q) * T.wpr + (x >> 5)] = 0;
+        }
+      }
+    }
+    wave_lds_sync();
+    if (lane == 0 && cnt) atomicAdd(a.per_task + t_cur, cnt);
+  }
+}
+
+}  // namespace bbtc

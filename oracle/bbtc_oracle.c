@@ -36,7 +36,10 @@
  *     std::sort's order for ties is unspecified;
  *   - the per-worker stamp table of a dense task is as wide as the ac band
  *     (the reference sizes it by the widest dense tile, cpu_dense_kernel.cpp
- *     53-59, and would write past it for a wider ac band).
+ *     53-59, and would write past it for a wider ac band);
+ *   - rows_ has one more word than (N+P)*P and the prefix sum covers it: with
+ *     P = 1 the reference reads that word past its array (block_csr.cpp:292,
+ *     348-351); its CLI refuses --ncut 1 (utils.hpp:122-125).
  */
 #include <math.h>
 #include <omp.h>
@@ -428,7 +431,8 @@ ORC_API orc_bcsr* orc_bcsr_build(const uint64_t* off, const uint32_t* col, uint3
                          __ATOMIC_RELAXED);
     }
   }
-  for (uint64_t i = 1; i < nrpart; ++i) B->rows[i] += B->rows[i - 1];
+  /* the prefix also covers the one extra word, which only P = 1 reaches */
+  for (uint64_t i = 1; i <= nrpart; ++i) B->rows[i] += B->rows[i - 1];
 #pragma omp parallel for schedule(dynamic, 2048)
   for (int64_t u = 0; u < (int64_t)n; ++u) {
     uint32_t x = 0, y = 0;
