@@ -27,7 +27,7 @@ CLI_PATH = os.path.join(_HERE, "bbtc")
 
 PART = {"uni": 0, "n1d": 1, "pal": 2}
 ORDER = {"lex": 0, "desc": 1, "asc": 2}
-KERNEL = {"auto": 0, "merge": 1, "bitmap": 2}
+KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3}
 
 
 class BbtcError(RuntimeError):
@@ -82,19 +82,24 @@ def _declare(lib):
         "bbtc_tasks_build": (st, [vp, i32, vp, u64, C.POINTER(u64)]),
         "bbtc_tasks_bytes": (st, [vp, vp, u64, u32, vp, C.POINTER(u64), C.POINTER(u64)]),
         "bbtc_count_cpu": (st, [vp, vp, u64, i32, vp, C.POINTER(u64)]),
+        "bbtc_device_count": (st, [C.POINTER(i32)]),
         "bbtc_ctx_create": (st, [i32, C.POINTER(bbtc_opts), pvp]),
         "bbtc_ctx_destroy": (None, [vp]),
         "bbtc_ctx_bytes_in_use": (u64, [vp]),
+        "bbtc_ctx_synchronize": (st, [vp]),
         "bbtc_graph_upload": (st, [vp, vp, pvp]),
         "bbtc_graph_bytes": (u64, [vp]),
         "bbtc_graph_release": (None, [vp]),
         "bbtc_plan_create": (st, [vp, vp, vp, u64, i32, pvp]),
+        "bbtc_plan_create_tuned": (st, [vp, vp, vp, u64, i32, vp, pvp]),
         "bbtc_plan_launch": (st, [vp, vp]),
         "bbtc_plan_results": (st, [vp, vp, C.POINTER(u64)]),
         "bbtc_plan_time": (st, [vp, i32, vp]),
         "bbtc_plan_bitmap_tasks": (u64, [vp]),
         "bbtc_plan_destroy": (None, [vp]),
         "bbtc_count": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
+        "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, C.POINTER(u64),
+                                   C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -270,6 +275,12 @@ class Layout:
             self._h = None
 
 
+def device_count() -> int:
+    n = C.c_int()
+    _check(lib().bbtc_device_count(C.byref(n)))
+    return n.value
+
+
 class Device:
     """A bbtc context on one GPU (stream, counters, memory accounting)."""
 
@@ -282,6 +293,9 @@ class Device:
     @property
     def bytes_in_use(self) -> int:
         return lib().bbtc_ctx_bytes_in_use(self._h)
+
+    def synchronize(self):
+        _check(lib().bbtc_ctx_synchronize(self._h))
 
     def upload(self, layout: Layout) -> "DeviceGraph":
         return DeviceGraph(self, layout)
@@ -307,8 +321,8 @@ class DeviceGraph:
     def bytes(self) -> int:
         return lib().bbtc_graph_bytes(self._h)
 
-    def plan(self, tasks, kernel: str = "auto") -> "Plan":
-        return Plan(self, tasks, kernel)
+    def plan(self, tasks, kernel: str = "auto", choice=None) -> "Plan":
+        return Plan(self, tasks, kernel, choice)
 
     def count(self, tasks, kernel: str = "auto"):
         t = tasks_array(tasks)
@@ -317,6 +331,16 @@ class DeviceGraph:
         _check(lib().bbtc_count(self.dev._h, self._h, _ptr(t), len(t), KERNEL[kernel], _ptr(per),
                                 C.byref(tot), None))
         return per, tot.value
+
+    def count_hybrid(self, tasks, separator, nthreads=0, nstreams=1, kernel="auto"):
+        """Device from the front, host workers from the back (--border)."""
+        t = tasks_array(tasks)
+        per = np.zeros(len(t), dtype=np.uint64)
+        tot, gt = C.c_uint64(), C.c_uint64()
+        _check(lib().bbtc_count_hybrid(self.dev._h, self._h, self.layout._h, _ptr(t), len(t), separator,
+                                       nthreads, nstreams, KERNEL[kernel], _ptr(per), C.byref(tot),
+                                       C.byref(gt)))
+        return per, tot.value, gt.value
 
     def release(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -328,12 +352,17 @@ class DeviceGraph:
 
 
 class Plan:
-    def __init__(self, g: DeviceGraph, tasks, kernel: str = "auto"):
+    def __init__(self, g: DeviceGraph, tasks, kernel: str = "auto", choice=None):
         self.g = g
         self.tasks = tasks_array(tasks)
         h = C.c_void_p()
-        _check(lib().bbtc_plan_create(g.dev._h, g._h, _ptr(self.tasks), len(self.tasks),
-                                      KERNEL[kernel], C.byref(h)))
+        if choice is None:
+            _check(lib().bbtc_plan_create(g.dev._h, g._h, _ptr(self.tasks), len(self.tasks),
+                                          KERNEL[kernel], C.byref(h)))
+        else:
+            self.choice = np.ascontiguousarray(choice, dtype=np.uint8)
+            _check(lib().bbtc_plan_create_tuned(g.dev._h, g._h, _ptr(self.tasks), len(self.tasks),
+                                                KERNEL[kernel], _ptr(self.choice), C.byref(h)))
         self._h = h
 
     @property

@@ -16,6 +16,7 @@
 
 #include "../host/handles.hpp"
 #include "kernels.hpp"
+#include "../host/device_api.hpp"
 
 namespace bbtc {
 
@@ -35,7 +36,8 @@ struct bbtc_ctx {
   int device = 0;
   int num_cu = 0;
   int blocks_per_cu = 1;  // resident blocks of count_probe per CU
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;          // slot 0
+  std::vector<hipStream_t> slots;         // slot i > 0, created on first use
   uint64_t budget = 0;
   uint64_t in_use = 0;
 };
@@ -47,9 +49,11 @@ struct bbtc_dgraph {
   uint32_t* rowptr = nullptr;
   uint32_t* cols = nullptr;
   uint32_t* nzrows = nullptr;
+  uint32_t* dense = nullptr;
   bbtc::TileDesc* tiles = nullptr;
   std::vector<bbtc::TileDesc> htiles;
   std::vector<float> deg;
+  std::vector<uint32_t> dwpr;  // bitmap words per row of each band
 };
 
 struct bbtc_plan {
@@ -61,8 +65,9 @@ struct bbtc_plan {
   uint64_t bytes = 0;
   bbtc::TaskDesc* tasks = nullptr;
   uint64_t* item_prefix = nullptr;
+  std::vector<uint64_t> hprefix;        // host copy of item_prefix
   unsigned long long* per_task = nullptr;
-  unsigned long long* head = nullptr;
+  unsigned long long* heads = nullptr;  // one work counter per launch slot
   hipStream_t last_stream = nullptr;
 };
 
@@ -94,6 +99,7 @@ void release_graph(bbtc_dgraph* g) {
   dev_free(g->ctx, g->rowptr);
   dev_free(g->ctx, g->cols);
   dev_free(g->ctx, g->nzrows);
+  dev_free(g->ctx, g->dense);
   dev_free(g->ctx, g->tiles);
   if (g->ctx) g->ctx->in_use -= std::min(g->ctx->in_use, g->bytes);
   delete g;
@@ -104,7 +110,7 @@ void release_plan(bbtc_plan* p) {
   dev_free(p->ctx, p->tasks);
   dev_free(p->ctx, p->item_prefix);
   dev_free(p->ctx, p->per_task);
-  dev_free(p->ctx, p->head);
+  dev_free(p->ctx, p->heads);
   if (p->ctx) p->ctx->in_use -= std::min(p->ctx->in_use, p->bytes);
   delete p;
 }
@@ -113,33 +119,101 @@ bool skipped(const bbtc_dgraph* g, uint32_t ab, uint32_t bc, uint32_t ac) {
   return g->deg[bc] == 0 || g->deg[ac] == 0 || g->deg[ab] == 0;
 }
 
-void launch(bbtc_plan* p, hipStream_t s) {
-  bbtc_dgraph* g = p->g;
+void zero(bbtc_plan* p, hipStream_t s) {
   const uint64_t zn = std::max<uint64_t>(p->ntasks, 1);
   hipLaunchKernelGGL(zero_counts, dim3(unsigned((zn + 255) / 256)), dim3(256), 0, s, p->per_task,
-                     p->ntasks, p->head);
+                     p->ntasks);
   HIP_OK(hipGetLastError());
-  if (p->nitems == 0) return;
+}
+
+// Counts tasks [t0, t1) of the plan on stream s with work counter `slot`.
+// Launches on one slot must be ordered (same stream); distinct slots may run
+// concurrently.
+void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t s) {
+  bbtc_dgraph* g = p->g;
+  const uint64_t i0 = p->hprefix[t0], i1 = p->hprefix[t1];
+  p->last_stream = s;
+  if (i1 <= i0) return;
+  unsigned long long* head = p->heads + slot;
+  hipLaunchKernelGGL(set_head, dim3(1), dim3(1), 0, s, head, (unsigned long long)i0);
+  HIP_OK(hipGetLastError());
   CountArgs a;
   a.rowptr = g->rowptr;
   a.cols = g->cols;
   a.nzrows = g->nzrows;
+  a.dense = g->dense;
   a.tiles = g->tiles;
   a.tasks = p->tasks;
   a.item_prefix = p->item_prefix;
   a.ntasks = uint32_t(p->ntasks);
-  a.nitems = p->nitems;
+  a.item_end = i1;
   a.per_task = p->per_task;
-  a.head = p->head;
-  const uint64_t want_blocks = (p->nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+  a.head = head;
+  const uint64_t want_blocks = (i1 - i0 + kWavesPerBlock - 1) / kWavesPerBlock;
   const uint64_t resident = uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu);
   const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident)));
   hipLaunchKernelGGL(count_probe, dim3(blocks), dim3(kBlock), 0, s, a);
   HIP_OK(hipGetLastError());
-  p->last_stream = s;
+}
+
+void launch(bbtc_plan* p, hipStream_t s) {
+  zero(p, s);
+  launch_range(p, 0, p->ntasks, 0, s);
 }
 
 }  // namespace
+}  // namespace bbtc
+
+namespace bbtc {
+namespace dev {
+
+namespace {
+hipStream_t slot_stream(bbtc_ctx* c, int slot) {
+  if (slot <= 0) return c->stream;
+  if (slot >= kMaxSlots) fail(BBTC_ERR_INVALID, "stream slot out of range");
+  if (c->slots.size() < size_t(slot) + 1) c->slots.resize(size_t(slot) + 1, nullptr);
+  if (!c->slots[slot]) HIP_OK(hipStreamCreateWithFlags(&c->slots[slot], hipStreamNonBlocking));
+  return c->slots[slot];
+}
+}  // namespace
+
+int max_slots() { return kMaxSlots; }
+
+void plan_zero_sync(bbtc_plan* p) {
+  HIP_OK(hipSetDevice(p->ctx->device));
+  zero(p, p->ctx->stream);
+  HIP_OK(hipStreamSynchronize(p->ctx->stream));
+}
+
+void plan_launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot) {
+  if (t0 > t1 || t1 > p->ntasks) fail(BBTC_ERR_INVALID, "task range out of bounds");
+  HIP_OK(hipSetDevice(p->ctx->device));
+  launch_range(p, t0, t1, slot, slot_stream(p->ctx, slot));
+}
+
+uint64_t plan_items(const bbtc_plan* p, uint64_t t0, uint64_t t1) { return p->hprefix[t1] - p->hprefix[t0]; }
+
+uint64_t plan_batch_items(const bbtc_plan* p) {
+  return uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu) * kWavesPerBlock * 4;
+}
+
+bool plan_slot_idle(bbtc_plan* p, int slot) {
+  const hipError_t e = hipStreamQuery(slot_stream(p->ctx, slot));
+  if (e == hipSuccess) return true;
+  if (e == hipErrorNotReady) return false;
+  HIP_OK(e);
+  return false;
+}
+
+void plan_slot_wait(bbtc_plan* p, int slot) { HIP_OK(hipStreamSynchronize(slot_stream(p->ctx, slot))); }
+
+void plan_copy_counts(bbtc_plan* p, uint64_t* out) {
+  HIP_OK(hipSetDevice(p->ctx->device));
+  HIP_OK(hipDeviceSynchronize());
+  if (p->ntasks) HIP_OK(hipMemcpy(out, p->per_task, p->ntasks * 8, hipMemcpyDeviceToHost));
+}
+
+}  // namespace dev
 }  // namespace bbtc
 
 #define DEV_TRY try {
@@ -168,6 +242,14 @@ void launch(bbtc_plan* p, hipStream_t s) {
   } while (0)
 
 extern "C" {
+
+bbtc_status bbtc_device_count(int* n) {
+  DEV_REQUIRE(n, "null argument");
+  *n = 0;
+  int c = 0;
+  if (hipGetDeviceCount(&c) == hipSuccess) *n = c;
+  return BBTC_OK;
+}
 
 bbtc_status bbtc_ctx_create(int device, const bbtc_opts* opts, bbtc_ctx** out) {
   DEV_REQUIRE(out, "null argument");
@@ -200,10 +282,23 @@ void bbtc_ctx_destroy(bbtc_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  for (hipStream_t s : ctx->slots) {
+    if (!s) continue;
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
   delete ctx;
 }
 
 uint64_t bbtc_ctx_bytes_in_use(const bbtc_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+bbtc_status bbtc_ctx_synchronize(bbtc_ctx* ctx) {
+  DEV_REQUIRE(ctx, "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipDeviceSynchronize());
+  DEV_CATCH
+}
 
 bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph** out) {
   DEV_REQUIRE(ctx && Lh && out, "null argument");
@@ -219,22 +314,28 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
       bbtc::TileDesc d{};
       const uint32_t i = uint32_t(t / L.P), k = uint32_t(t % L.P);
       d.h = L.height(i);
+      d.dense = ~0ull;
+      d.dwpr = L.dwpr[k];
       if (i <= k) {
         d.rp = L.rp_base[t];
         d.col = L.col_base[t];
         d.nz = L.nz_base[t];
         d.nzr = L.nz_rows[t];
+        d.dense = L.dense_base[t];
       }
       g->htiles[t] = d;
     }
     g->deg = L.deg;
+    g->dwpr = L.dwpr;
     g->rowptr = bbtc::dev_alloc<uint32_t>(ctx, L.rowptr.size(), &g->bytes);
     g->cols = bbtc::dev_alloc<uint32_t>(ctx, L.cols.size(), &g->bytes);
     g->nzrows = bbtc::dev_alloc<uint32_t>(ctx, L.nzrows.size(), &g->bytes);
+    g->dense = bbtc::dev_alloc<uint32_t>(ctx, L.dense.size(), &g->bytes);
     g->tiles = bbtc::dev_alloc<bbtc::TileDesc>(ctx, T, &g->bytes);
     HIP_OK(hipMemcpyAsync(g->rowptr, L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(g->cols, L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(g->nzrows, L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(g->dense, L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(g->tiles, g->htiles.data(), T * sizeof(bbtc::TileDesc), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
   } catch (...) {
@@ -254,10 +355,16 @@ void bbtc_graph_release(bbtc_dgraph* g) {
 
 bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                              bbtc_kernel kernel, bbtc_plan** out) {
+  return bbtc_plan_create_tuned(ctx, g, tasks, n, kernel, nullptr, out);
+}
+
+bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                                   bbtc_kernel kernel, const uint8_t* choice, bbtc_plan** out) {
   DEV_REQUIRE(ctx && g && out && (tasks || n == 0), "null argument");
   DEV_REQUIRE(g->ctx == ctx, "graph belongs to another context");
   DEV_REQUIRE(n < 0xFFFFFFFFull, "too many tasks");
-  DEV_REQUIRE(kernel == BBTC_KERNEL_AUTO || kernel == BBTC_KERNEL_MERGE || kernel == BBTC_KERNEL_BITMAP,
+  DEV_REQUIRE(kernel == BBTC_KERNEL_AUTO || kernel == BBTC_KERNEL_MERGE || kernel == BBTC_KERNEL_BITMAP ||
+                  kernel == BBTC_KERNEL_DENSE,
               "unknown kernel choice");
   for (uint64_t q = 0; q < n; ++q)
     DEV_REQUIRE(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < g->P,
@@ -277,15 +384,27 @@ bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tas
       d.ab = x.i * P + x.j;
       d.bc = x.j * P + x.k;
       d.ac = x.i * P + x.k;
-      // LDS form of ac(u): a bitmap over band k needs ceil(width/32) words per
-      // row; the wave slice holds kLdsWords words.
-      const uint32_t width = g->htiles[x.k * P + x.k].h;
-      const uint32_t wpr = (width + 31) / 32;
+      // LDS form of ac(u): a bitmap over band k takes dwpr words per row; the
+      // wave slice holds kLdsWords words.
+      const uint32_t wpr = g->dwpr[x.k];
       const uint32_t fit = wpr ? uint32_t(bbtc::kLdsWords) / wpr : uint32_t(bbtc::kWave);
-      bool use_bitmap = false;
-      if (kernel == BBTC_KERNEL_BITMAP) use_bitmap = fit >= 1;
-      else if (kernel == BBTC_KERNEL_AUTO) use_bitmap = fit >= 32;
-      d.kind = use_bitmap ? bbtc::kKindBitmap : bbtc::kKindSearch;
+      const bool has_dense = g->htiles[d.bc].dense != ~0ull;
+      bbtc_kernel k = kernel;
+      if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
+      else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
+      else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
+      uint32_t kind = bbtc::kKindSearch;
+      if (k == BBTC_KERNEL_BITMAP) {
+        if (fit >= 1) kind = bbtc::kKindBitmap;
+      } else if (k == BBTC_KERNEL_DENSE) {
+        if (has_dense && fit >= 1) kind = bbtc::kKindDense;
+        else if (fit >= 1) kind = bbtc::kKindBitmap;
+      } else if (k == BBTC_KERNEL_AUTO) {
+        if (has_dense && fit >= 1) kind = bbtc::kKindDense;
+        else if (fit >= uint32_t(bbtc::kAutoBitmapRows)) kind = bbtc::kKindBitmap;
+      }
+      const bool use_bitmap = kind != bbtc::kKindSearch;
+      d.kind = kind;
       d.rows = use_bitmap ? std::min<uint32_t>(fit, bbtc::kWave) : uint32_t(bbtc::kWave);
       d.wpr = use_bitmap ? wpr : 0;
       td[q] = d;
@@ -297,11 +416,12 @@ bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tas
       pref[q + 1] = pref[q] + items;
     }
     p->nitems = pref[n];
+    p->hprefix = pref;
     HIP_OK(hipSetDevice(ctx->device));
     p->tasks = bbtc::dev_alloc<bbtc::TaskDesc>(ctx, td.size(), &p->bytes);
     p->item_prefix = bbtc::dev_alloc<uint64_t>(ctx, pref.size(), &p->bytes);
     p->per_task = bbtc::dev_alloc<unsigned long long>(ctx, std::max<uint64_t>(n, 1), &p->bytes);
-    p->head = bbtc::dev_alloc<unsigned long long>(ctx, 1, &p->bytes);
+    p->heads = bbtc::dev_alloc<unsigned long long>(ctx, bbtc::kMaxSlots, &p->bytes);
     HIP_OK(hipMemcpyAsync(p->tasks, td.data(), td.size() * sizeof(bbtc::TaskDesc), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(p->item_prefix, pref.data(), pref.size() * 8, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));

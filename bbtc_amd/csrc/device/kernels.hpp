@@ -30,24 +30,34 @@ namespace bbtc {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kLdsWords = 2048;  // 8 KiB of LDS per wave
+constexpr int kLdsWords = 1792;  // bitmap / sorted-list slice of a wave
+constexpr int kMarkWords = 4 * 64;  // per-step wedge marks of a wave
+constexpr int kWaveLds = kLdsWords + kMarkWords;  // 8 KiB of LDS per wave
+constexpr int kMaxSlots = 64;    // concurrent range launches of one plan
+// BBTC_KERNEL_AUTO uses the bitmap form when at least this many rows' bitmaps
+// of band k fit the wave slice (band width <= kLdsWords * 32 / this).
+constexpr int kAutoBitmapRows = 16;
 constexpr uint32_t kKindSearch = 0;
 constexpr uint32_t kKindBitmap = 1;
+constexpr uint32_t kKindDense = 2;
 
 struct TileDesc {
-  uint64_t rp;   // offset of the tile's row offsets in rowptr
-  uint64_t col;  // offset of the tile's columns in cols
-  uint64_t nz;   // offset of the tile's non-empty row list in nzrows
-  uint32_t nzr;  // number of non-empty rows
-  uint32_t h;    // band height
+  uint64_t rp;     // offset of the tile's row offsets in rowptr
+  uint64_t col;    // offset of the tile's columns in cols
+  uint64_t nz;     // offset of the tile's non-empty row list in nzrows
+  uint64_t dense;  // offset of the tile's bitmap rows in dense (~0: none)
+  uint32_t nzr;    // number of non-empty rows
+  uint32_t h;      // band height
+  uint32_t dwpr;   // bitmap words per row (multiple of 4)
+  uint32_t pad;
 };
-static_assert(sizeof(TileDesc) == 32, "TileDesc layout");
+static_assert(sizeof(TileDesc) == 48, "TileDesc layout");
 
 struct TaskDesc {
   uint32_t ab, bc, ac;  // tile ids i*P+j, j*P+k, i*P+k
-  uint32_t kind;        // kKindSearch or kKindBitmap
+  uint32_t kind;        // kKindSearch, kKindBitmap or kKindDense
   uint32_t rows;        // rows of the ab non-empty list per item (1..64)
-  uint32_t wpr;         // bitmap words per row (bitmap kind)
+  uint32_t wpr;         // bitmap words per row (bitmap and dense kinds, multiple of 4)
   uint32_t pad0, pad1;
 };
 static_assert(sizeof(TaskDesc) == 32, "TaskDesc layout");
@@ -56,13 +66,14 @@ struct CountArgs {
   const uint32_t* __restrict__ rowptr;
   const uint32_t* __restrict__ cols;
   const uint32_t* __restrict__ nzrows;
+  const uint32_t* __restrict__ dense;
   const TileDesc* __restrict__ tiles;
   const TaskDesc* __restrict__ tasks;
   const uint64_t* __restrict__ item_prefix;  // ntasks + 1 entries
   uint32_t ntasks;
-  uint64_t nitems;
+  uint64_t item_end;          // items [*head at launch, item_end) are counted
   unsigned long long* per_task;
-  unsigned long long* head;
+  unsigned long long* head;   // set to the first item of the launch beforehand
 };
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
@@ -71,6 +82,12 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
     const uint32_t y = __shfl_up(x, off, kWave);
     if (lane >= off) x += y;
   }
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
   return x;
 }
 
@@ -117,17 +134,23 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void zero_counts(unsigned long long* __restrict__ p, uint64_t n,
-                            unsigned long long* __restrict__ head) {
+// Zeroes the per-task counters (a fill kernel rather than a memset, so the
+// launch sequence stays valid inside a captured graph).
+__global__ void zero_counts(unsigned long long* __restrict__ p, uint64_t n) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
-  if (i == 0) *head = 0;
 }
 
+__global__ void set_head(unsigned long long* __restrict__ head, unsigned long long v) { *head = v; }
+
 __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
-  __shared__ uint32_t lds_all[kWavesPerBlock * kLdsWords];
+  __shared__ __attribute__((aligned(16))) uint32_t lds_all[kWavesPerBlock * kWaveLds];
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t* lds = lds_all + (threadIdx.x / kWave) * kLdsWords;
+  uint32_t* lds = lds_all + (threadIdx.x / kWave) * kWaveLds;
+  uint32_t* mk_d = lds + kLdsWords;  // x-index base of the wedge starting here
+  uint32_t* mk_s = mk_d + kWave;     // its row's membership slot
+  uint32_t* mk_n = mk_s + kWave;     // its row's list length (search form)
+  uint32_t* mk_f = mk_n + kWave;     // a wedge starts here
   // bitmap slices are all-zero between items
   for (int x = lane; x < kLdsWords; x += kWave) lds[x] = 0;
   wave_lds_sync();
@@ -138,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     unsigned long long item = 0;
     if (lane == 0) item = atomicAdd(a.head, 1ull);
     item = __shfl(item, 0, kWave);
-    if (item >= a.nitems) break;
+    if (item >= a.item_end) break;
     if (!(item >= t_lo && item < t_hi)) {
       t_cur = find_task(a.item_prefix, a.ntasks, item);
       t_lo = a.item_prefix[t_cur];
@@ -154,7 +177,8 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     const uint32_t* __restrict__ ab_c = a.cols + AB.col;
     const uint32_t* __restrict__ ac_c = a.cols + AC.col;
     const uint32_t* __restrict__ bc_c = a.cols + BC.col;
-    const bool bitmap = T.kind == kKindBitmap;
+    const bool dense = T.kind == kKindDense;
+    const bool bitmap = T.kind == kKindBitmap || dense;  // ac(u) staged as bitmaps
 
     // one row per lane
     const uint64_t z = (item - t_lo) * T.rows + uint64_t(lane);
@@ -196,7 +220,8 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     const uint32_t inclW = wave_incl_scan(dab, lane);
     const uint32_t exclW = inclW - dab;
     const uint32_t W = __shfl(inclW, kWave - 1, kWave);
-    unsigned long long cnt = 0;
+    unsigned long long cnt = 0;   // wave-uniform (ballot counts)
+    unsigned long long lcnt = 0;  // per lane (dense kind)
     for (uint32_t wb = 0; wb < W; wb += kWave) {
       const uint32_t w = wb + lane;
       const int q = owner_of(inclW, w);
@@ -204,6 +229,24 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       const uint32_t q_ex = __shfl(exclW, q, kWave);
       const uint32_t q_seg = __shfl(seg, q, kWave);
       const uint32_t q_dac = __shfl(dac, q, kWave);
+      if (dense) {
+        // |ac(u) ∩ bc(v)| = popcount of the AND of the two bitmap rows: one
+        // lane per wedge, 16 bytes per step from LDS and from HBM.
+        if (w < W) {
+          const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
+          const uint4* __restrict__ brow =
+              reinterpret_cast<const uint4*>(a.dense + BC.dense + uint64_t(v) * T.wpr);
+          const uint4* arow = reinterpret_cast<const uint4*>(lds + q_seg);
+          uint32_t c = 0;
+          for (uint32_t j = 0; j < T.wpr / 4; ++j) {
+            const uint4 bv = brow[j];
+            const uint4 av = arow[j];
+            c += __popc(av.x & bv.x) + __popc(av.y & bv.y) + __popc(av.z & bv.z) + __popc(av.w & bv.w);
+          }
+          lcnt += c;
+        }
+        continue;
+      }
       uint32_t b0 = 0, dbc = 0;
       if (w < W) {
         const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
@@ -212,26 +255,51 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       }
       const uint32_t inclP = wave_incl_scan(dbc, lane);
       const uint32_t exclP = inclP - dbc;
-      const uint32_t Pn = __shfl(inclP, kWave - 1, kWave);
+      const uint32_t Pn = __builtin_amdgcn_readlane(inclP, kWave - 1);
+      // probe p of this lane's wedge reads bc_c[my_d + p]
+      const uint32_t my_d = b0 - exclP;
+      // Each 64-probe step: wedges starting inside the step post their data at
+      // their start position; a lane's wedge is the last start at or before
+      // it (ballot + count of leading zeros), else the wedge carried over
+      // from the previous step.
+      uint32_t c_d = 0, c_s = 0, c_n = 0;
       for (uint32_t pb = 0; pb < Pn; pb += kWave) {
+        mk_f[lane] = 0;
+        wave_lds_sync();
+        if (dbc != 0 && exclP >= pb && exclP < pb + kWave) {
+          const uint32_t st = exclP - pb;
+          mk_f[st] = 1;
+          mk_d[st] = my_d;
+          mk_s[st] = q_seg;
+          mk_n[st] = q_dac;
+        }
+        wave_lds_sync();
+        const uint64_t upto = __ballot(mk_f[lane] != 0) & ((2ull << lane) - 1);
+        uint32_t d = c_d, sg = c_s, nn = c_n;
+        if (upto) {
+          const int hb = 63 - __clzll(upto);
+          d = mk_d[hb];
+          sg = mk_s[hb];
+          nn = mk_n[hb];
+        }
+        c_d = __builtin_amdgcn_readlane(d, kWave - 1);
+        c_s = __builtin_amdgcn_readlane(sg, kWave - 1);
+        c_n = __builtin_amdgcn_readlane(nn, kWave - 1);
         const uint32_t p = pb + lane;
-        const int q2 = owner_of(inclP, p);
-        const uint32_t p_b0 = __shfl(b0, q2, kWave);
-        const uint32_t p_ex = __shfl(exclP, q2, kWave);
-        const uint32_t p_seg = __shfl(q_seg, q2, kWave);
-        const uint32_t p_dac = __shfl(q_dac, q2, kWave);
         bool found = false;
         if (p < Pn) {
-          const uint32_t x = bc_c[p_b0 + (p - p_ex)];
-          if (bitmap) found = (lds[p_seg + (x >> 5)] >> (x & 31)) & 1u;
-          else found = sorted_contains(search_base + p_seg, p_dac, x);
+          const uint32_t x = bc_c[d + p];
+          if (bitmap) found = (lds[sg + (x >> 5)] >> (x & 31)) & 1u;
+          else found = sorted_contains(search_base + sg, nn, x);
         }
         cnt += __popcll(__ballot(found));
+        wave_lds_sync();
       }
     }
 
-    if (bitmap) {  // restore the all-zero slice
-      wave_lds_sync();
+    // restore the all-zero slice that bitmap items rely on
+    wave_lds_sync();
+    if (bitmap) {
       for (uint32_t s = 0; s < A; s += kWave) {
         const uint32_t e = s + lane;
         const int q = owner_of(inclA, e);
@@ -242,8 +310,11 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
           lds[uint32_t(q) * T.wpr + (x >> 5)] = 0;
         }
       }
+    } else if (in_lds) {
+      for (uint32_t e = lane; e < A; e += kWave) lds[e] = 0;
     }
     wave_lds_sync();
+    if (dense) cnt += wave_sum64(lcnt);
     if (lane == 0 && cnt) atomicAdd(a.per_task + t_cur, cnt);
   }
 }

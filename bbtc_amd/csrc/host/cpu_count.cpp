@@ -90,6 +90,15 @@ uint64_t dense_task(const TaskView& t, std::vector<uint32_t>& stamp, uint32_t wi
 
 }  // namespace
 
+uint64_t count_task_cpu(const Layout& L, const bbtc_task& x, std::vector<uint32_t>& stamp) {
+  if (task_skipped(L, x)) return 0;
+  const TaskView v = view(L, x);
+  const bool dense = L.deg[L.tile(x.i, x.k)] > kDenseDegree ||
+                     L.deg[L.tile(x.j, x.k)] > kDenseDegree ||
+                     L.deg[L.tile(x.i, x.j)] > kDenseDegree;
+  return dense ? dense_task(v, stamp, L.height(x.k)) : sparse_task(v);
+}
+
 uint64_t count_cpu(const Layout& L, const bbtc_task* tasks, uint64_t n, int nthreads,
                    uint64_t* per_task) {
   std::atomic<int64_t> next(int64_t(n) - 1);
@@ -100,15 +109,7 @@ uint64_t count_cpu(const Layout& L, const bbtc_task* tasks, uint64_t n, int nthr
     std::vector<uint32_t> stamp;
     uint64_t mine = 0;
     for (int64_t q = next.fetch_sub(1); q >= 0; q = next.fetch_sub(1)) {
-      const bbtc_task x = tasks[q];
-      uint64_t c = 0;
-      if (!task_skipped(L, x)) {
-        const TaskView v = view(L, x);
-        const bool dense = L.deg[L.tile(x.i, x.k)] > kDenseDegree ||
-                           L.deg[L.tile(x.j, x.k)] > kDenseDegree ||
-                           L.deg[L.tile(x.i, x.j)] > kDenseDegree;
-        c = dense ? dense_task(v, stamp, L.height(x.k)) : sparse_task(v);
-      }
+      const uint64_t c = count_task_cpu(L, tasks[q], stamp);
       if (per_task) per_task[q] = c;
       mine += c;
     }

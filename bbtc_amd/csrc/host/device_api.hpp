@@ -1,0 +1,29 @@
+// Internal interface between the OpenMP host code (g++) and the HIP device
+// code (hipcc) of libbbtc.so. No HIP types appear here; a "slot" names one of
+// the context's streams and the plan's work counter of the same index.
+#pragma once
+
+#include <cstdint>
+
+#include "bbtc.h"
+
+namespace bbtc {
+namespace dev {
+
+// Zeroes every per-task counter of the plan on slot 0 and waits.
+void plan_zero_sync(bbtc_plan* p);
+// Enqueues the count of tasks [t0, t1) of the plan on slot `slot`.
+void plan_launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot);
+// Work items of tasks [t0, t1) (prefix difference).
+uint64_t plan_items(const bbtc_plan* p, uint64_t t0, uint64_t t1);
+// Items the device keeps busy with: resident waves times a small factor.
+uint64_t plan_batch_items(const bbtc_plan* p);
+// True when every launch enqueued on the slot has finished.
+bool plan_slot_idle(bbtc_plan* p, int slot);
+void plan_slot_wait(bbtc_plan* p, int slot);
+// Waits for all slots, copies all per-task counters to host.
+void plan_copy_counts(bbtc_plan* p, uint64_t* out);
+int max_slots();
+
+}  // namespace dev
+}  // namespace bbtc

@@ -39,6 +39,14 @@ std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p);
 //   nzrows[nz_base[t] + q]   q-th non-empty local row
 // col_base is a multiple of 32 entries (128 bytes) so every tile segment
 // starts on a cache line.
+//
+// Tiles whose column band is narrow also keep a bitmap form: row r of tile
+// t is dense[dense_base[t] + r * dwpr[k] .. + dwpr[k]), one bit per column of
+// band k, dwpr[k] = ceil(width/32) rounded up to 4 words (16-byte rows).
+constexpr uint32_t kDenseMaxBits = 2048;             // widest band kept as bitmaps
+constexpr uint64_t kDenseTileWords = 64ull << 20;    // largest bitmap tile (256 MiB)
+constexpr uint64_t kDenseTotalWords = 4096ull << 20; // all bitmap tiles (16 GiB)
+
 struct Layout {
   uint32_t P = 0;
   uint32_t n = 0;
@@ -50,15 +58,18 @@ struct Layout {
   std::vector<uint64_t> nnz;       // P*P
   std::vector<uint32_t> nz_rows;   // P*P
   std::vector<float> deg;          // P*P, (float)nnz / (float)h
+  std::vector<uint64_t> dense_base; // P*P, ~0 when the tile has no bitmap form
+  std::vector<uint32_t> dwpr;       // P, bitmap words per row of band k
   Buf<uint32_t> rowptr;
   Buf<uint32_t> cols;
   Buf<uint32_t> nzrows;
+  Buf<uint32_t> dense;
 
   uint32_t height(uint32_t b) const { return cuts[b + 1] - cuts[b]; }
   size_t tile(uint32_t i, uint32_t k) const { return size_t(i) * P + k; }
   uint64_t device_bytes() const {
-    return rowptr.size() * 4 + cols.size() * 4 + nzrows.size() * 4 + cuts.size() * 4 +
-           3 * rp_base.size() * 8 + nnz.size() * 8;
+    return rowptr.size() * 4 + cols.size() * 4 + nzrows.size() * 4 + dense.size() * 4 +
+           cuts.size() * 4 + 4 * rp_base.size() * 8 + nnz.size() * 8;
   }
 };
 Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts);
@@ -75,5 +86,7 @@ bool task_skipped(const Layout& L, const bbtc_task& t);
 // ---- CPU count ------------------------------------------------------------
 uint64_t count_cpu(const Layout& L, const bbtc_task* t, uint64_t n, int nthreads,
                    uint64_t* per_task);
+// One task; `stamp` is the worker's reusable stamp table for dense tasks.
+uint64_t count_task_cpu(const Layout& L, const bbtc_task& t, std::vector<uint32_t>& stamp);
 
 }  // namespace bbtc

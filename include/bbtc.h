@@ -170,19 +170,28 @@ typedef struct bbtc_dgraph bbtc_dgraph;
 typedef struct bbtc_plan bbtc_plan;
 
 typedef enum bbtc_kernel {
-  BBTC_KERNEL_AUTO = 0,    /* per-task choice (bitmap where the ac band fits LDS and pays) */
-  BBTC_KERNEL_MERGE = 1,   /* wave-cooperative merge / binary-search kernel for every task */
-  BBTC_KERNEL_BITMAP = 2   /* LDS bitmap kernel wherever the ac band fits, merge elsewhere */
+  BBTC_KERNEL_AUTO = 0,    /* per task: dense where the bc tile has a bitmap form, else bitmap
+                              where at least 16 rows of the k band fit the LDS slice, else merge */
+  BBTC_KERNEL_MERGE = 1,   /* wave-cooperative merge / binary-search form for every task */
+  BBTC_KERNEL_BITMAP = 2,  /* LDS bitmap of ac(u) probed by bc(v) entries, where the band fits */
+  BBTC_KERNEL_DENSE = 3    /* LDS bitmap of ac(u) ANDed with the bitmap row of bc(v) where the
+                              bc tile has one, else as BITMAP */
 } bbtc_kernel;
 
 typedef struct bbtc_opts {
   uint64_t hbm_budget_bytes;  /* 0: no limit beyond the device's own */
 } bbtc_opts;
 
+/* Number of visible HIP devices (0 when there is none); replaces the
+ * cudaGetDeviceCount calls of utils.hpp:94 and bb_tc.cpp:19. */
+bbtc_status bbtc_device_count(int* n);
+
 /* Replaces BBTc::PrepareGpu (bb_tc_helper.cpp:53-73): one context per
  * device holds the stream, counters and scratch. */
 bbtc_status bbtc_ctx_create(int device, const bbtc_opts* opts, bbtc_ctx** out);
 void bbtc_ctx_destroy(bbtc_ctx* ctx);
+/* Waits until all work queued on the context's device has finished. */
+bbtc_status bbtc_ctx_synchronize(bbtc_ctx* ctx);
 /* Device bytes currently held by graphs and plans of this context. */
 uint64_t bbtc_ctx_bytes_in_use(const bbtc_ctx* ctx);
 
@@ -198,6 +207,13 @@ void bbtc_graph_release(bbtc_dgraph* g);
  * NoOfTrianglesInGPU (bb_tc.cpp:76-115, 142-181). */
 bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                              bbtc_kernel kernel, bbtc_plan** out);
+/* As bbtc_plan_create, with a per-task override choice[q]: 0 keeps `kernel`'s
+ * rule, 1 forces the search (merge / binary-search) form, 2 the bitmap form
+ * where the band fits the LDS slice, 3 the dense form where the bc tile has a
+ * bitmap form. Used by the autotuner; choice may be NULL. */
+bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks,
+                                   uint64_t n, bbtc_kernel kernel, const uint8_t* choice,
+                                   bbtc_plan** out);
 /* Zeroes the plan's per-task counters and enqueues the count on stream
  * (NULL: the context stream). Asynchronous. */
 bbtc_status bbtc_plan_launch(bbtc_plan* plan, void* stream);
@@ -216,6 +232,18 @@ void bbtc_plan_destroy(bbtc_plan* plan);
  * graph; per_task may be NULL. */
 bbtc_status bbtc_count(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                        bbtc_kernel kernel, uint64_t* per_task, uint64_t* total, void* stream);
+
+/* Replaces BBTc::NoOfTriangles of the GPU build with its --border semantics
+ * (bbtc.cu:54-62; bb_tc.cpp:142-252): tasks [0, separator) run on the device
+ * only; tasks [separator, n) are shared, the device taking batches from the
+ * front (one persistent launch per batch, up to nstreams batches in flight)
+ * while nthreads-1 host workers take single tasks from the back. ctx and g
+ * NULL: every task on the host (--ngpu 0). gpu_tasks_out (may be NULL)
+ * receives how many leading tasks the device counted. */
+bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* L,
+                              const bbtc_task* tasks, uint64_t n, uint64_t separator, int nthreads,
+                              int nstreams, bbtc_kernel kernel, uint64_t* per_task,
+                              uint64_t* total, uint64_t* gpu_tasks_out);
 
 #ifdef __cplusplus
 }
