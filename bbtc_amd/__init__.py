@@ -27,7 +27,7 @@ CLI_PATH = os.path.join(_HERE, "bbtc")
 
 PART = {"uni": 0, "n1d": 1, "pal": 2}
 ORDER = {"lex": 0, "desc": 1, "asc": 2}
-KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3}
+KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4}
 
 
 class BbtcError(RuntimeError):
@@ -96,6 +96,10 @@ def _declare(lib):
         "bbtc_plan_results": (st, [vp, vp, C.POINTER(u64)]),
         "bbtc_plan_time": (st, [vp, i32, vp]),
         "bbtc_plan_bitmap_tasks": (u64, [vp]),
+        "bbtc_plan_time_tasks": (st, [vp, vp]),
+        "bbtc_plan_task_kinds": (st, [vp, vp]),
+        "bbtc_plan_profile": (st, [vp, vp]),
+        "bbtc_autotune": (st, [vp, vp, vp, u64, vp, vp]),
         "bbtc_plan_destroy": (None, [vp]),
         "bbtc_count": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
         "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, C.POINTER(u64),
@@ -321,6 +325,14 @@ class DeviceGraph:
     def bytes(self) -> int:
         return lib().bbtc_graph_bytes(self._h)
 
+    def autotune(self, tasks):
+        """Per-task form choice from measured clocks: (choice[n], cycles[n,4])."""
+        t = tasks_array(tasks)
+        choice = np.zeros(len(t), dtype=np.uint8)
+        cyc = np.zeros((len(t), 4), dtype=np.uint64)
+        _check(lib().bbtc_autotune(self.dev._h, self._h, _ptr(t), len(t), _ptr(choice), _ptr(cyc)))
+        return choice, cyc
+
     def plan(self, tasks, kernel: str = "auto", choice=None) -> "Plan":
         return Plan(self, tasks, kernel, choice)
 
@@ -382,6 +394,21 @@ class Plan:
         ms = np.zeros(iters, dtype=np.float32)
         _check(lib().bbtc_plan_time(self._h, iters, _ptr(ms)))
         return ms
+
+    def time_tasks(self) -> np.ndarray:
+        ms = np.zeros(len(self.tasks), dtype=np.float32)
+        _check(lib().bbtc_plan_time_tasks(self._h, _ptr(ms)))
+        return ms
+
+    def profile(self) -> np.ndarray:
+        c = np.zeros(len(self.tasks), dtype=np.uint64)
+        _check(lib().bbtc_plan_profile(self._h, _ptr(c)))
+        return c
+
+    def kinds(self) -> np.ndarray:
+        k = np.zeros(len(self.tasks), dtype=np.uint8)
+        _check(lib().bbtc_plan_task_kinds(self._h, _ptr(k)))
+        return k
 
     def destroy(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -66,8 +66,11 @@ struct bbtc_plan {
   bbtc::TaskDesc* tasks = nullptr;
   uint64_t* item_prefix = nullptr;
   std::vector<uint64_t> hprefix;        // host copy of item_prefix
+  std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense)
   unsigned long long* per_task = nullptr;
   unsigned long long* heads = nullptr;  // one work counter per launch slot
+  unsigned long long* cycles = nullptr; // per-task clock counters while profiling
+  bool profiling = false;
   hipStream_t last_stream = nullptr;
 };
 
@@ -111,6 +114,7 @@ void release_plan(bbtc_plan* p) {
   dev_free(p->ctx, p->item_prefix);
   dev_free(p->ctx, p->per_task);
   dev_free(p->ctx, p->heads);
+  dev_free(p->ctx, p->cycles);
   if (p->ctx) p->ctx->in_use -= std::min(p->ctx->in_use, p->bytes);
   delete p;
 }
@@ -149,6 +153,7 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   a.item_end = i1;
   a.per_task = p->per_task;
   a.head = head;
+  a.cycles = p->profiling ? p->cycles : nullptr;
   const uint64_t want_blocks = (i1 - i0 + kWavesPerBlock - 1) / kWavesPerBlock;
   const uint64_t resident = uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu);
   const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident)));
@@ -364,7 +369,7 @@ bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_tas
   DEV_REQUIRE(g->ctx == ctx, "graph belongs to another context");
   DEV_REQUIRE(n < 0xFFFFFFFFull, "too many tasks");
   DEV_REQUIRE(kernel == BBTC_KERNEL_AUTO || kernel == BBTC_KERNEL_MERGE || kernel == BBTC_KERNEL_BITMAP ||
-                  kernel == BBTC_KERNEL_DENSE,
+                  kernel == BBTC_KERNEL_DENSE || kernel == BBTC_KERNEL_RDENSE,
               "unknown kernel choice");
   for (uint64_t q = 0; q < n; ++q)
     DEV_REQUIRE(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < g->P,
@@ -393,25 +398,31 @@ bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_tas
       if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
       else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
       else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
+      else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
       uint32_t kind = bbtc::kKindSearch;
       if (k == BBTC_KERNEL_BITMAP) {
         if (fit >= 1) kind = bbtc::kKindBitmap;
       } else if (k == BBTC_KERNEL_DENSE) {
         if (has_dense && fit >= 1) kind = bbtc::kKindDense;
         else if (fit >= 1) kind = bbtc::kKindBitmap;
+      } else if (k == BBTC_KERNEL_RDENSE) {
+        if (has_dense) kind = bbtc::kKindRdense;
       } else if (k == BBTC_KERNEL_AUTO) {
-        if (has_dense && fit >= 1) kind = bbtc::kKindDense;
+        // without measurements: the ac(u) list against the bc(v) bitmap row
+        // where the bc tile has one, else the ac(u) bitmap where it fits
+        if (has_dense) kind = bbtc::kKindRdense;
         else if (fit >= uint32_t(bbtc::kAutoBitmapRows)) kind = bbtc::kKindBitmap;
       }
-      const bool use_bitmap = kind != bbtc::kKindSearch;
+      const bool use_bitmap = kind == bbtc::kKindBitmap || kind == bbtc::kKindDense;
       d.kind = kind;
+      p->hkind.push_back(kind + 1);
       d.rows = use_bitmap ? std::min<uint32_t>(fit, bbtc::kWave) : uint32_t(bbtc::kWave);
-      d.wpr = use_bitmap ? wpr : 0;
+      d.wpr = kind == bbtc::kKindSearch ? 0 : wpr;
       td[q] = d;
       uint64_t items = 0;
       if (!bbtc::skipped(g, d.ab, d.bc, d.ac)) {
         items = (uint64_t(g->htiles[d.ab].nzr) + d.rows - 1) / d.rows;
-        if (use_bitmap) p->bitmap_tasks++;
+        if (kind != bbtc::kKindSearch) p->bitmap_tasks++;
       }
       pref[q + 1] = pref[q] + items;
     }
@@ -479,6 +490,60 @@ bbtc_status bbtc_plan_time(bbtc_plan* plan, int iters, float* ms_each) {
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   DEV_CATCH
+}
+
+bbtc_status bbtc_plan_time_tasks(bbtc_plan* plan, float* ms_per_task) {
+  DEV_REQUIRE(plan && (ms_per_task || plan->ntasks == 0), "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(plan->ctx->device));
+  hipStream_t s = plan->ctx->stream;
+  const uint64_t n = plan->ntasks;
+  std::vector<hipEvent_t> ev(n + 1);
+  for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  try {
+    bbtc::zero(plan, s);
+    HIP_OK(hipEventRecord(ev[0], s));
+    for (uint64_t t = 0; t < n; ++t) {
+      bbtc::launch_range(plan, t, t + 1, 0, s);
+      HIP_OK(hipEventRecord(ev[t + 1], s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    for (uint64_t t = 0; t < n; ++t) HIP_OK(hipEventElapsedTime(&ms_per_task[t], ev[t], ev[t + 1]));
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  DEV_CATCH
+}
+
+bbtc_status bbtc_plan_profile(bbtc_plan* plan, uint64_t* cycles_per_task) {
+  DEV_REQUIRE(plan && (cycles_per_task || plan->ntasks == 0), "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(plan->ctx->device));
+  hipStream_t s = plan->ctx->stream;
+  const uint64_t n = std::max<uint64_t>(plan->ntasks, 1);
+  if (!plan->cycles) plan->cycles = bbtc::dev_alloc<unsigned long long>(plan->ctx, n, &plan->bytes);
+  hipLaunchKernelGGL(bbtc::zero_counts, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, plan->cycles, n);
+  HIP_OK(hipGetLastError());
+  plan->profiling = true;
+  try {
+    bbtc::launch(plan, s);
+  } catch (...) {
+    plan->profiling = false;
+    throw;
+  }
+  plan->profiling = false;
+  if (plan->ntasks)
+    HIP_OK(hipMemcpyAsync(cycles_per_task, plan->cycles, plan->ntasks * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  DEV_CATCH
+}
+
+bbtc_status bbtc_plan_task_kinds(const bbtc_plan* plan, uint8_t* kinds) {
+  DEV_REQUIRE(plan && (kinds || plan->ntasks == 0), "null argument");
+  for (uint64_t t = 0; t < plan->ntasks; ++t) kinds[t] = uint8_t(plan->hkind[t]);
+  return BBTC_OK;
 }
 
 uint64_t bbtc_plan_bitmap_tasks(const bbtc_plan* plan) { return plan ? plan->bitmap_tasks : 0; }

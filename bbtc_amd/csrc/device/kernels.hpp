@@ -34,12 +34,14 @@ constexpr int kLdsWords = 1792;  // bitmap / sorted-list slice of a wave
 constexpr int kMarkWords = 4 * 64;  // per-step wedge marks of a wave
 constexpr int kWaveLds = kLdsWords + kMarkWords;  // 8 KiB of LDS per wave
 constexpr int kMaxSlots = 64;    // concurrent range launches of one plan
+constexpr int kItemBatch = 16;   // work items a wave claims per atomic
 // BBTC_KERNEL_AUTO uses the bitmap form when at least this many rows' bitmaps
 // of band k fit the wave slice (band width <= kLdsWords * 32 / this).
-constexpr int kAutoBitmapRows = 16;
+constexpr int kAutoBitmapRows = 1;
 constexpr uint32_t kKindSearch = 0;
 constexpr uint32_t kKindBitmap = 1;
 constexpr uint32_t kKindDense = 2;
+constexpr uint32_t kKindRdense = 3;
 
 struct TileDesc {
   uint64_t rp;     // offset of the tile's row offsets in rowptr
@@ -55,7 +57,7 @@ static_assert(sizeof(TileDesc) == 48, "TileDesc layout");
 
 struct TaskDesc {
   uint32_t ab, bc, ac;  // tile ids i*P+j, j*P+k, i*P+k
-  uint32_t kind;        // kKindSearch, kKindBitmap or kKindDense
+  uint32_t kind;        // kKindSearch, kKindBitmap, kKindDense or kKindRdense
   uint32_t rows;        // rows of the ab non-empty list per item (1..64)
   uint32_t wpr;         // bitmap words per row (bitmap and dense kinds, multiple of 4)
   uint32_t pad0, pad1;
@@ -74,6 +76,7 @@ struct CountArgs {
   uint64_t item_end;          // items [*head at launch, item_end) are counted
   unsigned long long* per_task;
   unsigned long long* head;   // set to the first item of the launch beforehand
+  unsigned long long* cycles; // per task: shader clocks its items took (NULL: off)
 };
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
@@ -156,13 +159,28 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   wave_lds_sync();
 
   uint32_t t_cur = 0;
-  uint64_t t_lo = 1, t_hi = 0;  // item range of t_cur (empty at start)
+  uint64_t t_lo = 1, t_hi = 0;   // item range of t_cur (empty at start)
+  unsigned long long acc = 0;    // count of t_cur not yet added to its slot
+  unsigned long long acc_cyc = 0;
+  unsigned long long item = 0, batch_end = 0;
   for (;;) {
-    unsigned long long item = 0;
-    if (lane == 0) item = atomicAdd(a.head, 1ull);
-    item = __shfl(item, 0, kWave);
+    // Items are claimed kItemBatch at a time: one device-scope atomic per
+    // batch instead of per item.
+    if (item >= batch_end) {
+      unsigned long long b = 0;
+      if (lane == 0) b = atomicAdd(a.head, (unsigned long long)kItemBatch);
+      item = __shfl(b, 0, kWave);
+      batch_end = item + kItemBatch;
+      if (batch_end > a.item_end) batch_end = a.item_end;
+    }
     if (item >= a.item_end) break;
+    const unsigned long long t_start = a.cycles ? clock64() : 0;
     if (!(item >= t_lo && item < t_hi)) {
+      // flush the finished task's count before moving on
+      if (lane == 0 && acc) atomicAdd(a.per_task + t_cur, acc);
+      if (lane == 0 && acc_cyc) atomicAdd(a.cycles + t_cur, acc_cyc);
+      acc = 0;
+      acc_cyc = 0;
       t_cur = find_task(a.item_prefix, a.ntasks, item);
       t_lo = a.item_prefix[t_cur];
       t_hi = a.item_prefix[t_cur + 1];
@@ -178,7 +196,9 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     const uint32_t* __restrict__ ac_c = a.cols + AC.col;
     const uint32_t* __restrict__ bc_c = a.cols + BC.col;
     const bool dense = T.kind == kKindDense;
+    const bool rdense = T.kind == kKindRdense;
     const bool bitmap = T.kind == kKindBitmap || dense;  // ac(u) staged as bitmaps
+    const uint32_t* __restrict__ bc_d = a.dense + BC.dense;  // bc bitmap rows (dense, rdense)
 
     // one row per lane
     const uint64_t z = (item - t_lo) * T.rows + uint64_t(lane);
@@ -234,10 +254,10 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
         // lane per wedge, 16 bytes per step from LDS and from HBM.
         if (w < W) {
           const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
-          const uint4* __restrict__ brow =
-              reinterpret_cast<const uint4*>(a.dense + BC.dense + uint64_t(v) * T.wpr);
+          const uint4* __restrict__ brow = reinterpret_cast<const uint4*>(bc_d + uint64_t(v) * T.wpr);
           const uint4* arow = reinterpret_cast<const uint4*>(lds + q_seg);
           uint32_t c = 0;
+#pragma unroll 2
           for (uint32_t j = 0; j < T.wpr / 4; ++j) {
             const uint4 bv = brow[j];
             const uint4 av = arow[j];
@@ -247,18 +267,25 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
         }
         continue;
       }
-      uint32_t b0 = 0, dbc = 0;
+      // the list walked per wedge: bc(v) (search, bitmap) or ac(u) (rdense)
+      uint32_t b0 = 0, dlen = 0, v = 0;
       if (w < W) {
-        const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
-        b0 = bc_rp[v];
-        dbc = bc_rp[v + 1] - b0;
+        v = ab_c[q_ab0 + (w - q_ex)];
+        if (rdense) {
+          dlen = q_dac;
+        } else {
+          b0 = bc_rp[v];
+          dlen = bc_rp[v + 1] - b0;
+        }
       }
-      const uint32_t inclP = wave_incl_scan(dbc, lane);
-      const uint32_t exclP = inclP - dbc;
+      const uint32_t inclP = wave_incl_scan(dlen, lane);
+      const uint32_t exclP = inclP - dlen;
       const uint32_t Pn = __builtin_amdgcn_readlane(inclP, kWave - 1);
-      // probe p of this lane's wedge reads bc_c[my_d + p]
-      const uint32_t my_d = b0 - exclP;
-      // Each 64-probe step: wedges starting inside the step post their data at
+      // entry p of this lane's wedge is list[my_d + p]; my_s locates the
+      // structure it is tested against
+      const uint32_t my_d = (rdense ? q_seg : b0) - exclP;
+      const uint32_t my_s = rdense ? v * T.wpr : q_seg;
+      // Each 64-entry step: wedges starting inside the step post their data at
       // their start position; a lane's wedge is the last start at or before
       // it (ballot + count of leading zeros), else the wedge carried over
       // from the previous step.
@@ -266,11 +293,11 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       for (uint32_t pb = 0; pb < Pn; pb += kWave) {
         mk_f[lane] = 0;
         wave_lds_sync();
-        if (dbc != 0 && exclP >= pb && exclP < pb + kWave) {
+        if (dlen != 0 && exclP >= pb && exclP < pb + kWave) {
           const uint32_t st = exclP - pb;
           mk_f[st] = 1;
           mk_d[st] = my_d;
-          mk_s[st] = q_seg;
+          mk_s[st] = my_s;
           mk_n[st] = q_dac;
         }
         wave_lds_sync();
@@ -288,9 +315,14 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
         const uint32_t p = pb + lane;
         bool found = false;
         if (p < Pn) {
-          const uint32_t x = bc_c[d + p];
-          if (bitmap) found = (lds[sg + (x >> 5)] >> (x & 31)) & 1u;
-          else found = sorted_contains(search_base + sg, nn, x);
+          if (rdense) {
+            const uint32_t x = search_base[d + p];
+            found = (bc_d[sg + (x >> 5)] >> (x & 31)) & 1u;
+          } else {
+            const uint32_t x = bc_c[d + p];
+            if (bitmap) found = (lds[sg + (x >> 5)] >> (x & 31)) & 1u;
+            else found = sorted_contains(search_base + sg, nn, x);
+          }
         }
         cnt += __popcll(__ballot(found));
         wave_lds_sync();
@@ -315,8 +347,12 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     }
     wave_lds_sync();
     if (dense) cnt += wave_sum64(lcnt);
-    if (lane == 0 && cnt) atomicAdd(a.per_task + t_cur, cnt);
+    acc += cnt;
+    if (a.cycles) acc_cyc += clock64() - t_start;
+    ++item;
   }
+  if (lane == 0 && acc) atomicAdd(a.per_task + t_cur, acc);
+  if (lane == 0 && acc_cyc) atomicAdd(a.cycles + t_cur, acc_cyc);
 }
 
 }  // namespace bbtc

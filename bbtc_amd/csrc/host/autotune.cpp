@@ -1,0 +1,59 @@
+// Autotuner: per-task choice of the intersection form from measured cost.
+//
+// The reference picks its GPU kernel per task from two static thresholds
+// (bb_tc.cpp:98: bitmap iff width(ac) < 46336 and h(ab) < 46336). Here every
+// form is measured instead: the whole task list runs once per form (merge,
+// bitmap, dense, rdense - each forced wherever it applies), with per-item shader
+// clock counters, while all tasks share the device as in a real count. Each
+// task keeps the form whose items took the fewest clocks.
+#include <cstdint>
+#include <limits>
+#include <vector>
+
+#include "handles.hpp"
+
+namespace bbtc {
+void set_last_error(const std::string& s);
+}
+
+extern "C" bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks,
+                                     uint64_t n, uint8_t* choice_out, uint64_t* cycles_out) {
+  if (!ctx || !g || (!tasks && n) || (!choice_out && n)) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  const uint64_t kNone = std::numeric_limits<uint64_t>::max();
+  // form codes as in bbtc_plan_create_tuned / bbtc_plan_task_kinds
+  constexpr int kForms = 4;
+  const bbtc_kernel forced[kForms] = {BBTC_KERNEL_MERGE, BBTC_KERNEL_BITMAP, BBTC_KERNEL_DENSE,
+                                      BBTC_KERNEL_RDENSE};
+  std::vector<uint64_t> best(n, kNone);
+  std::vector<uint8_t> choice(n, 1);
+  std::vector<uint64_t> cyc(n);
+  std::vector<uint8_t> kinds(n);
+  if (cycles_out)
+    for (uint64_t x = 0; x < kForms * n; ++x) cycles_out[x] = kNone;
+  for (int f = 0; f < kForms; ++f) {
+    bbtc_plan* p = nullptr;
+    bbtc_status st = bbtc_plan_create(ctx, g, tasks, n, forced[f], &p);
+    if (st != BBTC_OK) return st;
+    st = bbtc_plan_launch(p, nullptr);  // warm caches and code
+    if (st == BBTC_OK) st = bbtc_plan_results(p, nullptr, nullptr);
+    if (st == BBTC_OK) st = bbtc_plan_profile(p, cyc.data());
+    if (st == BBTC_OK) st = bbtc_plan_task_kinds(p, kinds.data());
+    bbtc_plan_destroy(p);
+    if (st != BBTC_OK) return st;
+    for (uint64_t q = 0; q < n; ++q) {
+      // a forced form that does not apply falls back to another form; only
+      // record the form the plan really used
+      if (kinds[q] != uint8_t(f + 1)) continue;
+      if (cycles_out) cycles_out[kForms * q + f] = cyc[q];
+      if (cyc[q] < best[q]) {
+        best[q] = cyc[q];
+        choice[q] = uint8_t(f + 1);
+      }
+    }
+  }
+  for (uint64_t q = 0; q < n; ++q) choice_out[q] = choice[q];
+  return BBTC_OK;
+}

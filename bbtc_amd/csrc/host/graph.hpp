@@ -43,7 +43,7 @@ std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p);
 // Tiles whose column band is narrow also keep a bitmap form: row r of tile
 // t is dense[dense_base[t] + r * dwpr[k] .. + dwpr[k]), one bit per column of
 // band k, dwpr[k] = ceil(width/32) rounded up to 4 words (16-byte rows).
-constexpr uint32_t kDenseMaxBits = 2048;             // widest band kept as bitmaps
+constexpr uint32_t kDenseMaxBits = 16384;            // widest band kept as bitmaps
 constexpr uint64_t kDenseTileWords = 64ull << 20;    // largest bitmap tile (256 MiB)
 constexpr uint64_t kDenseTotalWords = 4096ull << 20; // all bitmap tiles (16 GiB)
 

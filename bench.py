@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--ncut", type=int, default=32)
     ap.add_argument("--partitioner", default="pal")
-    ap.add_argument("--kernel", default="auto")
+    ap.add_argument("--kernel", default="tuned", help="tuned | auto | merge | bitmap | dense | rdense")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -112,8 +112,19 @@ def main():
     log(f"algorithmic bytes: {total_bytes / 1e9:.1f} GB (o={offset_bytes}), wedges {wedges} ({time.time() - t0:.1f}s)")
 
     dev = B.Device(0)
+    t0 = time.time()
     dg = dev.upload(L)
-    plan = dg.plan(tasks, args.kernel)
+    log(f"upload: {time.time() - t0:.1f}s ({dg.bytes / 1e9:.2f} GB resident)")
+    tune_s = 0.0
+    if args.kernel == "tuned":
+        t0 = time.time()
+        choice, _ = dg.autotune(tasks)
+        tune_s = time.time() - t0
+        plan = dg.plan(tasks, "auto", choice)
+        log(f"autotune: {tune_s:.1f}s, forms merge/bitmap/dense/rdense = "
+            f"{np.bincount(choice, minlength=5)[1:].tolist()}")
+    else:
+        plan = dg.plan(tasks, args.kernel)
     if args.warmup:
         plan.time(args.warmup)
     dev.synchronize()
@@ -151,7 +162,7 @@ def main():
             "workload": f"rmat-s{args.scale}-ef{args.edge_factor} {args.partitioner} ncut{args.ncut}",
             "n": n, "m": m, "ncut": args.ncut, "tasks": int(len(tasks)),
             "partitioner": args.partitioner, "kernel": args.kernel,
-            "triangles": int(total),
+            "triangles": int(total), "autotune_s": round(tune_s, 2),
         },
         "triangles_per_s": total / (ms_per_step * 1e-3),
         "roofline": {

@@ -170,12 +170,16 @@ typedef struct bbtc_dgraph bbtc_dgraph;
 typedef struct bbtc_plan bbtc_plan;
 
 typedef enum bbtc_kernel {
-  BBTC_KERNEL_AUTO = 0,    /* per task: dense where the bc tile has a bitmap form, else bitmap
-                              where at least 16 rows of the k band fit the LDS slice, else merge */
-  BBTC_KERNEL_MERGE = 1,   /* wave-cooperative merge / binary-search form for every task */
-  BBTC_KERNEL_BITMAP = 2,  /* LDS bitmap of ac(u) probed by bc(v) entries, where the band fits */
-  BBTC_KERNEL_DENSE = 3    /* LDS bitmap of ac(u) ANDed with the bitmap row of bc(v) where the
+  BBTC_KERNEL_AUTO = 0,    /* per task without measurements: RDENSE where the bc tile has a
+                              bitmap form, else BITMAP where a row of the k band fits the LDS
+                              slice, else MERGE (bbtc_autotune measures instead) */
+  BBTC_KERNEL_MERGE = 1,   /* bc(v) entries binary-searched in ac(u) (LDS copy) */
+  BBTC_KERNEL_BITMAP = 2,  /* bc(v) entries tested against an LDS bitmap of ac(u), where the
+                              band fits the slice */
+  BBTC_KERNEL_DENSE = 3,   /* LDS bitmap of ac(u) ANDed with the bitmap row of bc(v) where the
                               bc tile has one, else as BITMAP */
+  BBTC_KERNEL_RDENSE = 4   /* ac(u) entries tested against the bitmap row of bc(v) where the
+                              bc tile has one, else as MERGE */
 } bbtc_kernel;
 
 typedef struct bbtc_opts {
@@ -208,9 +212,8 @@ void bbtc_graph_release(bbtc_dgraph* g);
 bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                              bbtc_kernel kernel, bbtc_plan** out);
 /* As bbtc_plan_create, with a per-task override choice[q]: 0 keeps `kernel`'s
- * rule, 1 forces the search (merge / binary-search) form, 2 the bitmap form
- * where the band fits the LDS slice, 3 the dense form where the bc tile has a
- * bitmap form. Used by the autotuner; choice may be NULL. */
+ * rule, 1 forces MERGE, 2 BITMAP, 3 DENSE, 4 RDENSE (each where it applies).
+ * Used with bbtc_autotune's output; choice may be NULL. */
 bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks,
                                    uint64_t n, bbtc_kernel kernel, const uint8_t* choice,
                                    bbtc_plan** out);
@@ -223,7 +226,25 @@ bbtc_status bbtc_plan_results(bbtc_plan* plan, uint64_t* per_task, uint64_t* tot
 /* Launches the plan iters times on the context stream, recording a HIP event
  * pair around each launch; ms_each[iters] receives the event times. */
 bbtc_status bbtc_plan_time(bbtc_plan* plan, int iters, float* ms_each);
-/* Number of tasks the plan routed to the bitmap kernel. */
+/* Counts every task of the plan alone, one launch per task in list order with
+ * a HIP event pair around each; ms_per_task[n] receives the times. This is
+ * the measurement the autotuner builds its per-task cost table from. */
+bbtc_status bbtc_plan_time_tasks(bbtc_plan* plan, float* ms_per_task);
+/* Runs the plan once with per-item clock counters: cycles_per_task[n]
+ * receives the shader clocks the device spent on each task's work items
+ * while all tasks share the device (the autotuner's cost measure). */
+bbtc_status bbtc_plan_profile(bbtc_plan* plan, uint64_t* cycles_per_task);
+/* Autotuner: measures every form on the whole task list (each forced where it
+ * applies, per-item clock counters) and writes per task the cheapest form
+ * (1 merge, 2 bitmap, 3 dense, 4 rdense) into choice_out[n], ready for
+ * bbtc_plan_create_tuned. cycles_out (may be NULL) receives n x 4 clock
+ * counts in that order, UINT64_MAX where a form does not apply.
+ * Replaces the fixed threshold of bb_tc.cpp:98. */
+bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                          uint8_t* choice_out, uint64_t* cycles_out);
+/* Form the plan chose per task: 1 merge, 2 bitmap, 3 dense, 4 rdense. */
+bbtc_status bbtc_plan_task_kinds(const bbtc_plan* plan, uint8_t* kinds);
+/* Number of tasks the plan routed to an LDS bitmap form (bitmap or dense). */
 uint64_t bbtc_plan_bitmap_tasks(const bbtc_plan* plan);
 void bbtc_plan_destroy(bbtc_plan* plan);
 
