@@ -34,22 +34,36 @@ extern "C" bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_t
   if (cycles_out)
     for (uint64_t x = 0; x < kForms * n; ++x) cycles_out[x] = kNone;
   for (int f = 0; f < kForms; ++f) {
+    // the tasks this form applies to: a forced form that does not apply
+    // falls back to another one, which the plan reports
     bbtc_plan* p = nullptr;
     bbtc_status st = bbtc_plan_create(ctx, g, tasks, n, forced[f], &p);
     if (st != BBTC_OK) return st;
-    st = bbtc_plan_launch(p, nullptr);  // warm caches and code
-    if (st == BBTC_OK) st = bbtc_plan_results(p, nullptr, nullptr);
-    if (st == BBTC_OK) st = bbtc_plan_profile(p, cyc.data());
-    if (st == BBTC_OK) st = bbtc_plan_task_kinds(p, kinds.data());
+    st = bbtc_plan_task_kinds(p, kinds.data());
+    if (st == BBTC_OK && f == 0) {  // warm caches and code once
+      st = bbtc_plan_launch(p, nullptr);
+      if (st == BBTC_OK) st = bbtc_plan_results(p, nullptr, nullptr);
+    }
     bbtc_plan_destroy(p);
     if (st != BBTC_OK) return st;
-    for (uint64_t q = 0; q < n; ++q) {
-      // a forced form that does not apply falls back to another form; only
-      // record the form the plan really used
-      if (kinds[q] != uint8_t(f + 1)) continue;
-      if (cycles_out) cycles_out[kForms * q + f] = cyc[q];
-      if (cyc[q] < best[q]) {
-        best[q] = cyc[q];
+    std::vector<uint64_t> idx;
+    std::vector<bbtc_task> sub;
+    for (uint64_t q = 0; q < n; ++q)
+      if (kinds[q] == uint8_t(f + 1)) {
+        idx.push_back(q);
+        sub.push_back(tasks[q]);
+      }
+    if (sub.empty()) continue;
+    st = bbtc_plan_create(ctx, g, sub.data(), sub.size(), forced[f], &p);
+    if (st != BBTC_OK) return st;
+    st = bbtc_plan_profile(p, cyc.data());
+    bbtc_plan_destroy(p);
+    if (st != BBTC_OK) return st;
+    for (size_t x = 0; x < idx.size(); ++x) {
+      const uint64_t q = idx[x];
+      if (cycles_out) cycles_out[kForms * q + f] = cyc[x];
+      if (cyc[x] < best[q]) {
+        best[q] = cyc[x];
         choice[q] = uint8_t(f + 1);
       }
     }

@@ -1,0 +1,212 @@
+"""GPU parity tests: every kernel form, per task, bit-exact.
+
+Per-task uint64 counts from the device are compared with
+  - the golden vectors written by the reference's own code (tests/golden),
+  - the CPU oracle (restated reference CPU path) on random graphs,
+  - closed forms;
+and, at the full RMAT scale-26 size, with size-independent properties
+(sum of tasks = total, forms agree, total invariant across cuts) plus the
+oracle on a sample of tasks.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import bbtc_amd as B
+import oracle as O
+from tests.conftest import graph_from_spec, load_golden
+from tests.test_count_cpu import CLOSED
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["merge", "bitmap", "dense", "rdense", "auto"]
+GOLDEN = load_golden()
+COUNTED = [(d, r) for d in GOLDEN for r in d["runs"] if "tasks" in r]
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert B.device_count() >= 1, "no HIP device"
+    d = B.Device(0)
+    yield d
+    d.close()
+
+
+def all_forms(dev, L, tasks):
+    dg = dev.upload(L)
+    out = {}
+    for f in FORMS:
+        out[f] = dg.count(tasks, f)
+    choice, _ = dg.autotune(tasks)
+    pl = dg.plan(tasks, "auto", choice)
+    pl.launch()
+    out["tuned"] = pl.results()
+    pl.destroy()
+    dg.release()
+    return out
+
+
+@pytest.mark.parametrize("d,r", COUNTED, ids=[f"{d['name']}-{r['partitioner']}{r['ncut']}" for d, r in COUNTED])
+def test_golden_per_task_all_forms(dev, d, r):
+    log("case", d["name"], r["partitioner"], r["ncut"])
+    g = graph_from_spec(d["graph"])
+    L = B.Layout(g, np.array(r["cuts"], dtype=np.uint32))
+    ref = np.array(r["tasks"], dtype=np.uint64)
+    tasks = ref[:, :3].astype(np.uint32)
+    for form, (per, tot) in all_forms(dev, L, tasks).items():
+        assert tot == r["total"], form
+        assert np.array_equal(per, ref[:, 3]), form
+
+
+@pytest.mark.parametrize("name,g,expect", CLOSED, ids=[c[0] for c in CLOSED])
+def test_closed_forms_gpu(dev, name, g, expect):
+    for p in (1, 3):
+        L = B.Layout(g, g.partition("uni", p))
+        t = L.tasks("desc")
+        for form, (per, tot) in all_forms(dev, L, t).items():
+            assert tot == expect, (form, p)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_graphs_vs_oracle(dev, seed):
+    rng = np.random.default_rng(seed + 50)
+    if seed % 2:
+        g = B.Graph.rmat(int(rng.integers(8, 15)), int(rng.integers(4, 32)), seed=seed)
+    else:
+        n = int(rng.integers(50, 20000))
+        g = B.Graph.gnm(n, int(rng.integers(n, 20 * n)), seed)
+    part = ["uni", "n1d", "pal"][seed % 3]
+    p = int(rng.integers(1, 17))
+    log("case seed", seed, "n", g.n, "m", g.m, part, p)
+    cuts = g.partition(part, p)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    for form, (per, tot) in all_forms(dev, L, t).items():
+        assert tot == rtot and np.array_equal(per, rper), form
+
+
+@pytest.mark.parametrize("sep_frac", [0.0, 0.5, 1.0, 2.0])
+def test_hybrid_border(dev, sep_frac):
+    g = B.Graph.rmat(16, 16, seed=12)
+    L = B.Layout(g, g.partition("pal", 16))
+    t = L.tasks("desc")
+    cper, ctot = L.count_cpu(t)
+    dg = dev.upload(L)
+    sep = int(sep_frac * len(t))
+    for nthreads, nstreams in [(1, 1), (8, 2), (16, 4)]:
+        per, tot, gt = dg.count_hybrid(t, sep, nthreads=nthreads, nstreams=nstreams)
+        assert tot == ctot and np.array_equal(per, cper)
+        assert gt >= min(sep, len(t))
+        if nthreads == 1:
+            assert gt == len(t)  # no host workers: the device counts everything
+    dg.release()
+
+
+def test_plan_relaunch_and_timing(dev):
+    g = B.Graph.rmat(15, 16, seed=4)
+    L = B.Layout(g, g.partition("pal", 8))
+    t = L.tasks("desc")
+    dg = dev.upload(L)
+    pl = dg.plan(t, "auto")
+    pl.launch()
+    a = pl.results()[0]
+    ms = pl.time(3)
+    b = pl.results()[0]
+    assert np.array_equal(a, b) and len(ms) == 3 and (ms > 0).all()
+    tt = pl.time_tasks()
+    assert len(tt) == len(t) and np.array_equal(pl.results()[0], a)
+    cyc = pl.profile()
+    assert np.array_equal(pl.results()[0], a) and cyc.sum() > 0
+    pl.destroy()
+    dg.release()
+
+
+def test_subset_tasks_and_empty(dev):
+    g = B.Graph.rmat(12, 8, seed=2)
+    L = B.Layout(g, g.partition("uni", 4))
+    t = L.tasks("lex")
+    dg = dev.upload(L)
+    cper, _ = L.count_cpu(t)
+    sub = t[::3]
+    per, tot = dg.count(sub, "auto")
+    assert np.array_equal(per, cper[::3])
+    per, tot = dg.count(np.zeros((0, 3), np.uint32), "auto")
+    assert tot == 0
+    with pytest.raises(B.BbtcError):
+        dg.count(np.array([[2, 1, 3]], np.uint32))
+    dg.release()
+
+
+def test_budget_is_enforced():
+    g = B.Graph.rmat(12, 8, seed=2)
+    L = B.Layout(g, g.partition("uni", 4))
+    d = B.Device(0, hbm_budget_bytes=1024)
+    with pytest.raises(B.BbtcError) as e:
+        d.upload(L)
+    assert e.value.status == 7  # BBTC_ERR_BUDGET
+    assert d.bytes_in_use == 0
+    d.close()
+
+
+def test_cli_gpu(tmp_path):
+    d = load_golden("c1_rmat_s16_ef8")[0]
+    g = graph_from_spec(d["graph"])
+    path = str(tmp_path / "c1.bbtc")
+    g.save(path)
+    for run in d["runs"]:
+        for extra in ([], ["--border", "-1"], ["--border", "7", "--nstream", "2"], ["--copy-first"]):
+            out = tmp_path / "pt.txt"
+            cmd = [B.CLI_PATH, "--graph", path, "--partitioner", run["partitioner"], "--ncut", str(run["ncut"]),
+                   "--repeat", "2", "--ngpu", "1", "--per-task-out", str(out)] + extra
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            assert f"Number of Triangles: {run['total']}" in r.stdout
+            assert "Number of GPUs: 1" in r.stdout
+            got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
+            assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
+
+
+# ---- full size: RMAT scale 26 ------------------------------------------------
+@pytest.fixture(scope="module")
+def s26():
+    if os.environ.get("BBTC_SKIP_S26"):
+        pytest.skip("BBTC_SKIP_S26 set")
+    g = B.Graph.rmat(26, 16, 0.57, 0.19, 0.19, 3)
+    log("s26 generated", g.info)
+    return g
+
+
+def test_s26_properties(dev, s26):
+    g = s26
+    cuts = g.partition("pal", 32)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    dg = dev.upload(L)
+    per_a, tot_a = dg.count(t, "auto")
+    per_r, tot_r = dg.count(t, "rdense")
+    assert int(per_a.sum()) == tot_a
+    assert tot_a == tot_r and np.array_equal(per_a, per_r)
+    # oracle on every 97th task (lexicographic), bit-exact
+    lex = np.lexsort((t["k"], t["j"], t["i"]))[::97]
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    tt = np.stack([t["i"][lex], t["j"][lex], t["k"][lex]], 1).astype(np.uint32)
+    rper, _ = R.count(tt)
+    del R
+    assert np.array_equal(rper, per_a[lex])
+    dg.release()
+    log("s26 pal total", tot_a)
+    # the total does not depend on the cuts
+    L2 = B.Layout(g, g.partition("uni", 16))
+    t2 = L2.tasks("desc")
+    dg2 = dev.upload(L2)
+    per2, tot2 = dg2.count(t2, "auto")
+    dg2.release()
+    assert tot2 == tot_a
