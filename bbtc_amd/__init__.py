@@ -67,6 +67,8 @@ def _declare(lib):
         "bbtc_hgraph_from_edges": (st, [u32, u64, vp, vp, pvp]),
         "bbtc_hgraph_gen_rmat": (st, [u32, u32, C.c_double, C.c_double, C.c_double, u64, pvp]),
         "bbtc_hgraph_gen_gnm": (st, [u32, u64, u64, pvp]),
+        "bbtc_hgraph_gen_rmat_device": (st, [vp, u32, u32, C.c_double, C.c_double, C.c_double, u64, pvp]),
+        "bbtc_hgraph_gen_gnm_device": (st, [vp, u32, u64, u64, pvp]),
         "bbtc_hgraph_info": (None, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64)]),
         "bbtc_hgraph_offsets": (vp, [vp]),
         "bbtc_hgraph_cols": (vp, [vp]),
@@ -165,6 +167,16 @@ class Graph:
     @classmethod
     def gnm(cls, n, m, seed) -> "Graph":
         return cls._make(lib().bbtc_hgraph_gen_gnm, n, m, seed)
+
+    @classmethod
+    def rmat_device(cls, dev, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1) -> "Graph":
+        """Same graph as rmat(), generated on the GPU of `dev`."""
+        return cls._make(lib().bbtc_hgraph_gen_rmat_device, dev._h, scale, edge_factor, a, b, c, seed)
+
+    @classmethod
+    def gnm_device(cls, dev, n, m, seed) -> "Graph":
+        """Same graph as gnm(), generated on the GPU of `dev`."""
+        return cls._make(lib().bbtc_hgraph_gen_gnm_device, dev._h, n, m, seed)
 
     @classmethod
     def from_csr(cls, offsets, cols, n=None, ncols=None) -> "Graph":

@@ -14,65 +14,9 @@
 #include <string>
 #include <vector>
 
-#include "../host/handles.hpp"
-#include "kernels.hpp"
 #include "../host/device_api.hpp"
-
-namespace bbtc {
-
-#define HIP_OK(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      ::bbtc::fail(BBTC_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_));    \
-  } while (0)
-
-}  // namespace bbtc
-
-// ---------------------------------------------------------------------------
-// host objects
-// ---------------------------------------------------------------------------
-struct bbtc_ctx {
-  int device = 0;
-  int num_cu = 0;
-  int blocks_per_cu = 1;  // resident blocks of count_probe per CU
-  hipStream_t stream = nullptr;          // slot 0
-  std::vector<hipStream_t> slots;         // slot i > 0, created on first use
-  uint64_t budget = 0;
-  uint64_t in_use = 0;
-};
-
-struct bbtc_dgraph {
-  bbtc_ctx* ctx = nullptr;
-  uint32_t P = 0;
-  uint64_t bytes = 0;
-  uint32_t* rowptr = nullptr;
-  uint32_t* cols = nullptr;
-  uint32_t* nzrows = nullptr;
-  uint32_t* dense = nullptr;
-  bbtc::TileDesc* tiles = nullptr;
-  std::vector<bbtc::TileDesc> htiles;
-  std::vector<float> deg;
-  std::vector<uint32_t> dwpr;  // bitmap words per row of each band
-};
-
-struct bbtc_plan {
-  bbtc_ctx* ctx = nullptr;
-  bbtc_dgraph* g = nullptr;
-  uint64_t ntasks = 0;
-  uint64_t nitems = 0;
-  uint64_t bitmap_tasks = 0;
-  uint64_t bytes = 0;
-  bbtc::TaskDesc* tasks = nullptr;
-  uint64_t* item_prefix = nullptr;
-  std::vector<uint64_t> hprefix;        // host copy of item_prefix
-  std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense)
-  unsigned long long* per_task = nullptr;
-  unsigned long long* heads = nullptr;  // one work counter per launch slot
-  unsigned long long* cycles = nullptr; // per-task clock counters while profiling
-  bool profiling = false;
-  hipStream_t last_stream = nullptr;
-};
+#include "kernels.hpp"
+#include "objects.hpp"
 
 namespace bbtc {
 void set_last_error(const std::string& s);

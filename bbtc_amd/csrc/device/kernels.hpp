@@ -25,59 +25,9 @@
 
 #include <cstdint>
 
+#include "descriptors.hpp"
+
 namespace bbtc {
-
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kLdsWords = 1792;  // bitmap / sorted-list slice of a wave
-constexpr int kMarkWords = 4 * 64;  // per-step wedge marks of a wave
-constexpr int kWaveLds = kLdsWords + kMarkWords;  // 8 KiB of LDS per wave
-constexpr int kMaxSlots = 64;    // concurrent range launches of one plan
-constexpr int kItemBatch = 16;   // work items a wave claims per atomic
-// BBTC_KERNEL_AUTO uses the bitmap form when at least this many rows' bitmaps
-// of band k fit the wave slice (band width <= kLdsWords * 32 / this).
-constexpr int kAutoBitmapRows = 1;
-constexpr uint32_t kKindSearch = 0;
-constexpr uint32_t kKindBitmap = 1;
-constexpr uint32_t kKindDense = 2;
-constexpr uint32_t kKindRdense = 3;
-
-struct TileDesc {
-  uint64_t rp;     // offset of the tile's row offsets in rowptr
-  uint64_t col;    // offset of the tile's columns in cols
-  uint64_t nz;     // offset of the tile's non-empty row list in nzrows
-  uint64_t dense;  // offset of the tile's bitmap rows in dense (~0: none)
-  uint32_t nzr;    // number of non-empty rows
-  uint32_t h;      // band height
-  uint32_t dwpr;   // bitmap words per row (multiple of 4)
-  uint32_t pad;
-};
-static_assert(sizeof(TileDesc) == 48, "TileDesc layout");
-
-struct TaskDesc {
-  uint32_t ab, bc, ac;  // tile ids i*P+j, j*P+k, i*P+k
-  uint32_t kind;        // kKindSearch, kKindBitmap, kKindDense or kKindRdense
-  uint32_t rows;        // rows of the ab non-empty list per item (1..64)
-  uint32_t wpr;         // bitmap words per row (bitmap and dense kinds, multiple of 4)
-  uint32_t pad0, pad1;
-};
-static_assert(sizeof(TaskDesc) == 32, "TaskDesc layout");
-
-struct CountArgs {
-  const uint32_t* __restrict__ rowptr;
-  const uint32_t* __restrict__ cols;
-  const uint32_t* __restrict__ nzrows;
-  const uint32_t* __restrict__ dense;
-  const TileDesc* __restrict__ tiles;
-  const TaskDesc* __restrict__ tasks;
-  const uint64_t* __restrict__ item_prefix;  // ntasks + 1 entries
-  uint32_t ntasks;
-  uint64_t item_end;          // items [*head at launch, item_end) are counted
-  unsigned long long* per_task;
-  unsigned long long* head;   // set to the first item of the launch beforehand
-  unsigned long long* cycles; // per task: shader clocks its items took (NULL: off)
-};
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
 #pragma unroll

@@ -98,10 +98,12 @@ def main():
     import bbtc_amd as B
 
     offset_bytes = 8 if args.scale >= 26 else 4
+    dev = B.Device(0)
     t0 = time.time()
-    g = B.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
+    g = B.Graph.rmat_device(dev, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
     n, _, m = g.info
-    log(f"graph: RMAT s{args.scale} ef{args.edge_factor} n={n} m={m} ({time.time() - t0:.1f}s)")
+    log(f"graph: RMAT s{args.scale} ef{args.edge_factor} n={n} m={m} (generated on the GPU in "
+        f"{time.time() - t0:.1f}s)")
     t0 = time.time()
     cuts = g.partition(args.partitioner, args.ncut)
     L = B.Layout(g, cuts)
@@ -111,7 +113,6 @@ def main():
     bytes_per_task, total_bytes, wedges = L.task_bytes(tasks, offset_bytes)
     log(f"algorithmic bytes: {total_bytes / 1e9:.1f} GB (o={offset_bytes}), wedges {wedges} ({time.time() - t0:.1f}s)")
 
-    dev = B.Device(0)
     t0 = time.time()
     dg = dev.upload(L)
     log(f"upload: {time.time() - t0:.1f}s ({dg.bytes / 1e9:.2f} GB resident)")

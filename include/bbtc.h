@@ -199,6 +199,14 @@ bbtc_status bbtc_ctx_synchronize(bbtc_ctx* ctx);
 /* Device bytes currently held by graphs and plans of this context. */
 uint64_t bbtc_ctx_bytes_in_use(const bbtc_ctx* ctx);
 
+/* The seeded generators above run on the context's device (radix sorts in
+ * HBM) and return the same host CSR, bit for bit, much faster for large
+ * graphs. Sample or candidate counts must stay below 2^31. */
+bbtc_status bbtc_hgraph_gen_rmat_device(bbtc_ctx* ctx, uint32_t scale, uint32_t edge_factor, double a,
+                                        double b, double c, uint64_t seed, bbtc_hgraph** out);
+bbtc_status bbtc_hgraph_gen_gnm_device(bbtc_ctx* ctx, uint32_t n, uint64_t m, uint64_t seed,
+                                       bbtc_hgraph** out);
+
 /* Replaces CopyTilesToGpu / CopyTileToGpu (bb_tc_helper.cpp:75-172, the
  * --copy-first path): the whole block CSR becomes device resident once. */
 bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* L, bbtc_dgraph** out);

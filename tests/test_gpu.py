@@ -174,12 +174,39 @@ def test_cli_gpu(tmp_path):
             assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
 
 
+@pytest.mark.parametrize("spec", [("rmat", 10, 8, 1), ("rmat", 18, 16, 7), ("rmat", 20, 32, 5),
+                                  ("gnm", 5000, 40000, 3), ("gnm", 1 << 18, 1 << 21, 9)])
+def test_device_generator_matches_host(dev, spec):
+    from tests.golden.fingerprint import fingerprint
+    if spec[0] == "rmat":
+        h = B.Graph.rmat(spec[1], spec[2], seed=spec[3])
+        d = B.Graph.rmat_device(dev, spec[1], spec[2], seed=spec[3])
+    else:
+        h = B.Graph.gnm(spec[1], spec[2], spec[3])
+        d = B.Graph.gnm_device(dev, spec[1], spec[2], spec[3])
+    assert np.array_equal(h.offsets(), d.offsets()) and np.array_equal(h.cols(), d.cols())
+    assert dev.bytes_in_use == 0
+
+
+def test_device_generator_golden_graphs(dev):
+    from tests.golden.fingerprint import fingerprint
+    for d in GOLDEN:
+        s = d["graph"]
+        if s["kind"] == "rmat":
+            g = B.Graph.rmat_device(dev, s["scale"], s["ef"], s["a"], s["b"], s["c"], s["seed"])
+        else:
+            g = B.Graph.gnm_device(dev, s["n"], s["m"], s["seed"])
+        assert fingerprint(g.offsets(), g.cols()) == d["fingerprint"], d["name"]
+
+
 # ---- full size: RMAT scale 26 ------------------------------------------------
 @pytest.fixture(scope="module")
 def s26():
     if os.environ.get("BBTC_SKIP_S26"):
         pytest.skip("BBTC_SKIP_S26 set")
-    g = B.Graph.rmat(26, 16, 0.57, 0.19, 0.19, 3)
+    d = B.Device(0)
+    g = B.Graph.rmat_device(d, 26, 16, 0.57, 0.19, 0.19, 3)
+    d.close()
     log("s26 generated", g.info)
     return g
 
