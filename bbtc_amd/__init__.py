@@ -106,6 +106,19 @@ def _declare(lib):
         "bbtc_count": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
         "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, C.POINTER(u64),
                                    C.POINTER(u64)]),
+        "bbtc_layouts_build": (st, [vp, u64, vp, i32, vp]),
+        "bbtc_tune_ncut": (st, [vp, vp, u64, vp, u32, i32, vp, vp]),
+        "bbtc_rule_ncut": (u32, [vp, u64]),
+        "bbtc_batch_create": (st, [vp, vp, u64, i32, pvp]),
+        "bbtc_batch_ntasks": (u64, [vp]),
+        "bbtc_batch_bytes": (u64, [vp]),
+        "bbtc_batch_tasks": (st, [vp, vp, vp]),
+        "bbtc_batch_autotune": (st, [vp, vp]),
+        "bbtc_batch_set_forms": (st, [vp, i32, vp]),
+        "bbtc_batch_capture": (st, [vp]),
+        "bbtc_batch_launch": (st, [vp]),
+        "bbtc_batch_results": (st, [vp, vp, vp, vp]),
+        "bbtc_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -425,6 +438,102 @@ class Plan:
     def destroy(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.bbtc_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.destroy()
+
+
+# ---- batches -------------------------------------------------------------------
+def layouts_build(graphs, ncuts, kind: str = "pal"):
+    """Block layouts of many graphs, built in parallel (one graph per thread)."""
+    n = len(graphs)
+    gh = (C.c_void_p * max(n, 1))(*[g._h for g in graphs])
+    nc = np.ascontiguousarray(ncuts, dtype=np.uint32)
+    out = (C.c_void_p * max(n, 1))()
+    _check(lib().bbtc_layouts_build(gh, n, _ptr(nc), PART[kind], out))
+    layouts = []
+    for x in range(n):
+        L = Layout.__new__(Layout)
+        L.graph = graphs[x]
+        L._h = C.c_void_p(out[x])
+        L.p = int(nc[x])
+        L.cuts = None
+        layouts.append(L)
+    return layouts
+
+
+def tune_ncut(dev: "Device", sample, cands=(1, 2, 4, 8), kind: str = "pal"):
+    """Cut-count rule from measured device work on sample graphs:
+    returns (rule[64] indexed by floor(log2(m+1)), work[n, ncand])."""
+    n = len(sample)
+    gh = (C.c_void_p * max(n, 1))(*[g._h for g in sample])
+    cc = np.ascontiguousarray(cands, dtype=np.uint32)
+    rule = np.zeros(64, dtype=np.uint32)
+    work = np.zeros((max(n, 1), len(cc)), dtype=np.uint64)
+    _check(lib().bbtc_tune_ncut(dev._h, gh, n, _ptr(cc), len(cc), PART[kind], _ptr(rule), _ptr(work)))
+    return rule, work[:n]
+
+
+def rule_ncut(rule, m: int) -> int:
+    r = np.ascontiguousarray(rule, dtype=np.uint32)
+    return int(lib().bbtc_rule_ncut(_ptr(r), m))
+
+
+class Batch:
+    """Many graphs resident in one arena, counted by one persistent launch."""
+
+    def __init__(self, dev: "Device", layouts, kernel: str = "auto"):
+        self.dev = dev
+        self.layouts = list(layouts)
+        n = len(self.layouts)
+        lh = (C.c_void_p * max(n, 1))(*[L._h for L in self.layouts])
+        h = C.c_void_p()
+        _check(lib().bbtc_batch_create(dev._h, lh, n, KERNEL[kernel], C.byref(h)))
+        self._h = h
+        self.ngraphs = n
+
+    @property
+    def ntasks(self) -> int:
+        return lib().bbtc_batch_ntasks(self._h)
+
+    @property
+    def bytes(self) -> int:
+        return lib().bbtc_batch_bytes(self._h)
+
+    def tasks(self):
+        n = self.ntasks
+        g = np.zeros(max(n, 1), dtype=np.uint32)
+        t = np.zeros(max(n, 1), dtype=TASK_DTYPE)
+        _check(lib().bbtc_batch_tasks(self._h, _ptr(g), _ptr(t)))
+        return g[:n], t[:n]
+
+    def autotune(self):
+        c = np.zeros(max(self.ntasks, 1), dtype=np.uint8)
+        _check(lib().bbtc_batch_autotune(self._h, _ptr(c)))
+        return c[: self.ntasks]
+
+    def set_forms(self, kernel: str = "auto", choice=None):
+        ch = None if choice is None else np.ascontiguousarray(choice, dtype=np.uint8)
+        _check(lib().bbtc_batch_set_forms(self._h, KERNEL[kernel], None if ch is None else _ptr(ch)))
+
+    def capture(self):
+        _check(lib().bbtc_batch_capture(self._h))
+
+    def launch(self):
+        _check(lib().bbtc_batch_launch(self._h))
+
+    def results(self):
+        """(per-graph counts, per-task counts in batch order, per-graph latency ms)."""
+        pg = np.zeros(max(self.ngraphs, 1), dtype=np.uint64)
+        pt = np.zeros(max(self.ntasks, 1), dtype=np.uint64)
+        lat = np.zeros(max(self.ngraphs, 1), dtype=np.float64)
+        _check(lib().bbtc_batch_results(self._h, _ptr(pg), _ptr(pt), _ptr(lat)))
+        return pg[: self.ngraphs], pt[: self.ntasks], lat[: self.ngraphs]
+
+    def destroy(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

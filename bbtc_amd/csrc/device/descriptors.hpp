@@ -39,7 +39,8 @@ struct TaskDesc {
   uint32_t kind;        // kKindSearch, kKindBitmap, kKindDense or kKindRdense
   uint32_t rows;        // rows of the ab non-empty list per item (1..64)
   uint32_t wpr;         // bitmap words per row (bitmap and dense kinds, multiple of 4)
-  uint32_t pad0, pad1;
+  uint32_t graph;        // index of the task's graph in a batch (0 otherwise)
+  uint32_t pad1;
 };
 static_assert(sizeof(TaskDesc) == 32, "TaskDesc layout");
 
@@ -56,6 +57,10 @@ struct CountArgs {
   unsigned long long* per_task;
   unsigned long long* head;   // set to the first item of the launch beforehand
   unsigned long long* cycles; // per task: shader clocks its items took (NULL: off)
+  // Per-graph completion (batches; NULL: off): work items still to finish, and
+  // the device wall clock when a graph's last item finished.
+  unsigned long long* left;
+  unsigned long long* done;
 };
 
 }  // namespace bbtc

@@ -98,6 +98,8 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   a.per_task = p->per_task;
   a.head = head;
   a.cycles = p->profiling ? p->cycles : nullptr;
+  a.left = p->track_left;
+  a.done = p->track_done;
   const uint64_t want_blocks = (i1 - i0 + kWavesPerBlock - 1) / kWavesPerBlock;
   const uint64_t resident = uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu);
   const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident)));
@@ -108,6 +110,163 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
 void launch(bbtc_plan* p, hipStream_t s) {
   zero(p, s);
   launch_range(p, 0, p->ntasks, 0, s);
+}
+
+}  // namespace
+}  // namespace bbtc
+
+namespace bbtc {
+namespace {
+
+bool valid_kernel(bbtc_kernel k) {
+  return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
+         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE;
+}
+
+// A task named by its three tile ids in a graph's tile table, and the graph
+// it belongs to (0 for a single graph, the graph's index in a batch).
+struct TaskRef {
+  uint32_t ab, bc, ac, graph;
+};
+
+std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n) {
+  std::vector<TaskRef> r(n);
+  for (uint64_t q = 0; q < n; ++q) {
+    const bbtc_task& x = tasks[q];
+    r[q] = TaskRef{x.i * P + x.j, x.j * P + x.k, x.i * P + x.k, 0};
+  }
+  return r;
+}
+
+// Builds a plan over tile-addressed tasks: the form of each task (kernel
+// rule, or choice[q] = 1 merge, 2 bitmap, 3 dense, 4 rdense), its rows per
+// work item, and the item prefix over tasks.
+bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& refs, bbtc_kernel kernel,
+                     const uint8_t* choice) {
+  const uint64_t n = refs.size();
+  auto* p = new bbtc_plan;
+  p->ctx = ctx;
+  p->g = g;
+  p->ntasks = n;
+  try {
+    std::vector<TaskDesc> td(std::max<uint64_t>(n, 1));
+    std::vector<uint64_t> pref(n + 1, 0);
+    p->hkind.reserve(n);
+    for (uint64_t q = 0; q < n; ++q) {
+      const TaskRef& x = refs[q];
+      TaskDesc d{};
+      d.ab = x.ab;
+      d.bc = x.bc;
+      d.ac = x.ac;
+      d.graph = x.graph;
+      // LDS form of ac(u): a bitmap over band k takes dwpr words per row; the
+      // wave slice holds kLdsWords words.
+      const uint32_t wpr = g->htiles[x.ac].dwpr;
+      const uint32_t fit = wpr ? uint32_t(kLdsWords) / wpr : uint32_t(kWave);
+      const bool has_dense = g->htiles[x.bc].dense != ~0ull;
+      bbtc_kernel k = kernel;
+      if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
+      else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
+      else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
+      else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
+      uint32_t kind = kKindSearch;
+      if (k == BBTC_KERNEL_BITMAP) {
+        if (fit >= 1) kind = kKindBitmap;
+      } else if (k == BBTC_KERNEL_DENSE) {
+        if (has_dense && fit >= 1) kind = kKindDense;
+        else if (fit >= 1) kind = kKindBitmap;
+      } else if (k == BBTC_KERNEL_RDENSE) {
+        if (has_dense) kind = kKindRdense;
+      } else if (k == BBTC_KERNEL_AUTO) {
+        // without measurements: the ac(u) list against the bc(v) bitmap row
+        // where the bc tile has one, else the ac(u) bitmap where it fits
+        if (has_dense) kind = kKindRdense;
+        else if (fit >= uint32_t(kAutoBitmapRows)) kind = kKindBitmap;
+      }
+      const bool use_bitmap = kind == kKindBitmap || kind == kKindDense;
+      d.kind = kind;
+      p->hkind.push_back(kind + 1);
+      d.rows = use_bitmap ? std::min<uint32_t>(fit, kWave) : uint32_t(kWave);
+      d.wpr = kind == kKindSearch ? 0 : wpr;
+      td[q] = d;
+      uint64_t items = 0;
+      if (!skipped(g, d.ab, d.bc, d.ac)) {
+        items = (uint64_t(g->htiles[d.ab].nzr) + d.rows - 1) / d.rows;
+        if (kind != kKindSearch) p->bitmap_tasks++;
+      }
+      pref[q + 1] = pref[q] + items;
+    }
+    p->nitems = pref[n];
+    p->hprefix = pref;
+    HIP_OK(hipSetDevice(ctx->device));
+    p->tasks = dev_alloc<TaskDesc>(ctx, td.size(), &p->bytes);
+    p->item_prefix = dev_alloc<uint64_t>(ctx, pref.size(), &p->bytes);
+    p->per_task = dev_alloc<unsigned long long>(ctx, std::max<uint64_t>(n, 1), &p->bytes);
+    p->heads = dev_alloc<unsigned long long>(ctx, kMaxSlots, &p->bytes);
+    HIP_OK(hipMemcpyAsync(p->tasks, td.data(), td.size() * sizeof(TaskDesc), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(p->item_prefix, pref.data(), pref.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    release_plan(p);
+    throw;
+  }
+  return p;
+}
+
+// Per-task form from measured clocks: the task list runs once per form
+// (each on the tasks it applies to) with per-item clock counters while all
+// those tasks share the device; each task keeps its cheapest form. This
+// replaces the fixed kernel threshold of bb_tc.cpp:98.
+void autotune_refs(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& refs, uint8_t* choice_out,
+                   uint64_t* cycles_out) {
+  constexpr int kForms = 4;
+  const bbtc_kernel forced[kForms] = {BBTC_KERNEL_MERGE, BBTC_KERNEL_BITMAP, BBTC_KERNEL_DENSE,
+                                      BBTC_KERNEL_RDENSE};
+  const uint64_t n = refs.size();
+  const uint64_t none = ~0ull;
+  std::vector<uint64_t> best(n, none);
+  std::vector<uint8_t> choice(n, 1);
+  if (cycles_out)
+    for (uint64_t x = 0; x < kForms * n; ++x) cycles_out[x] = none;
+  for (int f = 0; f < kForms; ++f) {
+    std::vector<uint32_t> kinds;
+    {
+      bbtc_plan* p = make_plan(ctx, g, refs, forced[f], nullptr);
+      kinds = p->hkind;
+      if (f == 0) {  // warm caches and code once
+        try {
+          launch(p, ctx->stream);
+          HIP_OK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+          release_plan(p);
+          throw;
+        }
+      }
+      release_plan(p);
+    }
+    std::vector<uint64_t> idx;
+    std::vector<TaskRef> sub;
+    for (uint64_t q = 0; q < n; ++q)
+      if (kinds[q] == uint32_t(f + 1)) {  // the form really applies to this task
+        idx.push_back(q);
+        sub.push_back(refs[q]);
+      }
+    if (sub.empty()) continue;
+    bbtc_plan* p = make_plan(ctx, g, sub, forced[f], nullptr);
+    std::vector<uint64_t> cyc(sub.size());
+    const bbtc_status st = bbtc_plan_profile(p, cyc.data());
+    release_plan(p);
+    if (st != BBTC_OK) fail(st, bbtc_last_error());
+    for (size_t x = 0; x < idx.size(); ++x) {
+      const uint64_t q = idx[x];
+      if (cycles_out) cycles_out[kForms * q + f] = cyc[x];
+      if (cyc[x] < best[q]) {
+        best[q] = cyc[x];
+        choice[q] = uint8_t(f + 1);
+      }
+    }
+  }
+  std::copy(choice.begin(), choice.end(), choice_out);
 }
 
 }  // namespace
@@ -312,79 +471,12 @@ bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_tas
   DEV_REQUIRE(ctx && g && out && (tasks || n == 0), "null argument");
   DEV_REQUIRE(g->ctx == ctx, "graph belongs to another context");
   DEV_REQUIRE(n < 0xFFFFFFFFull, "too many tasks");
-  DEV_REQUIRE(kernel == BBTC_KERNEL_AUTO || kernel == BBTC_KERNEL_MERGE || kernel == BBTC_KERNEL_BITMAP ||
-                  kernel == BBTC_KERNEL_DENSE || kernel == BBTC_KERNEL_RDENSE,
-              "unknown kernel choice");
+  DEV_REQUIRE(bbtc::valid_kernel(kernel), "unknown kernel choice");
   for (uint64_t q = 0; q < n; ++q)
     DEV_REQUIRE(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < g->P,
                 "task out of range or not i <= j <= k");
   DEV_TRY
-  auto* p = new bbtc_plan;
-  p->ctx = ctx;
-  p->g = g;
-  p->ntasks = n;
-  try {
-    const uint32_t P = g->P;
-    std::vector<bbtc::TaskDesc> td(std::max<uint64_t>(n, 1));
-    std::vector<uint64_t> pref(n + 1, 0);
-    for (uint64_t q = 0; q < n; ++q) {
-      const bbtc_task& x = tasks[q];
-      bbtc::TaskDesc d{};
-      d.ab = x.i * P + x.j;
-      d.bc = x.j * P + x.k;
-      d.ac = x.i * P + x.k;
-      // LDS form of ac(u): a bitmap over band k takes dwpr words per row; the
-      // wave slice holds kLdsWords words.
-      const uint32_t wpr = g->dwpr[x.k];
-      const uint32_t fit = wpr ? uint32_t(bbtc::kLdsWords) / wpr : uint32_t(bbtc::kWave);
-      const bool has_dense = g->htiles[d.bc].dense != ~0ull;
-      bbtc_kernel k = kernel;
-      if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
-      else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
-      else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
-      else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
-      uint32_t kind = bbtc::kKindSearch;
-      if (k == BBTC_KERNEL_BITMAP) {
-        if (fit >= 1) kind = bbtc::kKindBitmap;
-      } else if (k == BBTC_KERNEL_DENSE) {
-        if (has_dense && fit >= 1) kind = bbtc::kKindDense;
-        else if (fit >= 1) kind = bbtc::kKindBitmap;
-      } else if (k == BBTC_KERNEL_RDENSE) {
-        if (has_dense) kind = bbtc::kKindRdense;
-      } else if (k == BBTC_KERNEL_AUTO) {
-        // without measurements: the ac(u) list against the bc(v) bitmap row
-        // where the bc tile has one, else the ac(u) bitmap where it fits
-        if (has_dense) kind = bbtc::kKindRdense;
-        else if (fit >= uint32_t(bbtc::kAutoBitmapRows)) kind = bbtc::kKindBitmap;
-      }
-      const bool use_bitmap = kind == bbtc::kKindBitmap || kind == bbtc::kKindDense;
-      d.kind = kind;
-      p->hkind.push_back(kind + 1);
-      d.rows = use_bitmap ? std::min<uint32_t>(fit, bbtc::kWave) : uint32_t(bbtc::kWave);
-      d.wpr = kind == bbtc::kKindSearch ? 0 : wpr;
-      td[q] = d;
-      uint64_t items = 0;
-      if (!bbtc::skipped(g, d.ab, d.bc, d.ac)) {
-        items = (uint64_t(g->htiles[d.ab].nzr) + d.rows - 1) / d.rows;
-        if (kind != bbtc::kKindSearch) p->bitmap_tasks++;
-      }
-      pref[q + 1] = pref[q] + items;
-    }
-    p->nitems = pref[n];
-    p->hprefix = pref;
-    HIP_OK(hipSetDevice(ctx->device));
-    p->tasks = bbtc::dev_alloc<bbtc::TaskDesc>(ctx, td.size(), &p->bytes);
-    p->item_prefix = bbtc::dev_alloc<uint64_t>(ctx, pref.size(), &p->bytes);
-    p->per_task = bbtc::dev_alloc<unsigned long long>(ctx, std::max<uint64_t>(n, 1), &p->bytes);
-    p->heads = bbtc::dev_alloc<unsigned long long>(ctx, bbtc::kMaxSlots, &p->bytes);
-    HIP_OK(hipMemcpyAsync(p->tasks, td.data(), td.size() * sizeof(bbtc::TaskDesc), hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipMemcpyAsync(p->item_prefix, pref.data(), pref.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipStreamSynchronize(ctx->stream));
-  } catch (...) {
-    bbtc::release_plan(p);
-    throw;
-  }
-  *out = p;
+  *out = bbtc::make_plan(ctx, g, bbtc::refs_of(g->P, tasks, n), kernel, choice);
   DEV_CATCH
 }
 
@@ -484,6 +576,18 @@ bbtc_status bbtc_plan_profile(bbtc_plan* plan, uint64_t* cycles_per_task) {
   DEV_CATCH
 }
 
+bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                          uint8_t* choice_out, uint64_t* cycles_out) {
+  DEV_REQUIRE(ctx && g && (tasks || n == 0) && (choice_out || n == 0), "null argument");
+  DEV_REQUIRE(g->ctx == ctx, "graph belongs to another context");
+  for (uint64_t q = 0; q < n; ++q)
+    DEV_REQUIRE(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < g->P,
+                "task out of range or not i <= j <= k");
+  DEV_TRY
+  bbtc::autotune_refs(ctx, g, bbtc::refs_of(g->P, tasks, n), choice_out, cycles_out);
+  DEV_CATCH
+}
+
 bbtc_status bbtc_plan_task_kinds(const bbtc_plan* plan, uint8_t* kinds) {
   DEV_REQUIRE(plan && (kinds || plan->ntasks == 0), "null argument");
   for (uint64_t t = 0; t < plan->ntasks; ++t) kinds[t] = uint8_t(plan->hkind[t]);
@@ -508,6 +612,298 @@ bbtc_status bbtc_count(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, ui
   if (st == BBTC_OK) st = bbtc_plan_results(p, per_task, total);
   bbtc_plan_destroy(p);
   return st;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many graphs, one arena, one persistent launch
+// ---------------------------------------------------------------------------
+struct bbtc_batch {
+  bbtc_ctx* ctx = nullptr;
+  bbtc_dgraph* arena = nullptr;  // all graphs' arrays; tile ids are global
+  bbtc_plan* plan = nullptr;
+  uint64_t ngraphs = 0;
+  std::vector<bbtc::TaskRef> refs;     // tasks in launch order
+  std::vector<uint32_t> task_ijk;      // i, j, k of each task (3 per task)
+  std::vector<unsigned long long> hleft0;
+  uint64_t bytes = 0;
+  unsigned long long* left0 = nullptr;
+  unsigned long long* left = nullptr;
+  unsigned long long* done = nullptr;
+  unsigned long long* start = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double wall_khz = 0;
+};
+
+namespace bbtc {
+namespace {
+
+void batch_drop_capture(bbtc_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void batch_release(bbtc_batch* b) {
+  if (!b) return;
+  batch_drop_capture(b);
+  release_plan(b->plan);
+  release_graph(b->arena);
+  if (b->ctx) {
+    dev_free(b->ctx, b->left0);
+    dev_free(b->ctx, b->left);
+    dev_free(b->ctx, b->done);
+    dev_free(b->ctx, b->start);
+    b->ctx->in_use -= std::min(b->ctx->in_use, b->bytes);
+  }
+  delete b;
+}
+
+// New plan for the batch's tasks (choice may be NULL) and per-graph item
+// counts for completion tracking.
+void batch_replan(bbtc_batch* b, bbtc_kernel kernel, const uint8_t* choice) {
+  batch_drop_capture(b);
+  if (b->plan) {
+    release_plan(b->plan);
+    b->plan = nullptr;
+  }
+  b->plan = make_plan(b->ctx, b->arena, b->refs, kernel, choice);
+  b->hleft0.assign(std::max<uint64_t>(b->ngraphs, 1), 0);
+  for (uint64_t q = 0; q < b->refs.size(); ++q)
+    b->hleft0[b->refs[q].graph] += b->plan->hprefix[q + 1] - b->plan->hprefix[q];
+  HIP_OK(hipMemcpyAsync(b->left0, b->hleft0.data(), b->hleft0.size() * 8, hipMemcpyHostToDevice,
+                        b->ctx->stream));
+  HIP_OK(hipStreamSynchronize(b->ctx->stream));
+  b->plan->track_left = b->left;
+  b->plan->track_done = b->done;
+}
+
+// The launch sequence of one batch count (no host work, no memsets: it can
+// be captured in a graph and replayed).
+void batch_enqueue(bbtc_batch* b, hipStream_t s) {
+  const uint64_t ng = std::max<uint64_t>(b->ngraphs, 1);
+  hipLaunchKernelGGL(batch_begin, dim3(unsigned((ng + 255) / 256)), dim3(256), 0, s, b->left0, b->ngraphs,
+                     b->left, b->done, b->start);
+  HIP_OK(hipGetLastError());
+  zero(b->plan, s);
+  launch_range(b->plan, 0, b->plan->ntasks, 0, s);
+}
+
+}  // namespace
+}  // namespace bbtc
+
+extern "C" {
+
+bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, uint64_t ngraphs,
+                              bbtc_kernel kernel, bbtc_batch** out) {
+  DEV_REQUIRE(ctx && out && (layouts || ngraphs == 0), "null argument");
+  DEV_REQUIRE(bbtc::valid_kernel(kernel), "unknown kernel choice");
+  DEV_REQUIRE(ngraphs < 0xFFFFFFFFull, "too many graphs");
+  for (uint64_t gi = 0; gi < ngraphs; ++gi) DEV_REQUIRE(layouts[gi], "null layout");
+  DEV_TRY
+  HIP_OK(hipSetDevice(ctx->device));
+  auto* b = new bbtc_batch;
+  b->ctx = ctx;
+  b->ngraphs = ngraphs;
+  try {
+    int khz = 0;
+    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    b->wall_khz = khz > 0 ? double(khz) : 1e5;
+    // arena sizes and per-graph bases
+    uint64_t n_rp = 0, n_col = 0, n_nz = 0, n_dense = 0, n_tiles = 0;
+    std::vector<uint64_t> rp0(ngraphs), col0(ngraphs), nz0(ngraphs), dn0(ngraphs), tile0(ngraphs);
+    for (uint64_t gi = 0; gi < ngraphs; ++gi) {
+      const bbtc::Layout& L = layouts[gi]->L;
+      rp0[gi] = n_rp;
+      col0[gi] = n_col;
+      nz0[gi] = n_nz;
+      dn0[gi] = n_dense;
+      tile0[gi] = n_tiles;
+      n_rp += L.rowptr.size();
+      n_col += L.cols.size();
+      n_nz += L.nzrows.size();
+      n_dense += L.dense.size();
+      n_tiles += uint64_t(L.P) * L.P;
+    }
+    if (n_tiles >= 0xFFFFFFFFull) bbtc::fail(BBTC_ERR_INVALID, "too many tiles in one batch");
+    auto* A = new bbtc_dgraph;
+    A->ctx = ctx;
+    A->P = 0;
+    b->arena = A;
+    A->htiles.resize(std::max<uint64_t>(n_tiles, 1));
+    A->deg.assign(std::max<uint64_t>(n_tiles, 1), 0.f);
+    A->rowptr = bbtc::dev_alloc<uint32_t>(ctx, n_rp, &A->bytes);
+    A->cols = bbtc::dev_alloc<uint32_t>(ctx, n_col, &A->bytes);
+    A->nzrows = bbtc::dev_alloc<uint32_t>(ctx, n_nz, &A->bytes);
+    A->dense = bbtc::dev_alloc<uint32_t>(ctx, n_dense, &A->bytes);
+    A->tiles = bbtc::dev_alloc<bbtc::TileDesc>(ctx, std::max<uint64_t>(n_tiles, 1), &A->bytes);
+    struct Order {
+      float w;
+      uint64_t q;
+    };
+    std::vector<Order> order;
+    for (uint64_t gi = 0; gi < ngraphs; ++gi) {
+      const bbtc::Layout& L = layouts[gi]->L;
+      const uint32_t P = L.P;
+      for (uint32_t i = 0; i < P; ++i)
+        for (uint32_t k = 0; k < P; ++k) {
+          const size_t t = L.tile(i, k);
+          bbtc::TileDesc d{};
+          d.h = L.height(i);
+          d.dense = ~0ull;
+          d.dwpr = L.dwpr[k];
+          if (i <= k) {
+            d.rp = rp0[gi] + L.rp_base[t];
+            d.col = col0[gi] + L.col_base[t];
+            d.nz = nz0[gi] + L.nz_base[t];
+            d.nzr = L.nz_rows[t];
+            if (L.dense_base[t] != ~0ull) d.dense = dn0[gi] + L.dense_base[t];
+          }
+          A->htiles[tile0[gi] + t] = d;
+          A->deg[tile0[gi] + t] = L.deg[t];
+        }
+      hipStream_t s = ctx->stream;
+      HIP_OK(hipMemcpyAsync(A->rowptr + rp0[gi], L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(A->cols + col0[gi], L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(A->dense + dn0[gi], L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, s));
+      // this graph's tasks, weighted as tasks.hpp:21-23 for the cross-graph order
+      for (uint32_t i = 0; i < P; ++i)
+        for (uint32_t j = i; j < P; ++j)
+          for (uint32_t k = j; k < P; ++k) {
+            const uint32_t base = uint32_t(tile0[gi]);
+            b->refs.push_back(bbtc::TaskRef{base + i * P + j, base + j * P + k, base + i * P + k, uint32_t(gi)});
+            b->task_ijk.insert(b->task_ijk.end(), {i, j, k});
+            const uint32_t nnz_ab = uint32_t(L.nnz[L.tile(i, j)]);
+            const float w = nnz_ab * std::max(L.deg[L.tile(j, k)], L.deg[L.tile(i, k)]);
+            order.push_back(Order{w, b->refs.size() - 1});
+          }
+    }
+    HIP_OK(hipMemcpyAsync(A->tiles, A->htiles.data(), A->htiles.size() * sizeof(bbtc::TileDesc),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    // heavy tasks of every graph first (NaN weights of empty bands last)
+    std::stable_sort(order.begin(), order.end(), [](const Order& l, const Order& r) {
+      const float a = l.w == l.w ? l.w : -1.f, c = r.w == r.w ? r.w : -1.f;
+      return a > c;
+    });
+    std::vector<bbtc::TaskRef> refs(order.size());
+    std::vector<uint32_t> ijk(3 * order.size());
+    for (size_t x = 0; x < order.size(); ++x) {
+      refs[x] = b->refs[order[x].q];
+      for (int c = 0; c < 3; ++c) ijk[3 * x + c] = b->task_ijk[3 * order[x].q + c];
+    }
+    b->refs.swap(refs);
+    b->task_ijk.swap(ijk);
+    const uint64_t ng = std::max<uint64_t>(ngraphs, 1);
+    b->left0 = bbtc::dev_alloc<unsigned long long>(ctx, ng, &b->bytes);
+    b->left = bbtc::dev_alloc<unsigned long long>(ctx, ng, &b->bytes);
+    b->done = bbtc::dev_alloc<unsigned long long>(ctx, ng, &b->bytes);
+    b->start = bbtc::dev_alloc<unsigned long long>(ctx, 1, &b->bytes);
+    bbtc::batch_replan(b, kernel, nullptr);
+  } catch (...) {
+    bbtc::batch_release(b);
+    throw;
+  }
+  *out = b;
+  DEV_CATCH
+}
+
+uint64_t bbtc_batch_ntasks(const bbtc_batch* b) { return b ? b->refs.size() : 0; }
+uint64_t bbtc_batch_bytes(const bbtc_batch* b) {
+  return b ? b->bytes + (b->arena ? b->arena->bytes : 0) + (b->plan ? b->plan->bytes : 0) : 0;
+}
+
+bbtc_status bbtc_batch_tasks(const bbtc_batch* b, uint32_t* graph, bbtc_task* tasks) {
+  DEV_REQUIRE(b, "null argument");
+  for (uint64_t q = 0; q < b->refs.size(); ++q) {
+    if (graph) graph[q] = b->refs[q].graph;
+    if (tasks) tasks[q] = bbtc_task{b->task_ijk[3 * q], b->task_ijk[3 * q + 1], b->task_ijk[3 * q + 2]};
+  }
+  return BBTC_OK;
+}
+
+bbtc_status bbtc_batch_autotune(bbtc_batch* b, uint8_t* choice_out) {
+  DEV_REQUIRE(b, "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(b->ctx->device));
+  std::vector<uint8_t> choice(b->refs.size());
+  bbtc::autotune_refs(b->ctx, b->arena, b->refs, choice.data(), nullptr);
+  bbtc::batch_replan(b, BBTC_KERNEL_AUTO, choice.data());
+  if (choice_out) std::copy(choice.begin(), choice.end(), choice_out);
+  DEV_CATCH
+}
+
+bbtc_status bbtc_batch_set_forms(bbtc_batch* b, bbtc_kernel kernel, const uint8_t* choice) {
+  DEV_REQUIRE(b && bbtc::valid_kernel(kernel), "bad argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(b->ctx->device));
+  bbtc::batch_replan(b, kernel, choice);
+  DEV_CATCH
+}
+
+bbtc_status bbtc_batch_capture(bbtc_batch* b) {
+  DEV_REQUIRE(b, "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(b->ctx->device));
+  bbtc::batch_drop_capture(b);
+  hipStream_t s = b->ctx->stream;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    bbtc::batch_enqueue(b, s);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIP_OK(hipStreamEndCapture(s, &b->graph));
+  HIP_OK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  DEV_CATCH
+}
+
+bbtc_status bbtc_batch_launch(bbtc_batch* b) {
+  DEV_REQUIRE(b, "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(b->ctx->device));
+  if (b->exec) HIP_OK(hipGraphLaunch(b->exec, b->ctx->stream));
+  else bbtc::batch_enqueue(b, b->ctx->stream);
+  DEV_CATCH
+}
+
+bbtc_status bbtc_batch_results(bbtc_batch* b, uint64_t* per_graph, uint64_t* per_task,
+                               double* latency_ms) {
+  DEV_REQUIRE(b, "null argument");
+  DEV_TRY
+  HIP_OK(hipSetDevice(b->ctx->device));
+  hipStream_t s = b->ctx->stream;
+  const uint64_t n = b->refs.size(), ng = b->ngraphs;
+  std::vector<uint64_t> pt(std::max<uint64_t>(n, 1), 0);
+  std::vector<unsigned long long> done(std::max<uint64_t>(ng, 1), 0);
+  unsigned long long start = 0;
+  if (n) HIP_OK(hipMemcpyAsync(pt.data(), b->plan->per_task, n * 8, hipMemcpyDeviceToHost, s));
+  if (ng) HIP_OK(hipMemcpyAsync(done.data(), b->done, ng * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(&start, b->start, 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (per_graph) {
+    for (uint64_t gi = 0; gi < ng; ++gi) per_graph[gi] = 0;
+    for (uint64_t q = 0; q < n; ++q) per_graph[b->refs[q].graph] += pt[q];
+  }
+  if (per_task) std::copy(pt.begin(), pt.begin() + n, per_task);
+  if (latency_ms)
+    for (uint64_t gi = 0; gi < ng; ++gi)
+      latency_ms[gi] = done[gi] >= start ? double(done[gi] - start) / b->wall_khz : 0.0;
+  DEV_CATCH
+}
+
+void bbtc_batch_destroy(bbtc_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  bbtc::batch_release(b);
 }
 
 }  // extern "C"

@@ -112,6 +112,20 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   uint64_t t_lo = 1, t_hi = 0;   // item range of t_cur (empty at start)
   unsigned long long acc = 0;    // count of t_cur not yet added to its slot
   unsigned long long acc_cyc = 0;
+  unsigned long long run = 0;    // items of t_cur finished since the last flush
+  auto flush = [&]() {
+    if (lane == 0) {
+      if (acc) atomicAdd(a.per_task + t_cur, acc);
+      if (acc_cyc) atomicAdd(a.cycles + t_cur, acc_cyc);
+      if (a.left && run) {
+        const uint32_t gi = a.tasks[t_cur].graph;
+        if (atomicSub(a.left + gi, run) == run) a.done[gi] = wall_clock64();
+      }
+    }
+    acc = 0;
+    acc_cyc = 0;
+    run = 0;
+  };
   unsigned long long item = 0, batch_end = 0;
   for (;;) {
     // Items are claimed kItemBatch at a time: one device-scope atomic per
@@ -126,11 +140,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     if (item >= a.item_end) break;
     const unsigned long long t_start = a.cycles ? clock64() : 0;
     if (!(item >= t_lo && item < t_hi)) {
-      // flush the finished task's count before moving on
-      if (lane == 0 && acc) atomicAdd(a.per_task + t_cur, acc);
-      if (lane == 0 && acc_cyc) atomicAdd(a.cycles + t_cur, acc_cyc);
-      acc = 0;
-      acc_cyc = 0;
+      flush();  // the finished task's count and progress, before moving on
       t_cur = find_task(a.item_prefix, a.ntasks, item);
       t_lo = a.item_prefix[t_cur];
       t_hi = a.item_prefix[t_cur + 1];
@@ -299,10 +309,24 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     if (dense) cnt += wave_sum64(lcnt);
     acc += cnt;
     if (a.cycles) acc_cyc += clock64() - t_start;
+    ++run;
     ++item;
   }
-  if (lane == 0 && acc) atomicAdd(a.per_task + t_cur, acc);
-  if (lane == 0 && acc_cyc) atomicAdd(a.cycles + t_cur, acc_cyc);
+  flush();
+}
+
+// Start of a batch launch: stamp the device wall clock and reset each
+// graph's outstanding item count (graphs without items finish at once).
+__global__ void batch_begin(const unsigned long long* __restrict__ left0, uint64_t ngraphs,
+                            unsigned long long* __restrict__ left, unsigned long long* __restrict__ done,
+                            unsigned long long* __restrict__ start) {
+  const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const unsigned long long now = wall_clock64();
+  if (g == 0) *start = now;
+  if (g < ngraphs) {
+    left[g] = left0[g];
+    done[g] = left0[g] ? 0ull : now;
+  }
 }
 
 }  // namespace bbtc

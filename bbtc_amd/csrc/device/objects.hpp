@@ -44,6 +44,8 @@ struct bbtc_dgraph {
   std::vector<uint32_t> dwpr;  // bitmap words per row of each band
 };
 
+struct bbtc_batch;
+
 struct bbtc_plan {
   bbtc_ctx* ctx = nullptr;
   bbtc_dgraph* g = nullptr;
@@ -59,6 +61,8 @@ struct bbtc_plan {
   unsigned long long* heads = nullptr;  // one work counter per launch slot
   unsigned long long* cycles = nullptr; // per-task clock counters while profiling
   bool profiling = false;
+  unsigned long long* track_left = nullptr;  // per-graph completion (batch-owned)
+  unsigned long long* track_done = nullptr;
   hipStream_t last_stream = nullptr;
 };
 

@@ -262,6 +262,51 @@ void bbtc_plan_destroy(bbtc_plan* plan);
 bbtc_status bbtc_count(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                        bbtc_kernel kernel, uint64_t* per_task, uint64_t* total, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batches: many independent graphs, one persistent launch.                  */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference (one graph per run). Block layouts of n
+ * graphs built in parallel, graph x with ncut[x] cuts of partitioner kind. */
+bbtc_status bbtc_layouts_build(const bbtc_hgraph* const* graphs, uint64_t n, const uint32_t* ncut,
+                               bbtc_part kind, bbtc_layout** out);
+
+/* Cut-count autotuner: counts every sample graph with every candidate cut
+ * count (forms autotuned per graph) and measures the device work (summed
+ * per-item clocks); rule[64] receives, per size class floor(log2(m+1)), the
+ * candidate with the least work (classes without samples copy the nearest
+ * sampled class). work_out (may be NULL) receives n x ncand clock sums. */
+bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint64_t n,
+                           const uint32_t* cands, uint32_t ncand, bbtc_part kind, uint32_t* rule,
+                           uint64_t* work_out);
+/* The rule's cut count for a graph with m edges. */
+uint32_t bbtc_rule_ncut(const uint32_t* rule, uint64_t m);
+
+typedef struct bbtc_batch bbtc_batch;
+/* Uploads n layouts into one device arena and plans every task of every
+ * graph (heavy tasks of all graphs first) for one persistent launch. */
+bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, uint64_t n,
+                              bbtc_kernel kernel, bbtc_batch** out);
+uint64_t bbtc_batch_ntasks(const bbtc_batch* b);
+uint64_t bbtc_batch_bytes(const bbtc_batch* b);
+/* The batch's task order: graph index and (i,j,k) of each task (either may be NULL). */
+bbtc_status bbtc_batch_tasks(const bbtc_batch* b, uint32_t* graph, bbtc_task* tasks);
+/* Measures every form on the batch (as bbtc_autotune) and replans with the
+ * cheapest per task; choice_out (may be NULL) receives the forms. */
+bbtc_status bbtc_batch_autotune(bbtc_batch* b, uint8_t* choice_out);
+/* Replans with a kernel rule and optional per-task forms (batch task order). */
+bbtc_status bbtc_batch_set_forms(bbtc_batch* b, bbtc_kernel kernel, const uint8_t* choice);
+/* Records the launch sequence (clock stamp, counter resets, persistent count)
+ * as a HIP graph; later bbtc_batch_launch calls replay it. */
+bbtc_status bbtc_batch_capture(bbtc_batch* b);
+/* Enqueues one count of the whole batch on the context stream. */
+bbtc_status bbtc_batch_launch(bbtc_batch* b);
+/* Waits, then writes per-graph counts (n), per-task counts (batch order) and
+ * each graph's completion time after the launch began, in ms, from the
+ * device wall clock. Any output may be NULL. */
+bbtc_status bbtc_batch_results(bbtc_batch* b, uint64_t* per_graph, uint64_t* per_task,
+                               double* latency_ms);
+void bbtc_batch_destroy(bbtc_batch* b);
+
 /* Replaces BBTc::NoOfTriangles of the GPU build with its --border semantics
  * (bbtc.cu:54-62; bb_tc.cpp:142-252): tasks [0, separator) run on the device
  * only; tasks [separator, n) are shared, the device taking batches from the
