@@ -14,6 +14,15 @@ constexpr int kMarkWords = 4 * 64;  // per-step wedge marks of a wave
 constexpr int kWaveLds = kLdsWords + kMarkWords;  // 8 KiB of LDS per wave
 constexpr int kMaxSlots = 64;    // concurrent range launches of one plan
 constexpr int kItemBatch = 16;   // work items a wave claims per atomic
+// Splitting of heavy items. A first-phase item that has spent more than
+// kDeferSteps wave steps (one step: 64 probes, one 64-wedge batch, or one
+// 16-byte bitmap word per lane) hands its remaining wedges to the second phase
+// in chunks of about kChunkSteps steps each; a launch slot holds at most
+// kSpillCap such records.
+constexpr uint32_t kDeferSteps = 512;
+constexpr uint32_t kChunkSteps = 256;
+constexpr uint32_t kSpillCap = 1u << 20;
+constexpr int kSpillShift = 36;  // SlotCtl::spill = (records << 36) | chunks
 // BBTC_KERNEL_AUTO uses the bitmap form when at least this many rows' bitmaps
 // of band k fit the wave slice (band width <= kLdsWords * 32 / this).
 constexpr int kAutoBitmapRows = 1;
@@ -44,6 +53,23 @@ struct TaskDesc {
 };
 static_assert(sizeof(TaskDesc) == 32, "TaskDesc layout");
 
+// Control words of one launch slot, reset before each launch.
+struct SlotCtl {
+  unsigned long long head;   // next first-phase item
+  unsigned long long spill;  // (records << kSpillShift) | chunks reserved
+  unsigned long long head2;  // next second-phase chunk
+  unsigned long long pad;
+};
+
+// Wedges [w0, w1) of plan item `item` (in the item's flattened wedge order),
+// counted by the second phase in chunks of wc wedges.
+struct SpillRec {
+  uint64_t item;
+  uint32_t w0, w1, wc;
+  uint32_t pad;
+};
+static_assert(sizeof(SpillRec) == 24, "SpillRec layout");
+
 struct CountArgs {
   const uint32_t* __restrict__ rowptr;
   const uint32_t* __restrict__ cols;
@@ -55,7 +81,11 @@ struct CountArgs {
   uint32_t ntasks;
   uint64_t item_end;          // items [*head at launch, item_end) are counted
   unsigned long long* per_task;
-  unsigned long long* head;   // set to the first item of the launch beforehand
+  SlotCtl* ctl;               // head set to the first item of the launch beforehand
+  SpillRec* spill;            // spill_cap records (first phase writes, second reads)
+  uint64_t* spill_base;       // first chunk of each record (increasing)
+  uint32_t spill_cap;         // 0: no splitting
+  uint32_t phase;             // 0: items from ctl->head; 1: chunks from ctl->head2
   unsigned long long* cycles; // per task: shader clocks its items took (NULL: off)
   // Per-graph completion (batches; NULL: off): work items still to finish, and
   // the device wall clock when a graph's last item finished.

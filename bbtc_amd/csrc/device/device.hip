@@ -57,7 +57,11 @@ void release_plan(bbtc_plan* p) {
   dev_free(p->ctx, p->tasks);
   dev_free(p->ctx, p->item_prefix);
   dev_free(p->ctx, p->per_task);
-  dev_free(p->ctx, p->heads);
+  dev_free(p->ctx, p->ctl);
+  for (auto& sp : p->spill) {
+    dev_free(p->ctx, sp.rec);
+    dev_free(p->ctx, sp.base);
+  }
   dev_free(p->ctx, p->cycles);
   if (p->ctx) p->ctx->in_use -= std::min(p->ctx->in_use, p->bytes);
   delete p;
@@ -82,8 +86,14 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   const uint64_t i0 = p->hprefix[t0], i1 = p->hprefix[t1];
   p->last_stream = s;
   if (i1 <= i0) return;
-  unsigned long long* head = p->heads + slot;
-  hipLaunchKernelGGL(set_head, dim3(1), dim3(1), 0, s, head, (unsigned long long)i0);
+  if (p->spill.size() <= size_t(slot)) p->spill.resize(size_t(slot) + 1);
+  bbtc_plan::Spill& sp = p->spill[slot];
+  if (!sp.rec) {
+    sp.rec = dev_alloc<SpillRec>(p->ctx, p->spill_cap, &p->bytes);
+    sp.base = dev_alloc<uint64_t>(p->ctx, p->spill_cap, &p->bytes);
+  }
+  SlotCtl* ctl = p->ctl + slot;
+  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, ctl, (unsigned long long)i0);
   HIP_OK(hipGetLastError());
   CountArgs a;
   a.rowptr = g->rowptr;
@@ -96,7 +106,11 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   a.ntasks = uint32_t(p->ntasks);
   a.item_end = i1;
   a.per_task = p->per_task;
-  a.head = head;
+  a.ctl = ctl;
+  a.spill = sp.rec;
+  a.spill_base = sp.base;
+  a.spill_cap = p->spill_cap;
+  a.phase = 0;
   a.cycles = p->profiling ? p->cycles : nullptr;
   a.left = p->track_left;
   a.done = p->track_done;
@@ -104,6 +118,10 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   const uint64_t resident = uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu);
   const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident)));
   hipLaunchKernelGGL(count_probe, dim3(blocks), dim3(kBlock), 0, s, a);
+  HIP_OK(hipGetLastError());
+  // spilled chunks of heavy items, over the whole machine
+  a.phase = 1;
+  hipLaunchKernelGGL(count_probe, dim3(unsigned(std::max<uint64_t>(1, resident))), dim3(kBlock), 0, s, a);
   HIP_OK(hipGetLastError());
 }
 
@@ -202,7 +220,11 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
     p->tasks = dev_alloc<TaskDesc>(ctx, td.size(), &p->bytes);
     p->item_prefix = dev_alloc<uint64_t>(ctx, pref.size(), &p->bytes);
     p->per_task = dev_alloc<unsigned long long>(ctx, std::max<uint64_t>(n, 1), &p->bytes);
-    p->heads = dev_alloc<unsigned long long>(ctx, kMaxSlots, &p->bytes);
+    p->ctl = dev_alloc<SlotCtl>(ctx, kMaxSlots, &p->bytes);
+    p->spill_cap = uint32_t(std::min<uint64_t>(std::max<uint64_t>(p->nitems, 1), kSpillCap));
+    p->spill.resize(1);
+    p->spill[0].rec = dev_alloc<SpillRec>(ctx, p->spill_cap, &p->bytes);
+    p->spill[0].base = dev_alloc<uint64_t>(ctx, p->spill_cap, &p->bytes);
     HIP_OK(hipMemcpyAsync(p->tasks, td.data(), td.size() * sizeof(TaskDesc), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(p->item_prefix, pref.data(), pref.size() * 8, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));

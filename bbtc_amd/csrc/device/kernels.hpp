@@ -94,8 +94,16 @@ __global__ void zero_counts(unsigned long long* __restrict__ p, uint64_t n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void set_head(unsigned long long* __restrict__ head, unsigned long long v) { *head = v; }
+// Resets a launch slot: first-phase items from i0, no spilled work.
+__global__ void reset_slot(SlotCtl* __restrict__ c, unsigned long long i0) {
+  c->head = i0;
+  c->spill = 0;
+  c->head2 = 0;
+}
 
+// One persistent launch per phase. Phase 0 drains items [ctl->head,
+// item_end); a unit is an item's whole wedge range. Phase 1 drains the
+// chunks that phase 0 spilled; a unit is one chunk of an item's wedges.
 __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   __shared__ __attribute__((aligned(16))) uint32_t lds_all[kWavesPerBlock * kWaveLds];
   const int lane = threadIdx.x & (kWave - 1);
@@ -104,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   uint32_t* mk_s = mk_d + kWave;     // its row's membership slot
   uint32_t* mk_n = mk_s + kWave;     // its row's list length (search form)
   uint32_t* mk_f = mk_n + kWave;     // a wedge starts here
-  // bitmap slices are all-zero between items
+  // bitmap slices are all-zero between units
   for (int x = lane; x < kLdsWords; x += kWave) lds[x] = 0;
   wave_lds_sync();
 
@@ -112,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   uint64_t t_lo = 1, t_hi = 0;   // item range of t_cur (empty at start)
   unsigned long long acc = 0;    // count of t_cur not yet added to its slot
   unsigned long long acc_cyc = 0;
-  unsigned long long run = 0;    // items of t_cur finished since the last flush
+  unsigned long long run = 0;    // units of t_cur finished since the last flush
   auto flush = [&]() {
     if (lane == 0) {
       if (acc) atomicAdd(a.per_task + t_cur, acc);
@@ -126,22 +134,58 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     acc_cyc = 0;
     run = 0;
   };
+
+  // second phase: spilled records and chunks, final once the first phase ended
+  uint32_t nrec = 0;
+  unsigned long long nchunks = 0;
+  if (a.phase) {
+    const unsigned long long sp = a.ctl->spill;
+    nrec = uint32_t(sp >> kSpillShift);
+    if (nrec > a.spill_cap) nrec = a.spill_cap;
+    nchunks = sp & ((1ull << kSpillShift) - 1);
+  }
+
   unsigned long long item = 0, batch_end = 0;
   for (;;) {
-    // Items are claimed kItemBatch at a time: one device-scope atomic per
-    // batch instead of per item.
-    if (item >= batch_end) {
-      unsigned long long b = 0;
-      if (lane == 0) b = atomicAdd(a.head, (unsigned long long)kItemBatch);
-      item = __shfl(b, 0, kWave);
-      batch_end = item + kItemBatch;
-      if (batch_end > a.item_end) batch_end = a.item_end;
+    uint64_t it;
+    uint32_t wlo, whi;
+    bool may_defer;
+    if (a.phase == 0) {
+      // Items are claimed kItemBatch at a time: one device-scope atomic per
+      // batch instead of per item.
+      if (item >= batch_end) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&a.ctl->head, (unsigned long long)kItemBatch);
+        item = __shfl(b, 0, kWave);
+        batch_end = item + kItemBatch;
+        if (batch_end > a.item_end) batch_end = a.item_end;
+      }
+      if (item >= a.item_end) break;
+      it = item++;
+      wlo = 0;
+      whi = 0xffffffffu;
+      may_defer = a.spill_cap != 0;
+    } else {
+      unsigned long long c = 0;
+      if (lane == 0) c = atomicAdd(&a.ctl->head2, 1ull);
+      c = __shfl(c, 0, kWave);
+      if (c >= nchunks || nrec == 0) break;
+      uint32_t lo = 0, hi = nrec;  // record holding chunk c: last base <= c
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.spill_base[mid] <= c) lo = mid;
+        else hi = mid;
+      }
+      const SpillRec R = a.spill[lo];
+      it = R.item;
+      wlo = R.w0 + uint32_t(c - a.spill_base[lo]) * R.wc;
+      whi = R.w1 - wlo > R.wc ? wlo + R.wc : R.w1;
+      may_defer = false;
     }
-    if (item >= a.item_end) break;
     const unsigned long long t_start = a.cycles ? clock64() : 0;
-    if (!(item >= t_lo && item < t_hi)) {
+    if (!(it >= t_lo && it < t_hi)) {
       flush();  // the finished task's count and progress, before moving on
-      t_cur = find_task(a.item_prefix, a.ntasks, item);
+      t_cur = find_task(a.item_prefix, a.ntasks, it);
       t_lo = a.item_prefix[t_cur];
       t_hi = a.item_prefix[t_cur + 1];
     }
@@ -161,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     const uint32_t* __restrict__ bc_d = a.dense + BC.dense;  // bc bitmap rows (dense, rdense)
 
     // one row per lane
-    const uint64_t z = (item - t_lo) * T.rows + uint64_t(lane);
+    const uint64_t z = (it - t_lo) * T.rows + uint64_t(lane);
     uint32_t ab0 = 0, dab = 0, ac0 = 0, dac = 0;
     if (uint32_t(lane) < T.rows && z < AB.nzr) {
       const uint32_t r = a.nzrows[AB.nz + z];
@@ -172,13 +216,160 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       if (dac == 0) dab = 0;
       if (dab == 0) dac = 0;
     }
-
-    // stage ac(u) of the rows that have wedges
-    const uint32_t inclA = wave_incl_scan(dac, lane);
-    const uint32_t exclA = inclA - dac;
+    // rows -> wedges (u, v), v in ab(u); the unit covers wedges [wlo, hi)
+    const uint32_t inclW = wave_incl_scan(dab, lane);
+    const uint32_t exclW = inclW - dab;
+    const uint32_t W = __shfl(inclW, kWave - 1, kWave);
+    const uint32_t hi = whi < W ? whi : W;
+    const bool active = dab != 0 && exclW < hi && inclW > wlo;
+    const uint32_t dact = active ? dac : 0;
+    // ac(u) entries of the rows the unit touches
+    const uint32_t inclA = wave_incl_scan(dact, lane);
+    const uint32_t exclA = inclA - dact;
     const uint32_t A = __shfl(inclA, kWave - 1, kWave);
-    const bool in_lds = bitmap || A <= uint32_t(kLdsWords);
-    if (in_lds) {
+
+    unsigned long long cnt = 0;   // wave-uniform (ballot counts)
+    unsigned long long lcnt = 0;  // per lane (dense kind)
+    uint32_t steps = 0, iters = 0;
+    bool stop = false;  // the rest of the unit was spilled
+
+    // Wedges [w_from, w_to) against ac(u) staged at seg (per row) in base.
+    auto wedges = [&](uint32_t w_from, uint32_t w_to, uint32_t seg, const uint32_t* base) {
+      for (uint32_t wb = w_from; wb < w_to; wb += kWave) {
+        const uint32_t w = wb + lane;
+        const bool valid = w < w_to;
+        const int q = owner_of(inclW, w);
+        const uint32_t q_ab0 = __shfl(ab0, q, kWave);
+        const uint32_t q_ex = __shfl(exclW, q, kWave);
+        const uint32_t q_seg = __shfl(seg, q, kWave);
+        const uint32_t q_dac = __shfl(dac, q, kWave);
+        if (dense) {
+          // |ac(u) ∩ bc(v)| = popcount of the AND of the two bitmap rows: one
+          // lane per wedge, 16 bytes per step from LDS and from HBM.
+          if (valid) {
+            const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
+            const uint4* __restrict__ brow = reinterpret_cast<const uint4*>(bc_d + uint64_t(v) * T.wpr);
+            const uint4* arow = reinterpret_cast<const uint4*>(lds + q_seg);
+            uint32_t c = 0;
+#pragma unroll 2
+            for (uint32_t j = 0; j < T.wpr / 4; ++j) {
+              const uint4 bv = brow[j];
+              const uint4 av = arow[j];
+              c += __popc(av.x & bv.x) + __popc(av.y & bv.y) + __popc(av.z & bv.z) + __popc(av.w & bv.w);
+            }
+            lcnt += c;
+          }
+          steps += 1 + T.wpr / 4;
+        } else {
+          // the list walked per wedge: bc(v) (search, bitmap) or ac(u) (rdense)
+          uint32_t b0 = 0, dlen = 0, v = 0;
+          if (valid) {
+            v = ab_c[q_ab0 + (w - q_ex)];
+            if (rdense) {
+              dlen = q_dac;
+            } else {
+              b0 = bc_rp[v];
+              dlen = bc_rp[v + 1] - b0;
+            }
+          }
+          const uint32_t inclP = wave_incl_scan(dlen, lane);
+          const uint32_t exclP = inclP - dlen;
+          const uint32_t Pn = __builtin_amdgcn_readlane(inclP, kWave - 1);
+          // entry p of this lane's wedge is list[my_d + p]; my_s locates the
+          // structure it is tested against
+          const uint32_t my_d = (rdense ? q_seg : b0) - exclP;
+          const uint32_t my_s = rdense ? v * T.wpr : q_seg;
+          // Each 64-entry step: wedges starting inside the step post their
+          // data at their start position; a lane's wedge is the last start at
+          // or before it (ballot + count of leading zeros), else the wedge
+          // carried over from the previous step.
+          uint32_t c_d = 0, c_s = 0, c_n = 0;
+          for (uint32_t pb = 0; pb < Pn; pb += kWave) {
+            mk_f[lane] = 0;
+            wave_lds_sync();
+            if (dlen != 0 && exclP >= pb && exclP < pb + kWave) {
+              const uint32_t st = exclP - pb;
+              mk_f[st] = 1;
+              mk_d[st] = my_d;
+              mk_s[st] = my_s;
+              mk_n[st] = q_dac;
+            }
+            wave_lds_sync();
+            const uint64_t upto = __ballot(mk_f[lane] != 0) & ((2ull << lane) - 1);
+            uint32_t d = c_d, sg = c_s, nn = c_n;
+            if (upto) {
+              const int hb = 63 - __clzll(upto);
+              d = mk_d[hb];
+              sg = mk_s[hb];
+              nn = mk_n[hb];
+            }
+            c_d = __builtin_amdgcn_readlane(d, kWave - 1);
+            c_s = __builtin_amdgcn_readlane(sg, kWave - 1);
+            c_n = __builtin_amdgcn_readlane(nn, kWave - 1);
+            const uint32_t p = pb + lane;
+            bool found = false;
+            if (p < Pn) {
+              if (rdense) {
+                const uint32_t x = base[d + p];
+                found = (bc_d[sg + (x >> 5)] >> (x & 31)) & 1u;
+              } else {
+                const uint32_t x = bc_c[d + p];
+                if (bitmap) found = (lds[sg + (x >> 5)] >> (x & 31)) & 1u;
+                else found = sorted_contains(base + sg, nn, x);
+              }
+            }
+            cnt += __popcll(__ballot(found));
+            wave_lds_sync();
+          }
+          steps += 1 + (Pn + kWave - 1) / kWave;
+        }
+        ++iters;
+        // Spill the rest of a heavy first-phase unit as second-phase chunks
+        // of about kChunkSteps steps each, sized from the steps per 64-wedge
+        // batch seen so far.
+        // first wedge not yet counted (a window may end inside this batch)
+        const uint32_t wn = wb + kWave < w_to ? wb + kWave : w_to;
+        if (may_defer && steps > kDeferSteps && wn < hi) {
+          may_defer = false;
+          const uint32_t per = steps / iters;
+          uint32_t k = kChunkSteps / (per ? per : 1);
+          k = k < 1 ? 1 : (k > 64 ? 64 : k);
+          const uint32_t wc = k * kWave;
+          const uint32_t nch = (hi - wn + wc - 1) / wc;
+          int ok = 0;
+          if (lane == 0) {
+            unsigned long long old = __hip_atomic_load(&a.ctl->spill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (;;) {
+              const unsigned long long r = old >> kSpillShift;
+              if (r >= a.spill_cap) break;
+              const unsigned long long prev =
+                  atomicCAS(&a.ctl->spill, old, old + (1ull << kSpillShift) + nch);
+              if (prev == old) {
+                SpillRec R;
+                R.item = it;
+                R.w0 = wn;
+                R.w1 = hi;
+                R.wc = wc;
+                R.pad = 0;
+                a.spill[r] = R;
+                a.spill_base[r] = old & ((1ull << kSpillShift) - 1);
+                if (a.left) atomicAdd(a.left + T.graph, (unsigned long long)nch);
+                ok = 1;
+                break;
+              }
+              old = prev;
+            }
+          }
+          if (__shfl(ok, 0, kWave)) {
+            stop = true;
+            break;
+          }
+        }
+      }
+    };
+
+    if (bitmap) {
+      // bitmaps of the touched rows (T.rows of them fit the slice)
       for (uint32_t s = 0; s < A; s += kWave) {
         const uint32_t e = s + lane;
         const int q = owner_of(inclA, e);
@@ -186,112 +377,13 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
         const uint32_t q_ex = __shfl(exclA, q, kWave);
         if (e < A) {
           const uint32_t x = ac_c[q_ac0 + (e - q_ex)];
-          if (bitmap) atomicOr(&lds[uint32_t(q) * T.wpr + (x >> 5)], 1u << (x & 31));
-          else lds[e] = x;
+          atomicOr(&lds[uint32_t(q) * T.wpr + (x >> 5)], 1u << (x & 31));
         }
       }
-    }
-    wave_lds_sync();
-    // where row `lane` keeps its membership structure
-    const uint32_t seg = bitmap ? uint32_t(lane) * T.wpr : (in_lds ? exclA : ac0);
-    const uint32_t* __restrict__ search_base = in_lds ? lds : ac_c;
-
-    // rows -> wedges -> probes
-    const uint32_t inclW = wave_incl_scan(dab, lane);
-    const uint32_t exclW = inclW - dab;
-    const uint32_t W = __shfl(inclW, kWave - 1, kWave);
-    unsigned long long cnt = 0;   // wave-uniform (ballot counts)
-    unsigned long long lcnt = 0;  // per lane (dense kind)
-    for (uint32_t wb = 0; wb < W; wb += kWave) {
-      const uint32_t w = wb + lane;
-      const int q = owner_of(inclW, w);
-      const uint32_t q_ab0 = __shfl(ab0, q, kWave);
-      const uint32_t q_ex = __shfl(exclW, q, kWave);
-      const uint32_t q_seg = __shfl(seg, q, kWave);
-      const uint32_t q_dac = __shfl(dac, q, kWave);
-      if (dense) {
-        // |ac(u) ∩ bc(v)| = popcount of the AND of the two bitmap rows: one
-        // lane per wedge, 16 bytes per step from LDS and from HBM.
-        if (w < W) {
-          const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
-          const uint4* __restrict__ brow = reinterpret_cast<const uint4*>(bc_d + uint64_t(v) * T.wpr);
-          const uint4* arow = reinterpret_cast<const uint4*>(lds + q_seg);
-          uint32_t c = 0;
-#pragma unroll 2
-          for (uint32_t j = 0; j < T.wpr / 4; ++j) {
-            const uint4 bv = brow[j];
-            const uint4 av = arow[j];
-            c += __popc(av.x & bv.x) + __popc(av.y & bv.y) + __popc(av.z & bv.z) + __popc(av.w & bv.w);
-          }
-          lcnt += c;
-        }
-        continue;
-      }
-      // the list walked per wedge: bc(v) (search, bitmap) or ac(u) (rdense)
-      uint32_t b0 = 0, dlen = 0, v = 0;
-      if (w < W) {
-        v = ab_c[q_ab0 + (w - q_ex)];
-        if (rdense) {
-          dlen = q_dac;
-        } else {
-          b0 = bc_rp[v];
-          dlen = bc_rp[v + 1] - b0;
-        }
-      }
-      const uint32_t inclP = wave_incl_scan(dlen, lane);
-      const uint32_t exclP = inclP - dlen;
-      const uint32_t Pn = __builtin_amdgcn_readlane(inclP, kWave - 1);
-      // entry p of this lane's wedge is list[my_d + p]; my_s locates the
-      // structure it is tested against
-      const uint32_t my_d = (rdense ? q_seg : b0) - exclP;
-      const uint32_t my_s = rdense ? v * T.wpr : q_seg;
-      // Each 64-entry step: wedges starting inside the step post their data at
-      // their start position; a lane's wedge is the last start at or before
-      // it (ballot + count of leading zeros), else the wedge carried over
-      // from the previous step.
-      uint32_t c_d = 0, c_s = 0, c_n = 0;
-      for (uint32_t pb = 0; pb < Pn; pb += kWave) {
-        mk_f[lane] = 0;
-        wave_lds_sync();
-        if (dlen != 0 && exclP >= pb && exclP < pb + kWave) {
-          const uint32_t st = exclP - pb;
-          mk_f[st] = 1;
-          mk_d[st] = my_d;
-          mk_s[st] = my_s;
-          mk_n[st] = q_dac;
-        }
-        wave_lds_sync();
-        const uint64_t upto = __ballot(mk_f[lane] != 0) & ((2ull << lane) - 1);
-        uint32_t d = c_d, sg = c_s, nn = c_n;
-        if (upto) {
-          const int hb = 63 - __clzll(upto);
-          d = mk_d[hb];
-          sg = mk_s[hb];
-          nn = mk_n[hb];
-        }
-        c_d = __builtin_amdgcn_readlane(d, kWave - 1);
-        c_s = __builtin_amdgcn_readlane(sg, kWave - 1);
-        c_n = __builtin_amdgcn_readlane(nn, kWave - 1);
-        const uint32_t p = pb + lane;
-        bool found = false;
-        if (p < Pn) {
-          if (rdense) {
-            const uint32_t x = search_base[d + p];
-            found = (bc_d[sg + (x >> 5)] >> (x & 31)) & 1u;
-          } else {
-            const uint32_t x = bc_c[d + p];
-            if (bitmap) found = (lds[sg + (x >> 5)] >> (x & 31)) & 1u;
-            else found = sorted_contains(search_base + sg, nn, x);
-          }
-        }
-        cnt += __popcll(__ballot(found));
-        wave_lds_sync();
-      }
-    }
-
-    // restore the all-zero slice that bitmap items rely on
-    wave_lds_sync();
-    if (bitmap) {
+      wave_lds_sync();
+      wedges(wlo, hi, uint32_t(lane) * T.wpr, lds);
+      // restore the all-zero slice
+      wave_lds_sync();
       for (uint32_t s = 0; s < A; s += kWave) {
         const uint32_t e = s + lane;
         const int q = owner_of(inclA, e);
@@ -302,15 +394,43 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
           lds[uint32_t(q) * T.wpr + (x >> 5)] = 0;
         }
       }
-    } else if (in_lds) {
-      for (uint32_t e = lane; e < A; e += kWave) lds[e] = 0;
+    } else {
+      // Sorted ac(u) lists, a window of consecutive rows at a time: as many
+      // rows as fit the slice together, or one row searched in HBM when its
+      // list alone is larger than the slice.
+      uint32_t a0 = 0, dirty = 0;
+      while (a0 < A && !stop) {
+        uint64_t win = __ballot(dact != 0 && exclA >= a0 && inclA - a0 <= uint32_t(kLdsWords));
+        const bool in_hbm = win == 0;
+        if (in_hbm) win = __ballot(dact != 0 && exclA == a0);
+        const int first = __ffsll((long long)win) - 1;
+        const int last = 63 - __clzll(win);
+        const uint32_t a1 = __shfl(inclA, last, kWave);
+        if (!in_hbm) {
+          for (uint32_t s = a0; s < a1; s += kWave) {
+            const uint32_t e = s + lane;
+            const int q = owner_of(inclA, e);
+            const uint32_t q_ac0 = __shfl(ac0, q, kWave);
+            const uint32_t q_ex = __shfl(exclA, q, kWave);
+            if (e < a1) lds[e - a0] = ac_c[q_ac0 + (e - q_ex)];
+          }
+          if (a1 - a0 > dirty) dirty = a1 - a0;
+        }
+        wave_lds_sync();
+        const uint32_t f_ex = __shfl(exclW, first, kWave);
+        const uint32_t l_in = __shfl(inclW, last, kWave);
+        wedges(f_ex > wlo ? f_ex : wlo, l_in < hi ? l_in : hi, in_hbm ? ac0 : exclA - a0,
+               in_hbm ? ac_c : lds);
+        wave_lds_sync();
+        a0 = a1;
+      }
+      for (uint32_t e = lane; e < dirty; e += kWave) lds[e] = 0;
     }
     wave_lds_sync();
     if (dense) cnt += wave_sum64(lcnt);
     acc += cnt;
     if (a.cycles) acc_cyc += clock64() - t_start;
     ++run;
-    ++item;
   }
   flush();
 }

@@ -58,7 +58,13 @@ struct bbtc_plan {
   std::vector<uint64_t> hprefix;        // host copy of item_prefix
   std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense)
   unsigned long long* per_task = nullptr;
-  unsigned long long* heads = nullptr;  // one work counter per launch slot
+  bbtc::SlotCtl* ctl = nullptr;         // one control block per launch slot
+  struct Spill {
+    bbtc::SpillRec* rec = nullptr;
+    uint64_t* base = nullptr;
+  };
+  std::vector<Spill> spill;             // per launch slot; slot 0 made with the plan
+  uint32_t spill_cap = 0;               // records per slot
   unsigned long long* cycles = nullptr; // per-task clock counters while profiling
   bool profiling = false;
   unsigned long long* track_left = nullptr;  // per-graph completion (batch-owned)

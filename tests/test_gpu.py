@@ -93,6 +93,36 @@ def test_random_graphs_vs_oracle(dev, seed):
         assert tot == rtot and np.array_equal(per, rper), form
 
 
+@pytest.mark.parametrize("part,p", [("uni", 1), ("pal", 2), ("n1d", 3)])
+def test_heavy_items_split(dev, part, p):
+    # RMAT s16 with one to three bands: hub-core items far above the per-item
+    # step budget (their wedges are spilled to the second phase in chunks) and
+    # ac(u) lists of 64 rows larger than the LDS slice (row windows)
+    g = B.Graph.rmat(16, 18, seed=1492)
+    cuts = g.partition(part, p)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    log("case heavy", part, p, "tasks", len(t))
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    for form, (per, tot) in all_forms(dev, L, t).items():
+        assert tot == rtot and np.array_equal(per, rper), form
+
+
+def test_complete_graph_rows_wider_than_lds(dev):
+    # K_2100: out-degrees up to 2099 exceed the 1792-word LDS slice, so the
+    # sorted-list forms search those rows in HBM
+    n = 2100
+    iu = np.triu_indices(n, 1)
+    g = B.Graph.from_edges(n, iu[0].astype(np.uint32), iu[1].astype(np.uint32))
+    expect = n * (n - 1) * (n - 2) // 6
+    for p in (1, 2):
+        L = B.Layout(g, g.partition("uni", p))
+        t = L.tasks("desc")
+        for form, (per, tot) in all_forms(dev, L, t).items():
+            assert tot == expect, (form, p)
+
+
 @pytest.mark.parametrize("sep_frac", [0.0, 0.5, 1.0, 2.0])
 def test_hybrid_border(dev, sep_frac):
     g = B.Graph.rmat(16, 16, seed=12)
