@@ -49,6 +49,23 @@ class bbtc_opts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64)]
 
 
+class bbtc_server_opts(C.Structure):
+    _fields_ = [("heap_bytes", C.c_uint64), ("quantum_items", C.c_uint64), ("max_inflight", C.c_uint32),
+                ("max_segments", C.c_uint32), ("sequential", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class bbtc_server_result(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("total", C.c_uint64), ("status", C.c_int32), ("launches", C.c_uint32),
+                ("bytes", C.c_uint64), ("arrival_s", C.c_double), ("admit_s", C.c_double),
+                ("ready_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double)]
+
+
+class bbtc_server_stats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("segments", C.c_uint64), ("items", C.c_uint64),
+                ("peak_resident_bytes", C.c_uint64), ("heap_bytes", C.c_uint64),
+                ("peak_active", C.c_uint32), ("peak_segments", C.c_uint32)]
+
+
 TASK_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("k", np.uint32)])
 
 _lib = None
@@ -119,6 +136,14 @@ def _declare(lib):
         "bbtc_batch_launch": (st, [vp]),
         "bbtc_batch_results": (st, [vp, vp, vp, vp]),
         "bbtc_batch_destroy": (None, [vp]),
+        "bbtc_server_create": (st, [vp, C.POINTER(bbtc_server_opts), pvp]),
+        "bbtc_server_reset_clock": (st, [vp]),
+        "bbtc_server_now": (C.c_double, [vp]),
+        "bbtc_server_submit": (st, [vp, vp, vp, u64, i32, C.c_double, vp, C.POINTER(u64)]),
+        "bbtc_server_poll": (st, [vp, C.POINTER(bbtc_server_result), u64, i32, C.POINTER(u64)]),
+        "bbtc_server_wait_idle": (st, [vp]),
+        "bbtc_server_get_stats": (st, [vp, C.POINTER(bbtc_server_stats)]),
+        "bbtc_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -535,6 +560,88 @@ class Batch:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.bbtc_batch_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self.destroy()
+
+
+# ---- serving -------------------------------------------------------------------
+@dataclass
+class ServerResult:
+    id: int
+    total: int
+    status: int
+    launches: int
+    bytes: int
+    arrival_s: float
+    admit_s: float
+    ready_s: float
+    start_s: float
+    done_s: float
+    per_task: np.ndarray = None
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.arrival_s
+
+
+class Server:
+    """Continuous-batching request loop on one device (bbtc_server_*)."""
+
+    def __init__(self, dev: "Device", heap_bytes: int = 0, quantum_items: int = 0, max_inflight: int = 0,
+                 max_segments: int = 0, sequential: bool = False):
+        o = bbtc_server_opts(heap_bytes, quantum_items, max_inflight, max_segments, int(sequential), 0)
+        h = C.c_void_p()
+        _check(lib().bbtc_server_create(dev._h, C.byref(o), C.byref(h)))
+        self._h = h
+        self.dev = dev
+        self._keep = {}  # id -> (layout, tasks, per-task buffer) until polled
+
+    def reset_clock(self):
+        _check(lib().bbtc_server_reset_clock(self._h))
+
+    def now(self) -> float:
+        return lib().bbtc_server_now(self._h)
+
+    def submit(self, layout: "Layout", tasks, arrival_s: float = 0.0, kernel: str = "auto") -> int:
+        t = tasks_array(tasks)
+        per = np.zeros(max(len(t), 1), dtype=np.uint64)
+        rid = C.c_uint64()
+        _check(lib().bbtc_server_submit(self._h, layout._h, _ptr(t), len(t), KERNEL[kernel], float(arrival_s),
+                                        _ptr(per), C.byref(rid)))
+        self._keep[rid.value] = (layout, t, per)
+        return rid.value
+
+    def poll(self, max_results: int = 1024, wait_ms: int = 0):
+        buf = (bbtc_server_result * max(max_results, 1))()
+        got = C.c_uint64()
+        _check(lib().bbtc_server_poll(self._h, buf, max_results, wait_ms, C.byref(got)))
+        out = []
+        for x in range(got.value):
+            r = buf[x]
+            layout, t, per = self._keep.pop(r.id)
+            out.append(ServerResult(r.id, r.total, r.status, r.launches, r.bytes, r.arrival_s, r.admit_s,
+                                    r.ready_s, r.start_s, r.done_s, per[: len(t)]))
+        return out
+
+    def wait_idle(self):
+        _check(lib().bbtc_server_wait_idle(self._h))
+
+    def drain(self):
+        """Waits for every outstanding request; returns their results."""
+        self.wait_idle()
+        return self.poll(max(len(self._keep), 1))
+
+    def stats(self) -> dict:
+        s = bbtc_server_stats()
+        _check(lib().bbtc_server_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def destroy(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bbtc_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
 
     def __del__(self):
         self.destroy()

@@ -61,7 +61,7 @@ struct SlotCtl {
   unsigned long long pad;
 };
 
-// Wedges [w0, w1) of plan item `item` (in the item's flattened wedge order),
+// Wedges [w0, w1) of launch-local item `item` (in its flattened wedge order),
 // counted by the second phase in chunks of wc wedges.
 struct SpillRec {
   uint64_t item;
@@ -70,25 +70,37 @@ struct SpillRec {
 };
 static_assert(sizeof(SpillRec) == 24, "SpillRec layout");
 
-struct CountArgs {
-  const uint32_t* __restrict__ rowptr;
-  const uint32_t* __restrict__ cols;
-  const uint32_t* __restrict__ nzrows;
-  const uint32_t* __restrict__ dense;
-  const TileDesc* __restrict__ tiles;
-  const TaskDesc* __restrict__ tasks;
-  const uint64_t* __restrict__ item_prefix;  // ntasks + 1 entries
-  uint32_t ntasks;
-  uint64_t item_end;          // items [*head at launch, item_end) are counted
+// One plan's item range in a launch. A launch counts the items of one or
+// more segments, numbered consecutively from 0 (launch-local items).
+struct SegDesc {
+  const uint32_t* rowptr;
+  const uint32_t* cols;
+  const uint32_t* nzrows;
+  const uint32_t* dense;
+  const TileDesc* tiles;
+  const TaskDesc* tasks;
+  const uint64_t* item_prefix;  // ntasks + 1 entries
   unsigned long long* per_task;
+  unsigned long long* cycles;   // per task: shader clocks its units took (NULL: off)
+  uint64_t item0, item1;        // plan items [item0, item1)
+  uint64_t launch0;             // launch-local index of item0
+  uint32_t ntasks;
+  uint32_t pad;
+};
+static_assert(sizeof(SegDesc) == 104, "SegDesc layout");
+
+struct CountArgs {
+  const SegDesc* __restrict__ segs;  // launch0 increasing
+  uint32_t nseg;
+  uint32_t phase;             // 0: items from ctl->head; 1: chunks from ctl->head2
+  uint64_t item_end;          // launch-local items [ctl->head at launch, item_end)
   SlotCtl* ctl;               // head set to the first item of the launch beforehand
   SpillRec* spill;            // spill_cap records (first phase writes, second reads)
   uint64_t* spill_base;       // first chunk of each record (increasing)
   uint32_t spill_cap;         // 0: no splitting
-  uint32_t phase;             // 0: items from ctl->head; 1: chunks from ctl->head2
-  unsigned long long* cycles; // per task: shader clocks its items took (NULL: off)
-  // Per-graph completion (batches; NULL: off): work items still to finish, and
-  // the device wall clock when a graph's last item finished.
+  uint32_t pad;
+  // Per-graph completion (batches; NULL: off): units still to finish, and
+  // the device wall clock when a graph's last unit finished.
   unsigned long long* left;
   unsigned long long* done;
 };

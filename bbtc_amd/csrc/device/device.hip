@@ -15,31 +15,120 @@
 #include <vector>
 
 #include "../host/device_api.hpp"
+#include "internal.hpp"
 #include "kernels.hpp"
-#include "objects.hpp"
 
 namespace bbtc {
 void set_last_error(const std::string& s);
 
+bool valid_kernel(bbtc_kernel k) {
+  return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
+         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE;
+}
+
+std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n) {
+  std::vector<TaskRef> r(n);
+  for (uint64_t q = 0; q < n; ++q) {
+    const bbtc_task& x = tasks[q];
+    r[q] = TaskRef{x.i * P + x.j, x.j * P + x.k, x.i * P + x.k, 0};
+  }
+  return r;
+}
+
+std::vector<TileDesc> tile_table(const Layout& L) {
+  const size_t T = size_t(L.P) * L.P;
+  std::vector<TileDesc> out(T);
+  for (size_t t = 0; t < T; ++t) {
+    TileDesc d{};
+    const uint32_t i = uint32_t(t / L.P), k = uint32_t(t % L.P);
+    d.h = L.height(i);
+    d.dense = ~0ull;
+    d.dwpr = L.dwpr[k];
+    if (i <= k) {
+      d.rp = L.rp_base[t];
+      d.col = L.col_base[t];
+      d.nz = L.nz_base[t];
+      d.nzr = L.nz_rows[t];
+      d.dense = L.dense_base[t];
+    }
+    out[t] = d;
+  }
+  return out;
+}
+
+void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s) {
+  const uint64_t zn = std::max<uint64_t>(n, 1);
+  hipLaunchKernelGGL(zero_counts, dim3(unsigned((zn + 255) / 256)), dim3(256), 0, s, p, n);
+  HIP_OK(hipGetLastError());
+}
+
+void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, unsigned blocks, hipStream_t s) {
+  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, i0);
+  HIP_OK(hipGetLastError());
+  a.phase = 0;
+  hipLaunchKernelGGL(count_probe, dim3(blocks), dim3(kBlock), 0, s, a);
+  HIP_OK(hipGetLastError());
+  // spilled chunks of heavy items, over the whole machine
+  a.phase = 1;
+  hipLaunchKernelGGL(count_probe, dim3(unsigned(resident_blocks(ctx))), dim3(kBlock), 0, s, a);
+  HIP_OK(hipGetLastError());
+}
+
+PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<float>& deg,
+                       const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice) {
+  const uint64_t n = refs.size();
+  PlanDesc P;
+  P.td.assign(std::max<uint64_t>(n, 1), TaskDesc{});
+  P.pref.assign(n + 1, 0);
+  P.hkind.reserve(n);
+  for (uint64_t q = 0; q < n; ++q) {
+    const TaskRef& x = refs[q];
+    TaskDesc d{};
+    d.ab = x.ab;
+    d.bc = x.bc;
+    d.ac = x.ac;
+    d.graph = x.graph;
+    // LDS form of ac(u): a bitmap over band k takes dwpr words per row; the
+    // wave slice holds kLdsWords words.
+    const uint32_t wpr = tiles[x.ac].dwpr;
+    const uint32_t fit = wpr ? uint32_t(kLdsWords) / wpr : uint32_t(kWave);
+    const bool has_dense = tiles[x.bc].dense != ~0ull;
+    bbtc_kernel k = kernel;
+    if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
+    else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
+    else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
+    else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
+    uint32_t kind = kKindSearch;
+    if (k == BBTC_KERNEL_BITMAP) {
+      if (fit >= 1) kind = kKindBitmap;
+    } else if (k == BBTC_KERNEL_DENSE) {
+      if (has_dense && fit >= 1) kind = kKindDense;
+      else if (fit >= 1) kind = kKindBitmap;
+    } else if (k == BBTC_KERNEL_RDENSE) {
+      if (has_dense) kind = kKindRdense;
+    } else if (k == BBTC_KERNEL_AUTO) {
+      // without measurements: the ac(u) list against the bc(v) bitmap row
+      // where the bc tile has one, else the ac(u) bitmap where it fits
+      if (has_dense) kind = kKindRdense;
+      else if (fit >= uint32_t(kAutoBitmapRows)) kind = kKindBitmap;
+    }
+    const bool use_bitmap = kind == kKindBitmap || kind == kKindDense;
+    d.kind = kind;
+    P.hkind.push_back(kind + 1);
+    d.rows = use_bitmap ? std::min<uint32_t>(fit, kWave) : uint32_t(kWave);
+    d.wpr = kind == kKindSearch ? 0 : wpr;
+    P.td[q] = d;
+    uint64_t items = 0;
+    if (!(deg[d.bc] == 0 || deg[d.ac] == 0 || deg[d.ab] == 0)) {
+      items = (uint64_t(tiles[d.ab].nzr) + d.rows - 1) / d.rows;
+      if (kind != kKindSearch) P.bitmap_tasks++;
+    }
+    P.pref[q + 1] = P.pref[q] + items;
+  }
+  return P;
+}
+
 namespace {
-
-template <class T>
-T* dev_alloc(bbtc_ctx* ctx, uint64_t count, uint64_t* acct) {
-  const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
-  if (ctx->budget && ctx->in_use + bytes > ctx->budget)
-    fail(BBTC_ERR_BUDGET, "device budget of " + std::to_string(ctx->budget) + " bytes exceeded");
-  void* p = nullptr;
-  HIP_OK(hipSetDevice(ctx->device));
-  HIP_OK(hipMalloc(&p, bytes));
-  ctx->in_use += bytes;
-  *acct += bytes;
-  return static_cast<T*>(p);
-}
-
-void dev_free(bbtc_ctx* ctx, void* p) {
-  if (p) (void)hipFree(p);
-  (void)ctx;
-}
 
 void release_graph(bbtc_dgraph* g) {
   if (!g) return;
@@ -56,6 +145,7 @@ void release_plan(bbtc_plan* p) {
   if (!p) return;
   dev_free(p->ctx, p->tasks);
   dev_free(p->ctx, p->item_prefix);
+  dev_free(p->ctx, p->segs);
   dev_free(p->ctx, p->per_task);
   dev_free(p->ctx, p->ctl);
   for (auto& sp : p->spill) {
@@ -67,22 +157,12 @@ void release_plan(bbtc_plan* p) {
   delete p;
 }
 
-bool skipped(const bbtc_dgraph* g, uint32_t ab, uint32_t bc, uint32_t ac) {
-  return g->deg[bc] == 0 || g->deg[ac] == 0 || g->deg[ab] == 0;
-}
-
-void zero(bbtc_plan* p, hipStream_t s) {
-  const uint64_t zn = std::max<uint64_t>(p->ntasks, 1);
-  hipLaunchKernelGGL(zero_counts, dim3(unsigned((zn + 255) / 256)), dim3(256), 0, s, p->per_task,
-                     p->ntasks);
-  HIP_OK(hipGetLastError());
-}
+void zero(bbtc_plan* p, hipStream_t s) { enqueue_zero(p->per_task, p->ntasks, s); }
 
 // Counts tasks [t0, t1) of the plan on stream s with work counter `slot`.
 // Launches on one slot must be ordered (same stream); distinct slots may run
 // concurrently.
 void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t s) {
-  bbtc_dgraph* g = p->g;
   const uint64_t i0 = p->hprefix[t0], i1 = p->hprefix[t1];
   p->last_stream = s;
   if (i1 <= i0) return;
@@ -92,37 +172,19 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
     sp.rec = dev_alloc<SpillRec>(p->ctx, p->spill_cap, &p->bytes);
     sp.base = dev_alloc<uint64_t>(p->ctx, p->spill_cap, &p->bytes);
   }
-  SlotCtl* ctl = p->ctl + slot;
-  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, ctl, (unsigned long long)i0);
-  HIP_OK(hipGetLastError());
-  CountArgs a;
-  a.rowptr = g->rowptr;
-  a.cols = g->cols;
-  a.nzrows = g->nzrows;
-  a.dense = g->dense;
-  a.tiles = g->tiles;
-  a.tasks = p->tasks;
-  a.item_prefix = p->item_prefix;
-  a.ntasks = uint32_t(p->ntasks);
+  CountArgs a{};
+  a.segs = p->segs + (p->profiling ? 1 : 0);
+  a.nseg = 1;
   a.item_end = i1;
-  a.per_task = p->per_task;
-  a.ctl = ctl;
+  a.ctl = p->ctl + slot;
   a.spill = sp.rec;
   a.spill_base = sp.base;
   a.spill_cap = p->spill_cap;
-  a.phase = 0;
-  a.cycles = p->profiling ? p->cycles : nullptr;
   a.left = p->track_left;
   a.done = p->track_done;
   const uint64_t want_blocks = (i1 - i0 + kWavesPerBlock - 1) / kWavesPerBlock;
-  const uint64_t resident = uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu);
-  const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident)));
-  hipLaunchKernelGGL(count_probe, dim3(blocks), dim3(kBlock), 0, s, a);
-  HIP_OK(hipGetLastError());
-  // spilled chunks of heavy items, over the whole machine
-  a.phase = 1;
-  hipLaunchKernelGGL(count_probe, dim3(unsigned(std::max<uint64_t>(1, resident))), dim3(kBlock), 0, s, a);
-  HIP_OK(hipGetLastError());
+  const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident_blocks(p->ctx))));
+  enqueue_count(p->ctx, a, i0, blocks, s);
 }
 
 void launch(bbtc_plan* p, hipStream_t s) {
@@ -136,29 +198,7 @@ void launch(bbtc_plan* p, hipStream_t s) {
 namespace bbtc {
 namespace {
 
-bool valid_kernel(bbtc_kernel k) {
-  return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
-         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE;
-}
-
-// A task named by its three tile ids in a graph's tile table, and the graph
-// it belongs to (0 for a single graph, the graph's index in a batch).
-struct TaskRef {
-  uint32_t ab, bc, ac, graph;
-};
-
-std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n) {
-  std::vector<TaskRef> r(n);
-  for (uint64_t q = 0; q < n; ++q) {
-    const bbtc_task& x = tasks[q];
-    r[q] = TaskRef{x.i * P + x.j, x.j * P + x.k, x.i * P + x.k, 0};
-  }
-  return r;
-}
-
-// Builds a plan over tile-addressed tasks: the form of each task (kernel
-// rule, or choice[q] = 1 merge, 2 bitmap, 3 dense, 4 rdense), its rows per
-// work item, and the item prefix over tasks.
+// Builds a plan over tile-addressed tasks of a resident graph.
 bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& refs, bbtc_kernel kernel,
                      const uint8_t* choice) {
   const uint64_t n = refs.size();
@@ -167,53 +207,11 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
   p->g = g;
   p->ntasks = n;
   try {
-    std::vector<TaskDesc> td(std::max<uint64_t>(n, 1));
-    std::vector<uint64_t> pref(n + 1, 0);
-    p->hkind.reserve(n);
-    for (uint64_t q = 0; q < n; ++q) {
-      const TaskRef& x = refs[q];
-      TaskDesc d{};
-      d.ab = x.ab;
-      d.bc = x.bc;
-      d.ac = x.ac;
-      d.graph = x.graph;
-      // LDS form of ac(u): a bitmap over band k takes dwpr words per row; the
-      // wave slice holds kLdsWords words.
-      const uint32_t wpr = g->htiles[x.ac].dwpr;
-      const uint32_t fit = wpr ? uint32_t(kLdsWords) / wpr : uint32_t(kWave);
-      const bool has_dense = g->htiles[x.bc].dense != ~0ull;
-      bbtc_kernel k = kernel;
-      if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
-      else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
-      else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
-      else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
-      uint32_t kind = kKindSearch;
-      if (k == BBTC_KERNEL_BITMAP) {
-        if (fit >= 1) kind = kKindBitmap;
-      } else if (k == BBTC_KERNEL_DENSE) {
-        if (has_dense && fit >= 1) kind = kKindDense;
-        else if (fit >= 1) kind = kKindBitmap;
-      } else if (k == BBTC_KERNEL_RDENSE) {
-        if (has_dense) kind = kKindRdense;
-      } else if (k == BBTC_KERNEL_AUTO) {
-        // without measurements: the ac(u) list against the bc(v) bitmap row
-        // where the bc tile has one, else the ac(u) bitmap where it fits
-        if (has_dense) kind = kKindRdense;
-        else if (fit >= uint32_t(kAutoBitmapRows)) kind = kKindBitmap;
-      }
-      const bool use_bitmap = kind == kKindBitmap || kind == kKindDense;
-      d.kind = kind;
-      p->hkind.push_back(kind + 1);
-      d.rows = use_bitmap ? std::min<uint32_t>(fit, kWave) : uint32_t(kWave);
-      d.wpr = kind == kKindSearch ? 0 : wpr;
-      td[q] = d;
-      uint64_t items = 0;
-      if (!skipped(g, d.ab, d.bc, d.ac)) {
-        items = (uint64_t(g->htiles[d.ab].nzr) + d.rows - 1) / d.rows;
-        if (kind != kKindSearch) p->bitmap_tasks++;
-      }
-      pref[q + 1] = pref[q] + items;
-    }
+    PlanDesc D = describe_plan(g->htiles, g->deg, refs, kernel, choice);
+    std::vector<TaskDesc>& td = D.td;
+    std::vector<uint64_t>& pref = D.pref;
+    p->hkind = std::move(D.hkind);
+    p->bitmap_tasks = D.bitmap_tasks;
     p->nitems = pref[n];
     p->hprefix = pref;
     HIP_OK(hipSetDevice(ctx->device));
@@ -225,8 +223,27 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
     p->spill.resize(1);
     p->spill[0].rec = dev_alloc<SpillRec>(ctx, p->spill_cap, &p->bytes);
     p->spill[0].base = dev_alloc<uint64_t>(ctx, p->spill_cap, &p->bytes);
+    p->cycles = dev_alloc<unsigned long long>(ctx, std::max<uint64_t>(n, 1), &p->bytes);
+    p->segs = dev_alloc<SegDesc>(ctx, 2, &p->bytes);
+    SegDesc sd[2] = {};
+    sd[0].rowptr = g->rowptr;
+    sd[0].cols = g->cols;
+    sd[0].nzrows = g->nzrows;
+    sd[0].dense = g->dense;
+    sd[0].tiles = g->tiles;
+    sd[0].tasks = p->tasks;
+    sd[0].item_prefix = p->item_prefix;
+    sd[0].per_task = p->per_task;
+    sd[0].cycles = nullptr;
+    sd[0].item0 = 0;
+    sd[0].item1 = p->nitems;
+    sd[0].launch0 = 0;
+    sd[0].ntasks = uint32_t(n);
+    sd[1] = sd[0];
+    sd[1].cycles = p->cycles;
     HIP_OK(hipMemcpyAsync(p->tasks, td.data(), td.size() * sizeof(TaskDesc), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(p->item_prefix, pref.data(), pref.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(p->segs, sd, sizeof(sd), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
   } catch (...) {
     release_plan(p);
@@ -439,22 +456,7 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
   g->P = L.P;
   try {
     const size_t T = size_t(L.P) * L.P;
-    g->htiles.resize(T);
-    for (size_t t = 0; t < T; ++t) {
-      bbtc::TileDesc d{};
-      const uint32_t i = uint32_t(t / L.P), k = uint32_t(t % L.P);
-      d.h = L.height(i);
-      d.dense = ~0ull;
-      d.dwpr = L.dwpr[k];
-      if (i <= k) {
-        d.rp = L.rp_base[t];
-        d.col = L.col_base[t];
-        d.nz = L.nz_base[t];
-        d.nzr = L.nz_rows[t];
-        d.dense = L.dense_base[t];
-      }
-      g->htiles[t] = d;
-    }
+    g->htiles = bbtc::tile_table(L);
     g->deg = L.deg;
     g->dwpr = L.dwpr;
     g->rowptr = bbtc::dev_alloc<uint32_t>(ctx, L.rowptr.size(), &g->bytes);

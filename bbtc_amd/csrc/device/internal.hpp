@@ -1,0 +1,70 @@
+// Internals shared by the device translation units (device.hip, server.hip):
+// device allocation, the host description of a plan, and the counting
+// launch sequence. The kernels themselves are compiled in device.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "objects.hpp"
+
+namespace bbtc {
+
+template <class T>
+T* dev_alloc(bbtc_ctx* ctx, uint64_t count, uint64_t* acct) {
+  const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+  if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+    fail(BBTC_ERR_BUDGET, "device budget of " + std::to_string(ctx->budget) + " bytes exceeded");
+  void* p = nullptr;
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipMalloc(&p, bytes));
+  ctx->in_use += bytes;
+  *acct += bytes;
+  return static_cast<T*>(p);
+}
+
+inline void dev_free(bbtc_ctx* ctx, void* p) {
+  if (p) (void)hipFree(p);
+  (void)ctx;
+}
+
+bool valid_kernel(bbtc_kernel k);
+
+// A task named by its three tile ids in a graph's tile table, and the graph
+// it belongs to (0 for a single graph, the graph's index in a batch).
+struct TaskRef {
+  uint32_t ab, bc, ac, graph;
+};
+
+std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n);
+
+// Device tile table of a layout (P*P entries, offsets into its arrays).
+std::vector<TileDesc> tile_table(const Layout& L);
+
+// Host side of a plan: the form and rows per item of each task (kernel rule,
+// or choice[q] = 1 merge, 2 bitmap, 3 dense, 4 rdense) and the item prefix.
+struct PlanDesc {
+  std::vector<TaskDesc> td;       // max(n, 1) entries
+  std::vector<uint64_t> pref;     // n + 1 entries
+  std::vector<uint32_t> hkind;    // form per task, 1-based as in choice
+  uint64_t bitmap_tasks = 0;
+};
+PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<float>& deg,
+                       const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice);
+
+// Resident blocks of count_probe on the context's device.
+inline uint64_t resident_blocks(const bbtc_ctx* ctx) {
+  return std::max<uint64_t>(1, uint64_t(ctx->num_cu) * uint64_t(ctx->blocks_per_cu));
+}
+
+// Enqueues one counting launch on s: reset of the slot (first item i0), the
+// first phase over `blocks` blocks, the second over the whole machine.
+void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, unsigned blocks, hipStream_t s);
+
+// Enqueues p[0..n) = 0 on s.
+void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s);
+
+}  // namespace bbtc

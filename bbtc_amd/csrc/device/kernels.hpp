@@ -1,11 +1,13 @@
-// Device code: task descriptors and the counting kernels for gfx950.
+// Device code: the counting kernels for gfx950.
 //
 // Work decomposition. A plan lists work items; item x of task t covers
 // rows [x*R, (x+1)*R) of the ab tile's compact non-empty row list (R <= 64,
-// per task). One wave owns one item at a time and draws the next from an
-// atomic head counter, so a single persistent launch drains every task.
+// per task). A launch counts one or more segments (item ranges of plans,
+// possibly of different graphs) numbered consecutively. One wave owns one
+// unit at a time and draws the next from an atomic head counter, so a single
+// persistent launch drains every task of every segment.
 //
-// Inside an item the wave flattens its work twice with wave prefix sums:
+// Inside a unit the wave flattens its work twice with wave prefix sums:
 //   rows  -> wedges (u, v), v in ab(u), for rows whose ac(u) is non-empty
 //   wedges -> probes (u, v, w), w in bc(v)
 // so each lane handles one probe per step, lanes read bc(v) lists
@@ -13,12 +15,20 @@
 // serialising one thread as the reference kernels do
 // (gpu_sparse_kernel.cu:25-46, gpu_dense_kernel.cu:74-93).
 //
-// Membership of w in ac(u) is tested against ac(u) staged in the wave's LDS
-// slice in one of two forms, chosen per task:
-//   search (d): the sorted list itself; a probe is a binary search.
-//   bitmap (c): one bit per column of band k per row; a probe is one bit test.
+// Two phases. The first takes whole items; an item that runs past a step
+// budget spills its remaining wedges as chunks, which the second phase
+// spreads over the whole machine, so a hub-core item never leaves one wave
+// running alone at the end of a launch.
+//
+// Membership of w in ac(u) is tested in one of four forms, chosen per task:
+//   search: ac(u) as a sorted list in the wave's LDS slice (a window of rows
+//           at a time); a probe is a binary search.
+//   bitmap: ac(u) as one bit per column of band k in LDS; a probe is one bit
+//           test.
+//   dense:  ac(u) bitmap in LDS AND the bc(v) bitmap row from HBM, popcount.
+//   rdense: ac(u) entries tested against the bc(v) bitmap row in HBM.
 // Matches are counted with ballot + popcount into a wave-uniform 64-bit count
-// and added to the task's slot with one atomic per item.
+// and added to the task's slot with one atomic per task change.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -116,17 +126,21 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   for (int x = lane; x < kLdsWords; x += kWave) lds[x] = 0;
   wave_lds_sync();
 
+  // current segment and task; [t_lo, t_hi) are t_cur's launch-local items
+  uint32_t s_cur = 0;
   uint32_t t_cur = 0;
-  uint64_t t_lo = 1, t_hi = 0;   // item range of t_cur (empty at start)
+  uint64_t t_lo = 1, t_hi = 0;   // (empty at start)
+  uint64_t t_first = 0;          // plan item of t_cur's first item
   unsigned long long acc = 0;    // count of t_cur not yet added to its slot
   unsigned long long acc_cyc = 0;
   unsigned long long run = 0;    // units of t_cur finished since the last flush
   auto flush = [&]() {
     if (lane == 0) {
-      if (acc) atomicAdd(a.per_task + t_cur, acc);
-      if (acc_cyc) atomicAdd(a.cycles + t_cur, acc_cyc);
+      const SegDesc& S = a.segs[s_cur];
+      if (acc) atomicAdd(S.per_task + t_cur, acc);
+      if (acc_cyc) atomicAdd(S.cycles + t_cur, acc_cyc);
       if (a.left && run) {
-        const uint32_t gi = a.tasks[t_cur].graph;
+        const uint32_t gi = S.tasks[t_cur].graph;
         if (atomicSub(a.left + gi, run) == run) a.done[gi] = wall_clock64();
       }
     }
@@ -182,33 +196,49 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       whi = R.w1 - wlo > R.wc ? wlo + R.wc : R.w1;
       may_defer = false;
     }
-    const unsigned long long t_start = a.cycles ? clock64() : 0;
     if (!(it >= t_lo && it < t_hi)) {
       flush();  // the finished task's count and progress, before moving on
-      t_cur = find_task(a.item_prefix, a.ntasks, it);
-      t_lo = a.item_prefix[t_cur];
-      t_hi = a.item_prefix[t_cur + 1];
+      uint32_t sg = 0;
+      if (a.nseg > 1) {  // last segment starting at or before it
+        uint32_t lo = 0, hi = a.nseg;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (a.segs[mid].launch0 <= it) lo = mid;
+          else hi = mid;
+        }
+        sg = lo;
+      }
+      s_cur = __builtin_amdgcn_readfirstlane(sg);
+      const SegDesc& S = a.segs[s_cur];
+      const uint64_t pi = S.item0 + (it - S.launch0);
+      t_cur = find_task(S.item_prefix, S.ntasks, pi);
+      t_first = S.item_prefix[t_cur];
+      const uint64_t e = S.item_prefix[t_cur + 1];
+      t_lo = S.launch0 + ((t_first > S.item0 ? t_first : S.item0) - S.item0);
+      t_hi = S.launch0 + ((e < S.item1 ? e : S.item1) - S.item0);
     }
-    const TaskDesc T = a.tasks[t_cur];
-    const TileDesc AB = a.tiles[T.ab];
-    const TileDesc BC = a.tiles[T.bc];
-    const TileDesc AC = a.tiles[T.ac];
-    const uint32_t* __restrict__ ab_rp = a.rowptr + AB.rp;
-    const uint32_t* __restrict__ ac_rp = a.rowptr + AC.rp;
-    const uint32_t* __restrict__ bc_rp = a.rowptr + BC.rp;
-    const uint32_t* __restrict__ ab_c = a.cols + AB.col;
-    const uint32_t* __restrict__ ac_c = a.cols + AC.col;
-    const uint32_t* __restrict__ bc_c = a.cols + BC.col;
+    const SegDesc& S = a.segs[s_cur];
+    const unsigned long long t_start = S.cycles ? clock64() : 0;
+    const TaskDesc T = S.tasks[t_cur];
+    const TileDesc AB = S.tiles[T.ab];
+    const TileDesc BC = S.tiles[T.bc];
+    const TileDesc AC = S.tiles[T.ac];
+    const uint32_t* __restrict__ ab_rp = S.rowptr + AB.rp;
+    const uint32_t* __restrict__ ac_rp = S.rowptr + AC.rp;
+    const uint32_t* __restrict__ bc_rp = S.rowptr + BC.rp;
+    const uint32_t* __restrict__ ab_c = S.cols + AB.col;
+    const uint32_t* __restrict__ ac_c = S.cols + AC.col;
+    const uint32_t* __restrict__ bc_c = S.cols + BC.col;
     const bool dense = T.kind == kKindDense;
     const bool rdense = T.kind == kKindRdense;
     const bool bitmap = T.kind == kKindBitmap || dense;  // ac(u) staged as bitmaps
-    const uint32_t* __restrict__ bc_d = a.dense + BC.dense;  // bc bitmap rows (dense, rdense)
+    const uint32_t* __restrict__ bc_d = S.dense + BC.dense;  // bc bitmap rows (dense, rdense)
 
     // one row per lane
-    const uint64_t z = (it - t_lo) * T.rows + uint64_t(lane);
+    const uint64_t z = (S.item0 + (it - S.launch0) - t_first) * T.rows + uint64_t(lane);
     uint32_t ab0 = 0, dab = 0, ac0 = 0, dac = 0;
     if (uint32_t(lane) < T.rows && z < AB.nzr) {
-      const uint32_t r = a.nzrows[AB.nz + z];
+      const uint32_t r = S.nzrows[AB.nz + z];
       ab0 = ab_rp[r];
       dab = ab_rp[r + 1] - ab0;
       ac0 = ac_rp[r];
@@ -429,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     wave_lds_sync();
     if (dense) cnt += wave_sum64(lcnt);
     acc += cnt;
-    if (a.cycles) acc_cyc += clock64() - t_start;
+    if (S.cycles) acc_cyc += clock64() - t_start;
     ++run;
   }
   flush();

@@ -55,6 +55,7 @@ struct bbtc_plan {
   uint64_t bytes = 0;
   bbtc::TaskDesc* tasks = nullptr;
   uint64_t* item_prefix = nullptr;
+  bbtc::SegDesc* segs = nullptr;        // the whole plan: [0] plain, [1] with clock counters
   std::vector<uint64_t> hprefix;        // host copy of item_prefix
   std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense)
   unsigned long long* per_task = nullptr;

@@ -1,0 +1,664 @@
+// Serving: count requests for many graphs on one device with continuous
+// batching.
+//
+// A request carries a host block CSR and its task list. It arrives at a
+// given time and is admitted, in arrival order, once its device footprint
+// fits the server's heap. The footprint is the block CSR (row offsets,
+// columns, non-empty rows, bitmap rows of dense tiles) plus the plan's task
+// table and counters. An upload thread copies admitted graphs into the heap
+// on a copy stream while the device counts earlier requests.
+//
+// The scheduler thread keeps up to `max_inflight` launches queued on the
+// compute stream. Each launch is one count_probe launch pair whose segments
+// are a slice (at most `quantum_items` work items) of every resident request
+// with work left, so a request that arrives while others run joins the next
+// launch and shares the machine with them. A request finishes with the launch
+// that holds its last slice. Its counters are read back and its heap block
+// returns to the heap, which lets the next arrival in.
+//
+// This replaces the reference's per-run task loop (bb_tc.cpp:142-181,
+// 200-252), which counts one graph per process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace bbtc {
+void set_last_error(const std::string& s);
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kAlign = 256;
+constexpr uint64_t kDefaultHeap = 200ull << 30;
+constexpr uint64_t kDefaultQuantum = 1ull << 19;
+constexpr uint32_t kDefaultInflight = 2;
+constexpr uint32_t kDefaultSegments = 1024;
+
+inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over one device range, with coalescing frees.
+class Heap {
+ public:
+  void init(uint64_t size) {
+    free_.clear();
+    if (size) free_[0] = size;
+    size_ = size;
+  }
+  // Offset of a block of `bytes`, or ~0 when no free range is large enough.
+  uint64_t alloc(uint64_t bytes) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const uint64_t off = it->first, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      used_ += bytes;
+      return off;
+    }
+    return ~0ull;
+  }
+  void release(uint64_t off, uint64_t bytes) {
+    used_ -= bytes;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + bytes == next->first) {
+      bytes += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += bytes;
+        return;
+      }
+    }
+    free_[off] = bytes;
+  }
+  uint64_t used() const { return used_; }
+  uint64_t size() const { return size_; }
+
+ private:
+  std::map<uint64_t, uint64_t> free_;
+  uint64_t size_ = 0, used_ = 0;
+};
+
+struct Request {
+  uint64_t id = 0;
+  const Layout* L = nullptr;
+  std::vector<TileDesc> tiles;
+  PlanDesc plan;
+  uint64_t ntasks = 0, nitems = 0;
+  uint64_t* per_task_out = nullptr;
+  double arrival = 0;
+  // placement of the arrays inside the request's heap block
+  uint64_t o_rp = 0, o_col = 0, o_nz = 0, o_dense = 0, o_tiles = 0, o_tasks = 0, o_pref = 0, o_cnt = 0;
+  uint64_t bytes = 0;
+  uint64_t heap_off = ~0ull;
+  hipEvent_t uploaded = nullptr;
+  SegDesc seg{};
+  uint64_t next = 0;  // next item to put in a launch
+  uint32_t launches = 0;
+  double admit_s = 0, ready_s = 0, start_s = -1, done_s = 0;
+  bbtc_status status = BBTC_OK;
+  uint64_t total = 0;
+};
+
+struct Launch {
+  hipEvent_t done = nullptr;
+  int ring = 0;
+  std::vector<std::shared_ptr<Request>> finishing;
+};
+
+}  // namespace
+}  // namespace bbtc
+
+struct bbtc_server {
+  bbtc_ctx* ctx = nullptr;
+  uint64_t quantum = 0;
+  uint32_t max_inflight = 0, max_segments = 0;
+  bool sequential = false;
+
+  char* heap_base = nullptr;
+  bbtc::Heap heap;
+  hipStream_t copy = nullptr, compute = nullptr, readback = nullptr;
+  bbtc::SlotCtl* ctl = nullptr;
+  bbtc::SpillRec* spill = nullptr;
+  uint64_t* spill_base = nullptr;
+  std::vector<bbtc::SegDesc*> seg_dev;   // per ring slot
+  std::vector<bbtc::SegDesc*> seg_host;  // pinned, per ring slot
+  std::vector<hipEvent_t> event_pool;
+
+  std::mutex mu;
+  std::condition_variable cv_admit, cv_sched, cv_done;
+  std::deque<std::shared_ptr<bbtc::Request>> pending;  // arrival order
+  std::deque<std::shared_ptr<bbtc::Request>> ready;    // uploading or uploaded
+  std::deque<bbtc_server_result> results;
+  uint64_t next_id = 1;
+  uint64_t submitted = 0, finished = 0;
+  uint64_t active = 0;  // admitted and not finished
+  bool stop = false;
+  bbtc_status fatal = BBTC_OK;
+  std::string fatal_msg;
+  bbtc_server_stats stats{};
+  bbtc::Clock::time_point epoch;
+
+  std::thread admit_thread, sched_thread;
+
+  double now() const { return std::chrono::duration<double>(bbtc::Clock::now() - epoch).count(); }
+};
+
+namespace bbtc {
+namespace {
+
+void set_fatal(bbtc_server* S, bbtc_status st, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->fatal == BBTC_OK) {
+    S->fatal = st;
+    S->fatal_msg = msg;
+  }
+  S->stop = true;
+  S->cv_admit.notify_all();
+  S->cv_sched.notify_all();
+  S->cv_done.notify_all();
+}
+
+// Places the request's arrays in one block and returns its size.
+uint64_t place(Request& r) {
+  const Layout& L = *r.L;
+  uint64_t o = 0;
+  auto put = [&](uint64_t& at, uint64_t bytes) {
+    at = o;
+    o += align_up(std::max<uint64_t>(bytes, 1));
+  };
+  put(r.o_rp, L.rowptr.size() * 4);
+  put(r.o_col, L.cols.size() * 4);
+  put(r.o_nz, L.nzrows.size() * 4);
+  put(r.o_dense, L.dense.size() * 4);
+  put(r.o_tiles, r.tiles.size() * sizeof(TileDesc));
+  put(r.o_tasks, r.plan.td.size() * sizeof(TaskDesc));
+  put(r.o_pref, r.plan.pref.size() * 8);
+  put(r.o_cnt, std::max<uint64_t>(r.ntasks, 1) * 8);
+  return o;
+}
+
+void push_result(bbtc_server* S, const Request& r) {
+  bbtc_server_result x{};
+  x.id = r.id;
+  x.total = r.total;
+  x.status = r.status;
+  x.launches = r.launches;
+  x.bytes = r.bytes;
+  x.arrival_s = r.arrival;
+  x.admit_s = r.admit_s;
+  x.ready_s = r.ready_s;
+  x.start_s = r.start_s < 0 ? r.ready_s : r.start_s;
+  x.done_s = r.done_s;
+  S->results.push_back(x);
+  S->finished++;
+}
+
+// Upload thread: admits arrived requests in arrival order while their
+// footprint fits the heap, and copies each into its block.
+void admit_loop(bbtc_server* S) {
+  try {
+    HIP_OK(hipSetDevice(S->ctx->device));
+    for (;;) {
+      std::shared_ptr<Request> r;
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        for (;;) {
+          if (S->stop && S->pending.empty()) return;
+          if (S->stop) {  // shutting down: cancel what was not admitted
+            while (!S->pending.empty()) {
+              auto q = S->pending.front();
+              S->pending.pop_front();
+              q->status = BBTC_ERR_INVALID;
+              q->done_s = S->now();
+              push_result(S, *q);
+            }
+            S->cv_done.notify_all();
+            return;
+          }
+          if (S->pending.empty()) {
+            S->cv_admit.wait(lk);
+            continue;
+          }
+          auto& head = S->pending.front();
+          const double t = S->now();
+          if (head->arrival > t) {
+            S->cv_admit.wait_for(lk, std::chrono::duration<double>(head->arrival - t));
+            continue;
+          }
+          if (S->sequential && S->active > 0) {  // plain mode: one request at a time
+            S->cv_admit.wait(lk);
+            continue;
+          }
+          const uint64_t off = S->heap.alloc(head->bytes);
+          if (off == ~0ull) {  // wait for a finished request to free memory
+            S->cv_admit.wait(lk);
+            continue;
+          }
+          r = head;
+          S->pending.pop_front();
+          r->heap_off = off;
+          r->admit_s = t;
+          S->active++;
+          S->stats.peak_resident_bytes = std::max<uint64_t>(S->stats.peak_resident_bytes, S->heap.used());
+          S->stats.peak_active = std::max<uint32_t>(S->stats.peak_active, uint32_t(S->active));
+          break;
+        }
+      }
+      // copy the block CSR and the plan into the block
+      const Layout& L = *r->L;
+      char* b = S->heap_base + r->heap_off;
+      hipStream_t s = S->copy;
+      HIP_OK(hipMemcpyAsync(b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(b + r->o_col, L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(b + r->o_nz, L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(b + r->o_dense, L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(b + r->o_tiles, r->tiles.data(), r->tiles.size() * sizeof(TileDesc),
+                            hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(b + r->o_tasks, r->plan.td.data(), r->plan.td.size() * sizeof(TaskDesc),
+                            hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(b + r->o_pref, r->plan.pref.data(), r->plan.pref.size() * 8, hipMemcpyHostToDevice, s));
+      auto* cnt = reinterpret_cast<unsigned long long*>(b + r->o_cnt);
+      enqueue_zero(cnt, r->ntasks, s);
+      SegDesc& g = r->seg;
+      g.rowptr = reinterpret_cast<const uint32_t*>(b + r->o_rp);
+      g.cols = reinterpret_cast<const uint32_t*>(b + r->o_col);
+      g.nzrows = reinterpret_cast<const uint32_t*>(b + r->o_nz);
+      g.dense = reinterpret_cast<const uint32_t*>(b + r->o_dense);
+      g.tiles = reinterpret_cast<const TileDesc*>(b + r->o_tiles);
+      g.tasks = reinterpret_cast<const TaskDesc*>(b + r->o_tasks);
+      g.item_prefix = reinterpret_cast<const uint64_t*>(b + r->o_pref);
+      g.per_task = cnt;
+      g.cycles = nullptr;
+      g.ntasks = uint32_t(r->ntasks);
+      HIP_OK(hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming));
+      HIP_OK(hipEventRecord(r->uploaded, s));
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->ready.push_back(r);
+      }
+      S->cv_sched.notify_all();
+    }
+  } catch (const Error& e) {
+    set_fatal(S, e.code, e.what());
+  } catch (const std::exception& e) {
+    set_fatal(S, BBTC_ERR_INTERNAL, e.what());
+  }
+}
+
+// Reads back a finished request's counters and frees its block.
+void finish(bbtc_server* S, Request& r) {
+  std::vector<uint64_t> per(std::max<uint64_t>(r.ntasks, 1), 0);
+  if (r.ntasks) {
+    HIP_OK(hipMemcpyAsync(per.data(), r.seg.per_task, r.ntasks * 8, hipMemcpyDeviceToHost, S->readback));
+    HIP_OK(hipStreamSynchronize(S->readback));
+  }
+  uint64_t tot = 0;
+  for (uint64_t t = 0; t < r.ntasks; ++t) tot += per[t];
+  r.total = tot;
+  if (r.per_task_out && r.ntasks) std::copy(per.begin(), per.begin() + r.ntasks, r.per_task_out);
+  if (r.uploaded) (void)hipEventDestroy(r.uploaded);
+  r.uploaded = nullptr;
+  std::lock_guard<std::mutex> lk(S->mu);
+  r.done_s = S->now();
+  S->heap.release(r.heap_off, r.bytes);
+  S->active--;
+  push_result(S, r);
+  S->cv_admit.notify_all();
+  S->cv_done.notify_all();
+}
+
+// Scheduler thread: builds and queues launches, retires finished ones.
+void sched_loop(bbtc_server* S) {
+  try {
+    HIP_OK(hipSetDevice(S->ctx->device));
+    std::vector<std::shared_ptr<Request>> act;  // resident, in admission order
+    std::deque<Launch> inflight;
+    std::vector<int> ring_free;
+    for (int i = int(S->max_inflight) - 1; i >= 0; --i) ring_free.push_back(i);
+    for (;;) {
+      bool progress = false;
+      // newly uploaded requests
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        while (!S->ready.empty()) {
+          auto& r = S->ready.front();
+          const hipError_t e = hipEventQuery(r->uploaded);
+          if (e == hipErrorNotReady) break;
+          HIP_OK(e);
+          r->ready_s = S->now();
+          act.push_back(r);
+          S->ready.pop_front();
+          progress = true;
+        }
+      }
+      // requests without work finish at once
+      for (size_t x = 0; x < act.size();) {
+        if (act[x]->nitems == 0) {
+          finish(S, *act[x]);
+          act.erase(act.begin() + long(x));
+          progress = true;
+        } else {
+          ++x;
+        }
+      }
+      // retire launches in order
+      while (!inflight.empty()) {
+        const hipError_t e = hipEventQuery(inflight.front().done);
+        if (e == hipErrorNotReady) break;
+        HIP_OK(e);
+        Launch l = std::move(inflight.front());
+        inflight.pop_front();
+        for (auto& r : l.finishing) {
+          finish(S, *r);
+          act.erase(std::find(act.begin(), act.end(), r));
+        }
+        S->event_pool.push_back(l.done);
+        ring_free.push_back(l.ring);
+        progress = true;
+      }
+      // queue a launch over every request with items left
+      if (!ring_free.empty()) {
+        const int ring = ring_free.back();
+        SegDesc* sh = S->seg_host[ring];
+        uint32_t nseg = 0;
+        uint64_t total = 0;
+        Launch l;
+        l.ring = ring;
+        const double t = S->now();
+        for (auto& r : act) {
+          if (r->next >= r->nitems) continue;
+          if (nseg == S->max_segments) break;
+          const uint64_t take = S->sequential ? r->nitems - r->next : std::min(r->nitems - r->next, S->quantum);
+          SegDesc d = r->seg;
+          d.item0 = r->next;
+          d.item1 = r->next + take;
+          d.launch0 = total;
+          sh[nseg++] = d;
+          total += take;
+          r->next += take;
+          r->launches++;
+          if (r->start_s < 0) r->start_s = t;
+          if (r->next == r->nitems) l.finishing.push_back(r);
+        }
+        if (nseg) {
+          ring_free.pop_back();
+          hipStream_t s = S->compute;
+          HIP_OK(hipMemcpyAsync(S->seg_dev[ring], sh, nseg * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+          CountArgs a{};
+          a.segs = S->seg_dev[ring];
+          a.nseg = nseg;
+          a.item_end = total;
+          a.ctl = S->ctl;
+          a.spill = S->spill;
+          a.spill_base = S->spill_base;
+          a.spill_cap = kSpillCap;
+          const uint64_t want = (total + kWavesPerBlock - 1) / kWavesPerBlock;
+          const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(S->ctx))));
+          enqueue_count(S->ctx, a, 0, blocks, s);
+          if (S->event_pool.empty()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            S->event_pool.push_back(e);
+          }
+          l.done = S->event_pool.back();
+          S->event_pool.pop_back();
+          HIP_OK(hipEventRecord(l.done, s));
+          inflight.push_back(std::move(l));
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->stats.launches++;
+          S->stats.segments += nseg;
+          S->stats.items += total;
+          S->stats.peak_segments = std::max<uint32_t>(S->stats.peak_segments, nseg);
+          progress = true;
+        }
+      }
+      if (progress) continue;
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        if (S->stop && S->pending.empty() && S->ready.empty() && act.empty() && inflight.empty()) return;
+        if (inflight.empty() && S->ready.empty()) {
+          S->cv_sched.wait_for(lk, std::chrono::milliseconds(2));
+          continue;
+        }
+      }
+      // launches or uploads outstanding: poll shortly
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  } catch (const Error& e) {
+    set_fatal(S, e.code, e.what());
+  } catch (const std::exception& e) {
+    set_fatal(S, BBTC_ERR_INTERNAL, e.what());
+  }
+}
+
+void server_free(bbtc_server* S) {
+  if (!S) return;
+  (void)hipSetDevice(S->ctx->device);
+  for (auto* p : S->seg_dev) (void)hipFree(p);
+  for (auto* p : S->seg_host) (void)hipHostFree(p);
+  for (auto e : S->event_pool) (void)hipEventDestroy(e);
+  if (S->heap_base) (void)hipFree(S->heap_base);
+  if (S->ctl) (void)hipFree(S->ctl);
+  if (S->spill) (void)hipFree(S->spill);
+  if (S->spill_base) (void)hipFree(S->spill_base);
+  for (hipStream_t s : {S->copy, S->compute, S->readback})
+    if (s) (void)hipStreamDestroy(s);
+  if (S->ctx) S->ctx->in_use -= std::min(S->ctx->in_use, S->heap.size());
+  delete S;
+}
+
+}  // namespace
+}  // namespace bbtc
+
+#define SRV_TRY try {
+#define SRV_CATCH                                                                \
+  }                                                                              \
+  catch (const bbtc::Error& e) {                                                 \
+    bbtc::set_last_error(e.what());                                              \
+    return e.code;                                                               \
+  }                                                                              \
+  catch (const std::bad_alloc&) {                                                \
+    bbtc::set_last_error("host allocation failed");                              \
+    return BBTC_ERR_NOMEM;                                                       \
+  }                                                                              \
+  catch (const std::exception& e) {                                              \
+    bbtc::set_last_error(e.what());                                              \
+    return BBTC_ERR_INTERNAL;                                                    \
+  }                                                                              \
+  return BBTC_OK;
+
+extern "C" {
+
+bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc_server** out) {
+  if (!ctx || !out) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  *out = nullptr;
+  SRV_TRY
+  HIP_OK(hipSetDevice(ctx->device));
+  auto* S = new bbtc_server;
+  S->ctx = ctx;
+  try {
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    uint64_t heap = opts && opts->heap_bytes ? opts->heap_bytes : bbtc::kDefaultHeap;
+    heap = heap / bbtc::kAlign * bbtc::kAlign;
+    if (!heap) bbtc::fail(BBTC_ERR_INVALID, "heap size is zero");
+    if (ctx->budget && ctx->in_use + heap > ctx->budget)
+      bbtc::fail(BBTC_ERR_BUDGET, "server heap exceeds the context's device budget");
+    if (heap + (1ull << 30) > free_b)
+      bbtc::fail(BBTC_ERR_BUDGET, "server heap of " + std::to_string(heap) + " bytes exceeds free device memory");
+    S->quantum = opts && opts->quantum_items ? opts->quantum_items : bbtc::kDefaultQuantum;
+    S->max_inflight = opts && opts->max_inflight ? opts->max_inflight : bbtc::kDefaultInflight;
+    S->max_segments = opts && opts->max_segments ? opts->max_segments : bbtc::kDefaultSegments;
+    S->sequential = opts && opts->sequential;
+    if (S->sequential) S->max_inflight = 1;
+    HIP_OK(hipMalloc(&S->heap_base, heap));
+    S->heap.init(heap);
+    ctx->in_use += heap;
+    HIP_OK(hipMalloc(&S->ctl, sizeof(bbtc::SlotCtl)));
+    HIP_OK(hipMalloc(&S->spill, sizeof(bbtc::SpillRec) * bbtc::kSpillCap));
+    HIP_OK(hipMalloc(&S->spill_base, sizeof(uint64_t) * bbtc::kSpillCap));
+    for (uint32_t i = 0; i < S->max_inflight; ++i) {
+      bbtc::SegDesc* d = nullptr;
+      bbtc::SegDesc* h = nullptr;
+      HIP_OK(hipMalloc(&d, sizeof(bbtc::SegDesc) * S->max_segments));
+      S->seg_dev.push_back(d);
+      HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(bbtc::SegDesc) * S->max_segments, hipHostMallocDefault));
+      S->seg_host.push_back(h);
+    }
+    HIP_OK(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&S->compute, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&S->readback, hipStreamNonBlocking));
+    S->epoch = bbtc::Clock::now();
+    S->admit_thread = std::thread(bbtc::admit_loop, S);
+    S->sched_thread = std::thread(bbtc::sched_loop, S);
+  } catch (...) {
+    bbtc::server_free(S);
+    throw;
+  }
+  *out = S;
+  SRV_CATCH
+}
+
+bbtc_status bbtc_server_reset_clock(bbtc_server* S) {
+  if (!S) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->submitted != S->finished) {
+    bbtc::set_last_error("requests are outstanding");
+    return BBTC_ERR_INVALID;
+  }
+  S->epoch = bbtc::Clock::now();
+  S->stats = bbtc_server_stats{};
+  return BBTC_OK;
+}
+
+double bbtc_server_now(const bbtc_server* S) { return S ? S->now() : 0.0; }
+
+bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_task* tasks, uint64_t ntasks,
+                               bbtc_kernel kernel, double arrival_s, uint64_t* per_task_out, uint64_t* id_out) {
+  if (!S || !L || (!tasks && ntasks) || !id_out) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  if (!bbtc::valid_kernel(kernel)) {
+    bbtc::set_last_error("unknown kernel choice");
+    return BBTC_ERR_INVALID;
+  }
+  const uint32_t P = L->L.P;
+  for (uint64_t q = 0; q < ntasks; ++q)
+    if (!(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < P)) {
+      bbtc::set_last_error("task out of range or not i <= j <= k");
+      return BBTC_ERR_INVALID;
+    }
+  SRV_TRY
+  auto r = std::make_shared<bbtc::Request>();
+  r->L = &L->L;
+  r->tiles = bbtc::tile_table(L->L);
+  r->plan = bbtc::describe_plan(r->tiles, L->L.deg, bbtc::refs_of(P, tasks, ntasks), kernel, nullptr);
+  r->ntasks = ntasks;
+  r->nitems = r->plan.pref[ntasks];
+  r->per_task_out = per_task_out;
+  r->arrival = arrival_s;
+  r->bytes = bbtc::place(*r);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->fatal != BBTC_OK) bbtc::fail(S->fatal, "server stopped: " + S->fatal_msg);
+    if (S->stop) bbtc::fail(BBTC_ERR_INVALID, "server is shutting down");
+    if (r->bytes > S->heap.size())
+      bbtc::fail(BBTC_ERR_BUDGET, "request footprint of " + std::to_string(r->bytes) +
+                                      " bytes exceeds the server heap of " + std::to_string(S->heap.size()));
+    if (!S->pending.empty() && arrival_s < S->pending.back()->arrival)
+      bbtc::fail(BBTC_ERR_INVALID, "arrival times must not decrease");
+    r->id = S->next_id++;
+    S->pending.push_back(r);
+    S->submitted++;
+  }
+  S->cv_admit.notify_all();
+  *id_out = r->id;
+  SRV_CATCH
+}
+
+bbtc_status bbtc_server_poll(bbtc_server* S, bbtc_server_result* out, uint64_t max, int wait_ms, uint64_t* got) {
+  if (!S || !got || (!out && max)) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (wait_ms != 0 && S->results.empty() && S->fatal == BBTC_OK) {
+    auto pred = [&] { return !S->results.empty() || S->fatal != BBTC_OK || S->submitted == S->finished; };
+    if (wait_ms < 0) S->cv_done.wait(lk, pred);
+    else S->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
+  }
+  uint64_t n = 0;
+  while (n < max && !S->results.empty()) {
+    out[n++] = S->results.front();
+    S->results.pop_front();
+  }
+  *got = n;
+  if (S->fatal != BBTC_OK) {
+    bbtc::set_last_error("server stopped: " + S->fatal_msg);
+    return S->fatal;
+  }
+  return BBTC_OK;
+}
+
+bbtc_status bbtc_server_wait_idle(bbtc_server* S) {
+  if (!S) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  std::unique_lock<std::mutex> lk(S->mu);
+  S->cv_done.wait(lk, [&] { return S->submitted == S->finished || S->fatal != BBTC_OK; });
+  if (S->fatal != BBTC_OK) {
+    bbtc::set_last_error("server stopped: " + S->fatal_msg);
+    return S->fatal;
+  }
+  return BBTC_OK;
+}
+
+bbtc_status bbtc_server_get_stats(bbtc_server* S, bbtc_server_stats* out) {
+  if (!S || !out) {
+    bbtc::set_last_error("null argument");
+    return BBTC_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = S->stats;
+  out->heap_bytes = S->heap.size();
+  return BBTC_OK;
+}
+
+void bbtc_server_destroy(bbtc_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_admit.notify_all();
+  S->cv_sched.notify_all();
+  if (S->admit_thread.joinable()) S->admit_thread.join();
+  if (S->sched_thread.joinable()) S->sched_thread.join();
+  (void)hipSetDevice(S->ctx->device);
+  (void)hipStreamSynchronize(S->compute);
+  (void)hipStreamSynchronize(S->copy);
+  bbtc::server_free(S);
+}
+
+}  // extern "C"

@@ -46,6 +46,10 @@ std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p);
 constexpr uint32_t kDenseMaxBits = 16384;            // widest band kept as bitmaps
 constexpr uint64_t kDenseTileWords = 64ull << 20;    // largest bitmap tile (256 MiB)
 constexpr uint64_t kDenseTotalWords = 4096ull << 20; // all bitmap tiles (16 GiB)
+// a tile is kept as bitmap rows only if they take at most this many words per
+// non-zero (a density of at least 1/512): sparser tiles gain nothing from the
+// bitmap forms on RMAT scale 26 and would only hold memory
+constexpr uint64_t kDenseWordsPerNnz = 16;
 
 struct Layout {
   uint32_t P = 0;

@@ -33,7 +33,7 @@ static void build_dense(Layout& L) {
     for (uint32_t k = i; k < P; ++k) {
       const uint64_t words = uint64_t(L.height(i)) * L.dwpr[k];
       if (L.height(k) == 0 || L.height(k) > kDenseMaxBits || words > kDenseTileWords) continue;
-      if (L.nnz[L.tile(i, k)] == 0) continue;
+      if (L.nnz[L.tile(i, k)] == 0 || words > kDenseWordsPerNnz * L.nnz[L.tile(i, k)]) continue;
       cand.emplace_back(words, L.tile(i, k));
     }
   std::sort(cand.begin(), cand.end());

@@ -1,10 +1,18 @@
-"""Seeded multi-graph workloads and the batched / plain-launch runners.
+"""Seeded multi-graph workloads and their runners: batched, serving, and the
+plain per-graph launch each is compared with.
 
 Batched workload (4,096 independent graphs, graph g seeded 1000 + g):
 70% RMAT (scale uniform in [10, 18], edge factor uniform in [8, 32],
 a=.57 b=.19 c=.19), 30% G(n, m) (n uniform in [2^10, 2^18], average degree d
-uniform in [4, 64], m = round(n d / 2)). Each graph's parameters come from a
-SplitMix64 draw of its seed, so the list is the same on every machine.
+uniform in [4, 64], m = round(n d / 2)).
+
+Serving workload (request r seeded 7_000_000 + r): 50% RMAT (scale uniform in
+[12, 24], edge factor 16), 50% G(n, m) (n uniform in [2^12, 2^22], average
+degree 32, m = 16 n); Poisson arrivals (exponential gaps from a generator
+seeded 7) at a fixed rate.
+
+Each graph's parameters come from a SplitMix64 draw of its seed, so the lists
+are the same on every machine.
 """
 from __future__ import annotations
 
@@ -12,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import Batch, Device, Graph, layouts_build, rule_ncut, tune_ncut
+from . import Batch, Device, Graph, Layout, Server, layouts_build, rule_ncut, tune_ncut
 
 M64 = (1 << 64) - 1
 
@@ -129,4 +137,138 @@ def run_batched(dev: Device, specs, log=print, sample_every: int = 64, cands=(1,
     log(f"batched: {res['batched_graphs_per_s']:.0f} graphs/s vs plain {res['plain_graphs_per_s']:.0f} "
         f"graphs/s ({res['speedup']:.2f}x); p50/p99 {res['batched_p50_ms']:.2f}/{res['batched_p99_ms']:.2f} ms "
         f"(plain {res['plain_p50_ms']:.3f}/{res['plain_p99_ms']:.3f} ms)")
+    return res
+
+
+# ---- serving -------------------------------------------------------------------
+def serving_specs(count: int, seed0: int = 7_000_000):
+    out = []
+    for r in range(count):
+        seed = seed0 + r
+        h1, h2 = _mix(seed), _mix(seed ^ 0x5A5A5A5A)
+        if h1 % 2 == 0:
+            out.append(dict(kind="rmat", scale=12 + (h1 >> 8) % 13, ef=16, seed=seed))
+        else:
+            n = (1 << 12) + (h2 >> 8) % ((1 << 22) - (1 << 12) + 1)
+            out.append(dict(kind="gnm", n=n, m=16 * n, seed=seed))
+    return out
+
+
+def poisson_arrivals(count: int, rate: float, seed: int = 7):
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, size=count)
+    return np.cumsum(gaps) - gaps[0]  # the first request arrives at 0
+
+
+def prepare_requests(dev: Device, specs, log=print, cands=(1, 2, 4, 8, 16), nsample: int = 8):
+    """Graphs, per-graph cut counts from a measured rule, layouts and task
+    lists (descending order) of every request."""
+    t0 = time.time()
+    graphs = [make_graph(dev, s) for s in specs]
+    ms = [g.m for g in graphs]
+    log(f"serving: {len(graphs)} graphs generated in {time.time() - t0:.1f}s, {sum(ms):,} edges")
+    t0 = time.time()
+    order = np.argsort(ms)
+    pick = np.unique(np.linspace(0, len(graphs) - 1, min(nsample, len(graphs))).round().astype(int))
+    rule, _ = tune_ncut(dev, [graphs[order[x]] for x in pick], cands)
+    ncut = [rule_ncut(rule, m) for m in ms]
+    log(f"serving: cut rule from {len(pick)} sample graphs in {time.time() - t0:.1f}s; "
+        f"ncut histogram {dict(zip(*np.unique(ncut, return_counts=True)))}")
+    t0 = time.time()
+    layouts = [None] * len(graphs)
+    big = [x for x in range(len(graphs)) if ms[x] > (8 << 20)]
+    small = [x for x in range(len(graphs)) if ms[x] <= (8 << 20)]
+    for x in big:  # parallel inside
+        layouts[x] = Layout(graphs[x], graphs[x].partition("pal", ncut[x]))
+    for x, L in zip(small, layouts_build([graphs[x] for x in small], [ncut[x] for x in small], "pal")):
+        layouts[x] = L
+    tasks = [L.tasks("desc") for L in layouts]
+    log(f"serving: layouts in {time.time() - t0:.1f}s, host block CSR "
+        f"{sum(L.device_bytes for L in layouts) / 1e9:.1f} GB")
+    return graphs, layouts, tasks, ncut
+
+
+def run_plain(dev: Device, layouts, tasks, arrivals=None):
+    """The plain per-graph launch, one request after another: upload, count,
+    release. arrivals (seconds, or None: all at once) delay each request until
+    its arrival. Returns (per-request totals, latencies s, makespan s)."""
+    n = len(layouts)
+    tot = np.zeros(n, dtype=np.uint64)
+    lat = np.zeros(n)
+    per = [None] * n
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for x in range(n):
+        a = 0.0 if arrivals is None else float(arrivals[x])
+        wait = a - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        dg = dev.upload(layouts[x])
+        per[x], tot[x] = dg.count(tasks[x], "auto")
+        dg.release()
+        lat[x] = time.perf_counter() - t0 - a
+    return tot, lat, time.perf_counter() - t0, per
+
+
+def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200 << 30, **opts):
+    """Every request through one continuous-batching server. Returns (totals,
+    latencies s, makespan s, per-task counts, server statistics)."""
+    n = len(layouts)
+    srv = Server(dev, heap_bytes=heap_bytes, **opts)
+    try:
+        srv.reset_clock()
+        ids = [srv.submit(layouts[x], tasks[x], 0.0 if arrivals is None else float(arrivals[x]))
+               for x in range(n)]
+        res = {r.id: r for r in srv.drain()}
+        stats = srv.stats()
+    finally:
+        srv.destroy()
+    rs = [res[i] for i in ids]
+    bad = [r.status for r in rs if r.status != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} requests failed in the server (status {bad[0]})")
+    tot = np.array([r.total for r in rs], dtype=np.uint64)
+    lat = np.array([r.latency_s for r in rs])
+    return tot, lat, max(r.done_s for r in rs), [r.per_task for r in rs], stats
+
+
+def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, quantum_items: int = 0):
+    """Serving workload: the plain per-graph launch and the server, first
+    saturated (every request at time 0: capacity), then at Poisson arrivals of
+    `rate` requests/s (default: `load` times the plain capacity). Per-task
+    counts must agree bit for bit between the two."""
+    graphs, layouts, tasks, ncut = prepare_requests(dev, specs, log)
+    del graphs
+    n = len(layouts)
+    # warm code paths on the smallest request
+    small = int(np.argmin([L.device_bytes for L in layouts]))
+    run_plain(dev, [layouts[small]], [tasks[small]])
+
+    p_tot, _, p_make, p_per = run_plain(dev, layouts, tasks)
+    p_cap = n / p_make
+    log(f"plain saturated: {p_cap:.2f} graphs/s ({p_make:.2f}s for {n})")
+    s_tot, _, s_make, s_per, st = run_server(dev, layouts, tasks, quantum_items=quantum_items)
+    s_cap = n / s_make
+    log(f"server saturated: {s_cap:.2f} graphs/s ({s_make:.2f}s); {st}")
+    if not np.array_equal(p_tot, s_tot) or not all(np.array_equal(a, b) for a, b in zip(p_per, s_per)):
+        raise RuntimeError("server counts differ from the plain per-graph launch")
+
+    lam = rate if rate else load * p_cap
+    arr = poisson_arrivals(n, lam)
+    p_tot2, p_lat, p_make2, _ = run_plain(dev, layouts, tasks, arr)
+    s_tot2, s_lat, s_make2, _, st2 = run_server(dev, layouts, tasks, arr, quantum_items=quantum_items)
+    if not (np.array_equal(p_tot2, p_tot) and np.array_equal(s_tot2, p_tot)):
+        raise RuntimeError("counts at the Poisson rate differ from the saturated runs")
+    res = dict(
+        requests=n, edges=int(sum(L.graph.m for L in layouts)), triangles=int(p_tot.sum()),
+        plain_capacity_gps=p_cap, server_capacity_gps=s_cap, capacity_ratio=s_cap / p_cap,
+        rate=lam, load_vs_plain=lam / p_cap,
+        plain_gps=n / p_make2, server_gps=n / s_make2,
+        plain_p50_ms=percentile(p_lat, 50) * 1e3, plain_p99_ms=percentile(p_lat, 99) * 1e3,
+        server_p50_ms=percentile(s_lat, 50) * 1e3, server_p99_ms=percentile(s_lat, 99) * 1e3,
+        server_launches=st2["launches"], server_peak_active=st2["peak_active"],
+        server_peak_resident_gb=st2["peak_resident_bytes"] / 1e9,
+    )
+    log(f"serving at {lam:.2f} req/s: server p50/p99 {res['server_p50_ms']:.1f}/{res['server_p99_ms']:.1f} ms, "
+        f"plain p50/p99 {res['plain_p50_ms']:.1f}/{res['plain_p99_ms']:.1f} ms; capacity "
+        f"{s_cap:.2f} vs {p_cap:.2f} graphs/s ({res['capacity_ratio']:.2f}x)")
     return res

@@ -319,6 +319,62 @@ bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* 
                               int nstreams, bbtc_kernel kernel, uint64_t* per_task,
                               uint64_t* total, uint64_t* gpu_tasks_out);
 
+/* ------------------------------------------------------------------------- */
+/* Serving: count requests for many graphs on one device with continuous      */
+/* batching. Replaces running the reference's main (bbtc.cu:22-89) once per   */
+/* graph: requests are admitted in arrival order while their device footprint */
+/* (block CSR, bitmap rows of dense tiles, plan tables) fits the server heap; */
+/* an upload thread copies them in while earlier requests count; every launch */
+/* takes a slice of each resident request with work left, so arrivals join    */
+/* the next launch and share the machine with requests already running.       */
+/* ------------------------------------------------------------------------- */
+typedef struct bbtc_server bbtc_server;
+
+typedef struct bbtc_server_opts {
+  uint64_t heap_bytes;     /* device heap for resident requests (0: 200 GiB) */
+  uint64_t quantum_items;  /* work items of one request per launch (0: 2^19) */
+  uint32_t max_inflight;   /* launches queued on the device at once (0: 2) */
+  uint32_t max_segments;   /* requests per launch (0: 1024) */
+  uint32_t sequential;     /* 1: one request at a time, whole plan per launch */
+  uint32_t reserved;
+} bbtc_server_opts;
+
+typedef struct bbtc_server_result {
+  uint64_t id;
+  uint64_t total;      /* triangles of the request's tasks */
+  int32_t status;      /* BBTC_OK, or why the request was dropped */
+  uint32_t launches;   /* launches its items were spread over */
+  uint64_t bytes;      /* device footprint while resident */
+  /* server clock (seconds since create / reset_clock) */
+  double arrival_s, admit_s, ready_s, start_s, done_s;
+} bbtc_server_result;
+
+typedef struct bbtc_server_stats {
+  uint64_t launches, segments, items;
+  uint64_t peak_resident_bytes, heap_bytes;
+  uint32_t peak_active, peak_segments;
+} bbtc_server_stats;
+
+bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc_server** out);
+/* Restarts the server clock and statistics; no request may be outstanding. */
+bbtc_status bbtc_server_reset_clock(bbtc_server* s);
+double bbtc_server_now(const bbtc_server* s);
+/* Queues a count of tasks[0..n) of L, arriving at arrival_s on the server
+ * clock (not decreasing between calls; the request waits until then). L and
+ * per_task_out (may be NULL) must stay valid until the result is polled. */
+bbtc_status bbtc_server_submit(bbtc_server* s, const bbtc_layout* L, const bbtc_task* tasks, uint64_t n,
+                               bbtc_kernel kernel, double arrival_s, uint64_t* per_task_out,
+                               uint64_t* id_out);
+/* Up to max finished requests in completion order; waits up to wait_ms
+ * (-1: until one finishes or nothing is outstanding). */
+bbtc_status bbtc_server_poll(bbtc_server* s, bbtc_server_result* out, uint64_t max, int wait_ms,
+                             uint64_t* got);
+/* Waits until every submitted request has finished. */
+bbtc_status bbtc_server_wait_idle(bbtc_server* s);
+bbtc_status bbtc_server_get_stats(bbtc_server* s, bbtc_server_stats* out);
+/* Finishes admitted requests, drops those not yet admitted, frees the heap. */
+void bbtc_server_destroy(bbtc_server* s);
+
 #ifdef __cplusplus
 }
 #endif

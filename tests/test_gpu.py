@@ -267,3 +267,94 @@ def test_s26_properties(dev, s26):
     per2, tot2 = dg2.count(t2, "auto")
     dg2.release()
     assert tot2 == tot_a
+
+
+# ---- serving -----------------------------------------------------------------
+def _serving_requests():
+    """Small requests of uneven size and cut count; per-task CPU counts."""
+    specs = [("rmat", 12, 8, 1, "pal", 4), ("gnm", 3000, 20000, 2, "uni", 2), ("rmat", 14, 16, 3, "pal", 8),
+             ("rmat", 10, 4, 4, "n1d", 1), ("gnm", 20000, 160000, 5, "pal", 3), ("rmat", 13, 24, 6, "uni", 5),
+             ("gnm", 64, 0, 7, "uni", 2), ("rmat", 16, 18, 1492, "uni", 1)]
+    reqs = []
+    for kind, a, b, seed, part, p in specs:
+        if kind == "rmat":
+            g = B.Graph.rmat(a, b, seed=seed)
+        elif b:
+            g = B.Graph.gnm(a, b, seed)
+        else:
+            g = B.Graph.from_csr(np.zeros(a + 1, np.uint64), np.zeros(0, np.uint32))
+        L = B.Layout(g, g.partition(part, p))
+        t = L.tasks("desc")
+        reqs.append((L, t, L.count_cpu(t)[0]))
+    return reqs
+
+
+def test_server_alone_vs_mix(dev):
+    reqs = _serving_requests()
+    # alone: one request in the server at a time
+    srv = B.Server(dev, heap_bytes=1 << 30)
+    alone = []
+    for L, t, cpu in reqs:
+        srv.submit(L, t)
+        (r,) = srv.drain()
+        assert r.status == 0 and np.array_equal(r.per_task, cpu)
+        alone.append(r.per_task.copy())
+    srv.destroy()
+    # in the mix: everything at once, 8 items per request per launch, so each
+    # launch mixes slices of several requests and large ones span many launches
+    srv = B.Server(dev, heap_bytes=1 << 30, quantum_items=8)
+    ids = [srv.submit(L, t, 0.0) for L, t, _ in reqs]
+    got = {r.id: r for r in srv.drain()}
+    st = srv.stats()
+    srv.destroy()
+    log("mix stats", st)
+    assert st["peak_segments"] > 1 and st["launches"] > len(reqs)
+    for x, i in enumerate(ids):
+        assert got[i].status == 0
+        assert np.array_equal(got[i].per_task, alone[x]), x
+        assert got[i].total == int(alone[x].sum())
+    assert max(r.launches for r in got.values()) > 1
+    # plain mode (one request at a time, whole plan per launch) agrees too
+    srv = B.Server(dev, heap_bytes=1 << 30, sequential=True)
+    ids = [srv.submit(L, t, 0.0) for L, t, _ in reqs]
+    got = {r.id: r for r in srv.drain()}
+    assert srv.stats()["peak_active"] == 1
+    srv.destroy()
+    for x, i in enumerate(ids):
+        assert np.array_equal(got[i].per_task, alone[x])
+
+
+def test_server_heap_admission(dev):
+    reqs = _serving_requests()
+    g = B.Graph.rmat(12, 8, seed=11)
+    L = B.Layout(g, g.partition("pal", 4))
+    t = L.tasks("desc")
+    cpu = L.count_cpu(t)[0]
+    srv = B.Server(dev, heap_bytes=4 << 20)
+    # larger than the whole heap: refused at submission
+    with pytest.raises(B.BbtcError) as e:
+        srv.submit(reqs[-1][0], reqs[-1][1])
+    assert e.value.status == 7  # BBTC_ERR_BUDGET
+    # twelve copies that fit only a few at a time: admission waits for memory
+    ids = [srv.submit(L, t, 0.0) for _ in range(12)]
+    got = srv.drain()
+    st = srv.stats()
+    srv.destroy()
+    assert len(got) == 12 and all(r.status == 0 and np.array_equal(r.per_task, cpu) for r in got)
+    assert st["peak_resident_bytes"] <= st["heap_bytes"] and st["peak_active"] < 12
+    assert sorted(r.id for r in got) == sorted(ids)
+
+
+def test_server_poisson_arrivals(dev):
+    from bbtc_amd.workloads import poisson_arrivals
+    reqs = _serving_requests() * 3
+    arr = poisson_arrivals(len(reqs), 200.0, seed=7)
+    srv = B.Server(dev, heap_bytes=1 << 30, quantum_items=256)
+    srv.reset_clock()
+    ids = [srv.submit(L, t, a) for (L, t, _), a in zip(reqs, arr)]
+    got = {r.id: r for r in srv.drain()}
+    srv.destroy()
+    for (L, t, cpu), i, a in zip(reqs, ids, arr):
+        r = got[i]
+        assert r.status == 0 and np.array_equal(r.per_task, cpu)
+        assert abs(r.arrival_s - a) < 1e-9 and r.admit_s >= a and r.done_s >= r.start_s >= r.ready_s >= r.admit_s
