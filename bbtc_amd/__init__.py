@@ -124,7 +124,7 @@ def _declare(lib):
         "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, C.POINTER(u64),
                                    C.POINTER(u64)]),
         "bbtc_layouts_build": (st, [vp, u64, vp, i32, vp]),
-        "bbtc_tune_ncut": (st, [vp, vp, u64, vp, u32, i32, vp, vp]),
+        "bbtc_tune_ncut": (st, [vp, vp, u64, vp, u32, i32, i32, vp, vp]),
         "bbtc_rule_ncut": (u32, [vp, u64]),
         "bbtc_batch_create": (st, [vp, vp, u64, i32, pvp]),
         "bbtc_batch_ntasks": (u64, [vp]),
@@ -488,15 +488,17 @@ def layouts_build(graphs, ncuts, kind: str = "pal"):
     return layouts
 
 
-def tune_ncut(dev: "Device", sample, cands=(1, 2, 4, 8), kind: str = "pal"):
-    """Cut-count rule from measured device work on sample graphs:
-    returns (rule[64] indexed by floor(log2(m+1)), work[n, ncand])."""
+def tune_ncut(dev: "Device", sample, cands=(1, 2, 4, 8), kind: str = "pal", tune_forms: bool = True):
+    """Cut-count rule from measured device work on sample graphs (forms
+    autotuned per graph, or the AUTO rule's forms): returns (rule[64] indexed
+    by floor(log2(m+1)), work[n, ncand])."""
     n = len(sample)
     gh = (C.c_void_p * max(n, 1))(*[g._h for g in sample])
     cc = np.ascontiguousarray(cands, dtype=np.uint32)
     rule = np.zeros(64, dtype=np.uint32)
     work = np.zeros((max(n, 1), len(cc)), dtype=np.uint64)
-    _check(lib().bbtc_tune_ncut(dev._h, gh, n, _ptr(cc), len(cc), PART[kind], _ptr(rule), _ptr(work)))
+    _check(lib().bbtc_tune_ncut(dev._h, gh, n, _ptr(cc), len(cc), PART[kind], int(tune_forms), _ptr(rule),
+                                _ptr(work)))
     return rule, work[:n]
 
 

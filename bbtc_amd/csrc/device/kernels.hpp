@@ -92,6 +92,13 @@ __device__ __forceinline__ uint32_t find_task(const uint64_t* __restrict__ prefi
   return lo;
 }
 
+// x as a wave-uniform (scalar) value; x must be equal in every lane.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       if (item >= batch_end) {
         unsigned long long b = 0;
         if (lane == 0) b = atomicAdd(&a.ctl->head, (unsigned long long)kItemBatch);
-        item = __shfl(b, 0, kWave);
+        item = uniform64(__shfl(b, 0, kWave));
         batch_end = item + kItemBatch;
         if (batch_end > a.item_end) batch_end = a.item_end;
       }
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     } else {
       unsigned long long c = 0;
       if (lane == 0) c = atomicAdd(&a.ctl->head2, 1ull);
-      c = __shfl(c, 0, kWave);
+      c = uniform64(__shfl(c, 0, kWave));
       if (c >= nchunks || nrec == 0) break;
       uint32_t lo = 0, hi = nrec;  // record holding chunk c: last base <= c
       while (hi - lo > 1) {

@@ -9,12 +9,14 @@
 // on a copy stream while the device counts earlier requests.
 //
 // The scheduler thread keeps up to `max_inflight` launches queued on the
-// compute stream. Each launch is one count_probe launch pair whose segments
-// are a slice (at most `quantum_items` work items) of every resident request
-// with work left, so a request that arrives while others run joins the next
-// launch and shares the machine with them. A request finishes with the launch
-// that holds its last slice. Its counters are read back and its heap block
-// returns to the heap, which lets the next arrival in.
+// compute stream. Each launch is one count_probe launch pair over a budget of
+// `quantum_items` work items, filled with slices of the resident requests in
+// order of remaining work (shortest first). A request that arrives while
+// others run joins the next launch and shares the machine with them; a large
+// request takes what the small ones leave and the whole budget when alone. A
+// request finishes with the launch that holds its last slice. Its counters
+// are read back and its heap block returns to the heap, which lets the next
+// arrival in.
 //
 // This replaces the reference's per-run task loop (bb_tc.cpp:142-181,
 // 200-252), which counts one graph per process.
@@ -41,7 +43,7 @@ using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kAlign = 256;
 constexpr uint64_t kDefaultHeap = 200ull << 30;
-constexpr uint64_t kDefaultQuantum = 1ull << 19;
+constexpr uint64_t kDefaultQuantum = 1ull << 18;
 constexpr uint32_t kDefaultInflight = 2;
 constexpr uint32_t kDefaultSegments = 1024;
 
@@ -378,10 +380,18 @@ void sched_loop(bbtc_server* S) {
         Launch l;
         l.ring = ring;
         const double t = S->now();
-        for (auto& r : act) {
-          if (r->next >= r->nitems) continue;
-          if (nseg == S->max_segments) break;
-          const uint64_t take = S->sequential ? r->nitems - r->next : std::min(r->nitems - r->next, S->quantum);
+        // shortest remaining work first, within the launch's item budget
+        std::vector<Request*> cand;
+        for (auto& r : act)
+          if (r->next < r->nitems) cand.push_back(r.get());
+        std::stable_sort(cand.begin(), cand.end(), [](const Request* x, const Request* y) {
+          return x->nitems - x->next < y->nitems - y->next;
+        });
+        uint64_t budget = S->sequential ? ~0ull : S->quantum;
+        for (Request* r : cand) {
+          if (nseg == S->max_segments || budget == 0) break;
+          const uint64_t take = std::min(r->nitems - r->next, budget);
+          budget -= take;
           SegDesc d = r->seg;
           d.item0 = r->next;
           d.item1 = r->next + take;
@@ -391,7 +401,9 @@ void sched_loop(bbtc_server* S) {
           r->next += take;
           r->launches++;
           if (r->start_s < 0) r->start_s = t;
-          if (r->next == r->nitems) l.finishing.push_back(r);
+          if (r->next == r->nitems)
+            for (auto& a : act)
+              if (a.get() == r) l.finishing.push_back(a);
         }
         if (nseg) {
           ring_free.pop_back();

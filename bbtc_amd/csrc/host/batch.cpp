@@ -72,8 +72,8 @@ bbtc_status bbtc_layouts_build(const bbtc_hgraph* const* graphs, uint64_t n, con
 }
 
 bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint64_t n,
-                           const uint32_t* cands, uint32_t ncand, bbtc_part kind, uint32_t* rule,
-                           uint64_t* work_out) {
+                           const uint32_t* cands, uint32_t ncand, bbtc_part kind, int tune_forms,
+                           uint32_t* rule, uint64_t* work_out) {
   if (!ctx || (!sample && n) || !cands || !ncand || !rule) {
     bbtc::set_last_error("null argument");
     return BBTC_ERR_INVALID;
@@ -95,9 +95,10 @@ bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint
       uint64_t got = 0;
       st = bbtc_tasks_build(L[x], BBTC_ORDER_DESC, t.data(), nt, &got);
       std::vector<uint8_t> choice(nt);
-      if (st == BBTC_OK) st = bbtc_autotune(ctx, g, t.data(), got, choice.data(), nullptr);
+      if (st == BBTC_OK && tune_forms) st = bbtc_autotune(ctx, g, t.data(), got, choice.data(), nullptr);
       bbtc_plan* p = nullptr;
-      if (st == BBTC_OK) st = bbtc_plan_create_tuned(ctx, g, t.data(), got, BBTC_KERNEL_AUTO, choice.data(), &p);
+      if (st == BBTC_OK)
+        st = bbtc_plan_create_tuned(ctx, g, t.data(), got, BBTC_KERNEL_AUTO, tune_forms ? choice.data() : nullptr, &p);
       std::vector<uint64_t> cyc(got);
       if (st == BBTC_OK) st = bbtc_plan_profile(p, cyc.data());
       if (st == BBTC_OK) {

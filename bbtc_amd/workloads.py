@@ -169,7 +169,7 @@ def prepare_requests(dev: Device, specs, log=print, cands=(1, 2, 4, 8, 16), nsam
     t0 = time.time()
     order = np.argsort(ms)
     pick = np.unique(np.linspace(0, len(graphs) - 1, min(nsample, len(graphs))).round().astype(int))
-    rule, _ = tune_ncut(dev, [graphs[order[x]] for x in pick], cands)
+    rule, _ = tune_ncut(dev, [graphs[order[x]] for x in pick], cands, tune_forms=False)
     ncut = [rule_ncut(rule, m) for m in ms]
     log(f"serving: cut rule from {len(pick)} sample graphs in {time.time() - t0:.1f}s; "
         f"ncut histogram {dict(zip(*np.unique(ncut, return_counts=True)))}")

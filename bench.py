@@ -15,6 +15,13 @@ Prints one JSON line:
   roofline      algorithmic bytes per step / mean event time vs 8 TB/s
   cpu_baseline  the CPU oracle (restated reference CPU path) on a sample of
                 the same tasks, scaled by algorithmic bytes
+  batched       4,096 uneven graphs (70% RMAT s10-18, 30% G(n,m)) resident,
+                one captured persistent launch vs the plain per-graph launch:
+                graphs/s, per-graph p50/p99, bit-exact per-graph counts
+  serving       requests (50% RMAT s12-24 ef16, 50% G(n,m) n 2^12-2^22 d 32)
+                through the continuous-batching server vs the plain
+                sequential launch: capacity, and p50/p99 latency at Poisson
+                arrivals of 80% of the plain capacity; bit-exact counts
 """
 from __future__ import annotations
 
@@ -88,6 +95,8 @@ def main():
     ap.add_argument("--kernel", default="tuned", help="tuned | auto | merge | bitmap | dense | rdense")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-graphs", type=int, default=4096, help="0: skip the batched workload")
+    ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -174,6 +183,23 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    plan.destroy()
+    dg.release()
+    del plan, dg, L, g
+
+    from bbtc_amd import workloads as W
+
+    def rounded(d):
+        return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+    if args.batched_graphs:
+        res = W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log)
+        out["batched"] = rounded(dict(res, metric="graphs/s", unit="graphs/s",
+                                      value=res["batched_graphs_per_s"], vs_plain=res["speedup"]))
+    if args.serving_requests:
+        res = W.run_serving(dev, W.serving_specs(args.serving_requests), log=log)
+        out["serving"] = rounded(dict(res, metric="graphs/s at capacity", unit="graphs/s",
+                                      value=res["server_capacity_gps"], vs_plain=res["capacity_ratio"]))
     print(json.dumps(out), flush=True)
 
 

@@ -271,13 +271,14 @@ bbtc_status bbtc_layouts_build(const bbtc_hgraph* const* graphs, uint64_t n, con
                                bbtc_part kind, bbtc_layout** out);
 
 /* Cut-count autotuner: counts every sample graph with every candidate cut
- * count (forms autotuned per graph) and measures the device work (summed
- * per-item clocks); rule[64] receives, per size class floor(log2(m+1)), the
- * candidate with the least work (classes without samples copy the nearest
- * sampled class). work_out (may be NULL) receives n x ncand clock sums. */
+ * count and measures the device work (summed per-item clocks), with the
+ * forms autotuned per graph (tune_forms != 0) or chosen by BBTC_KERNEL_AUTO;
+ * rule[64] receives, per size class floor(log2(m+1)), the candidate with the
+ * least work (classes without samples copy the nearest sampled class).
+ * work_out (may be NULL) receives n x ncand clock sums. */
 bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint64_t n,
-                           const uint32_t* cands, uint32_t ncand, bbtc_part kind, uint32_t* rule,
-                           uint64_t* work_out);
+                           const uint32_t* cands, uint32_t ncand, bbtc_part kind, int tune_forms,
+                           uint32_t* rule, uint64_t* work_out);
 /* The rule's cut count for a graph with m edges. */
 uint32_t bbtc_rule_ncut(const uint32_t* rule, uint64_t m);
 
@@ -324,15 +325,16 @@ bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* 
 /* batching. Replaces running the reference's main (bbtc.cu:22-89) once per   */
 /* graph: requests are admitted in arrival order while their device footprint */
 /* (block CSR, bitmap rows of dense tiles, plan tables) fits the server heap; */
-/* an upload thread copies them in while earlier requests count; every launch */
-/* takes a slice of each resident request with work left, so arrivals join    */
-/* the next launch and share the machine with requests already running.       */
+/* an upload thread copies them in while earlier requests count; each launch  */
+/* fills a budget of work items with slices of the resident requests, least   */
+/* remaining work first, so arrivals join the next launch and share the      */
+/* machine with requests already running.                                     */
 /* ------------------------------------------------------------------------- */
 typedef struct bbtc_server bbtc_server;
 
 typedef struct bbtc_server_opts {
   uint64_t heap_bytes;     /* device heap for resident requests (0: 200 GiB) */
-  uint64_t quantum_items;  /* work items of one request per launch (0: 2^19) */
+  uint64_t quantum_items;  /* work items per launch (0: 2^18) */
   uint32_t max_inflight;   /* launches queued on the device at once (0: 2) */
   uint32_t max_segments;   /* requests per launch (0: 1024) */
   uint32_t sequential;     /* 1: one request at a time, whole plan per launch */

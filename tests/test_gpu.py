@@ -300,9 +300,9 @@ def test_server_alone_vs_mix(dev):
         assert r.status == 0 and np.array_equal(r.per_task, cpu)
         alone.append(r.per_task.copy())
     srv.destroy()
-    # in the mix: everything at once, 8 items per request per launch, so each
-    # launch mixes slices of several requests and large ones span many launches
-    srv = B.Server(dev, heap_bytes=1 << 30, quantum_items=8)
+    # in the mix: everything at once, 64 items per launch, so launches mix
+    # slices of several requests and large ones span many launches
+    srv = B.Server(dev, heap_bytes=1 << 30, quantum_items=64)
     ids = [srv.submit(L, t, 0.0) for L, t, _ in reqs]
     got = {r.id: r for r in srv.drain()}
     st = srv.stats()
