@@ -37,6 +37,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "count_probe_rmat-s26-ef16_pal32_traffic.json")
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per count from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    runs of this workload (scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if t.get("workload") != workload:
+        return None, None
+    return int(t["traffic_bytes"]), os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
 def log(*a):
@@ -155,6 +169,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(g, cuts, tasks, per, bytes_per_task, total_bytes, m, args.cpu_seconds)
 
+    workload = f"rmat-s{args.scale}-ef{args.edge_factor} {args.partitioner} ncut{args.ncut}"
+    traffic, traffic_src = measured_traffic(workload) if args.kernel == "tuned" else (None, None)
     out = {
         "metric": "edges/s (RMAT s26 block-based triangle count)",
         "value": m / (ms_per_step * 1e-3),
@@ -169,7 +185,7 @@ def main():
         "dtype": "u32",
         "data": f"synthetic (seeded RMAT generator, seed {args.seed})",
         "config": {
-            "workload": f"rmat-s{args.scale}-ef{args.edge_factor} {args.partitioner} ncut{args.ncut}",
+            "workload": workload,
             "n": n, "m": m, "ncut": args.ncut, "tasks": int(len(tasks)),
             "partitioner": args.partitioner, "kernel": args.kernel,
             "triangles": int(total), "autotune_s": round(tune_s, 2),
@@ -177,7 +193,7 @@ def main():
         "triangles_per_s": total / (ms_per_step * 1e-3),
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "algorithmic_bytes": total_bytes, "offset_bytes": offset_bytes,
             "kernel": "count_probe", "event_ms_mean": ev_mean_s * 1e3,
         },
