@@ -121,7 +121,7 @@ def _declare(lib):
         "bbtc_autotune": (st, [vp, vp, vp, u64, vp, vp]),
         "bbtc_plan_destroy": (None, [vp]),
         "bbtc_count": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
-        "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, C.POINTER(u64),
+        "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, vp, C.POINTER(u64),
                                    C.POINTER(u64)]),
         "bbtc_layouts_build": (st, [vp, u64, vp, i32, vp]),
         "bbtc_tune_ncut": (st, [vp, vp, u64, vp, u32, i32, i32, vp, vp]),
@@ -394,14 +394,15 @@ class DeviceGraph:
                                 C.byref(tot), None))
         return per, tot.value
 
-    def count_hybrid(self, tasks, separator, nthreads=0, nstreams=1, kernel="auto"):
+    def count_hybrid(self, tasks, separator, nthreads=0, nstreams=1, kernel="auto", choice=None):
         """Device from the front, host workers from the back (--border)."""
         t = tasks_array(tasks)
         per = np.zeros(len(t), dtype=np.uint64)
         tot, gt = C.c_uint64(), C.c_uint64()
+        ch = None if choice is None else np.ascontiguousarray(choice, dtype=np.uint8)
         _check(lib().bbtc_count_hybrid(self.dev._h, self._h, self.layout._h, _ptr(t), len(t), separator,
-                                       nthreads, nstreams, KERNEL[kernel], _ptr(per), C.byref(tot),
-                                       C.byref(gt)))
+                                       nthreads, nstreams, KERNEL[kernel], None if ch is None else _ptr(ch),
+                                       _ptr(per), C.byref(tot), C.byref(gt)))
         return per, tot.value, gt.value
 
     def release(self):

@@ -11,7 +11,7 @@
 //   * argument errors exit with status 2 (the reference exits 0);
 //   * "Cuda Initial Memory Allocations and Stream Creation" is reported as
 //     "HIP Initial Memory Allocations and Stream Creation";
-//   * extra flags: --kernel auto|merge|bitmap, --threads n, --cuts-file path,
+//   * extra flags: --kernel auto|merge|bitmap|dense|rdense|tuned, --threads n, --cuts-file path,
 //     --per-task-out path.
 #include <omp.h>
 #include <strings.h>
@@ -65,7 +65,7 @@ void usage() {
             << "                          Note: if set to -1, GPUs process all.\n"
             << "  --copy-first          If set then copy graph to GPUs first       (false)\n"
             << "  --repeat <int>        Run bbTC algorithm '--repeat' times        (5)\n"
-            << "  --kernel <str>        auto | merge | bitmap                      (auto)\n"
+            << "  --kernel <str>        auto | merge | bitmap | dense | rdense | tuned (auto)\n"
             << "  --threads <int>       Host worker threads                        (all)\n"
             << "  --cuts-file <path>    Read the p+1 cuts from a text file instead of partitioning\n"
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
@@ -140,7 +140,9 @@ Params parse(int argc, char** argv) {
       if (p.part != "pal" && p.part != "n1d" && p.part != "uni") bad("Wrong partitioning algorithm.");
     } else if (!strcasecmp(a, "--kernel")) {
       p.kernel = val("--kernel");
-      if (p.kernel != "auto" && p.kernel != "merge" && p.kernel != "bitmap") bad("Wrong kernel.");
+      if (p.kernel != "auto" && p.kernel != "merge" && p.kernel != "bitmap" && p.kernel != "dense" &&
+          p.kernel != "rdense" && p.kernel != "tuned")
+        bad("Wrong kernel.");
     } else if (!strcasecmp(a, "--threads")) {
       p.threads = to_int(val("--threads"), "--threads");
       check_int("--threads", p.threads);
@@ -239,8 +241,13 @@ int main(int argc, char** argv) {
   if (P.border == -1 || uint64_t(P.border) > ntasks) separator = ntasks;
   else separator = uint64_t(P.border);
 
-  const bbtc_kernel kern = P.kernel == "merge" ? BBTC_KERNEL_MERGE
-                           : P.kernel == "bitmap" ? BBTC_KERNEL_BITMAP : BBTC_KERNEL_AUTO;
+  const bbtc_kernel kern = P.kernel == "merge"    ? BBTC_KERNEL_MERGE
+                           : P.kernel == "bitmap" ? BBTC_KERNEL_BITMAP
+                           : P.kernel == "dense"  ? BBTC_KERNEL_DENSE
+                           : P.kernel == "rdense" ? BBTC_KERNEL_RDENSE
+                                                  : BBTC_KERNEL_AUTO;
+  // --kernel tuned: per-task forms measured once on the first upload
+  std::vector<uint8_t> choice;
   bbtc_ctx* ctx = nullptr;
   F.begin("HIP Initial Memory Allocations and Stream Creation");
   if (P.ngpu > 0) {
@@ -252,6 +259,15 @@ int main(int argc, char** argv) {
   std::vector<uint64_t> per(ntasks, 0);
   uint64_t total = 0;
   bbtc_dgraph* dg = nullptr;
+  if (ctx && P.kernel == "tuned") {
+    F.begin("Autotune kernel forms (s)");
+    st = bbtc_graph_upload(ctx, L, &dg);
+    if (st) die(st, "uploading the graph");
+    choice.resize(std::max<uint64_t>(ntasks, 1));
+    st = bbtc_autotune(ctx, dg, tasks.data(), ntasks, choice.data(), nullptr);
+    if (st) die(st, "autotuning");
+    F.end();
+  }
   for (int r = 0; r < P.repeat; ++r) {
     if (dg) {  // as ResetFlags (bb_tc_helper.cpp:31-50), every repeat starts without device tiles
       bbtc_graph_release(dg);
@@ -269,7 +285,7 @@ int main(int argc, char** argv) {
       if (st) die(st, "uploading the graph");
     }
     st = bbtc_count_hybrid(ctx, dg, L, tasks.data(), ntasks, separator, P.threads, P.nstream, kern,
-                           per.data(), &total, nullptr);
+                           choice.empty() ? nullptr : choice.data(), per.data(), &total, nullptr);
     if (st) die(st, "counting");
     if (ctx) bbtc_ctx_synchronize(ctx);
     F.end();

@@ -35,9 +35,9 @@ struct HybridResult {
 
 HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bbtc_task* tasks,
                         uint64_t n, uint64_t sep, int nthreads, int nstreams, bbtc_kernel kernel,
-                        uint64_t* per_task) {
+                        const uint8_t* choice, uint64_t* per_task) {
   bbtc_plan* plan = nullptr;
-  const bbtc_status st = bbtc_plan_create(ctx, g, tasks, n, kernel, &plan);
+  const bbtc_status st = bbtc_plan_create_tuned(ctx, g, tasks, n, kernel, choice, &plan);
   if (st != BBTC_OK) fail(st, bbtc_last_error());
   struct Guard {
     bbtc_plan* p;
@@ -124,7 +124,7 @@ HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bb
 extern "C" bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* L,
                                          const bbtc_task* tasks, uint64_t n, uint64_t separator,
                                          int nthreads, int nstreams, bbtc_kernel kernel,
-                                         uint64_t* per_task, uint64_t* total,
+                                         const uint8_t* choice, uint64_t* per_task, uint64_t* total,
                                          uint64_t* gpu_tasks_out) {
   if (!L || (!tasks && n) || (!ctx) != (!g)) {
     bbtc::set_last_error("null argument (ctx and graph must both be set or both be NULL)");
@@ -145,7 +145,7 @@ extern "C" bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bb
       tot = bbtc::count_cpu(L->L, tasks, n, nthreads, per_task);
     } else {
       const auto r = bbtc::run_hybrid(ctx, g, L->L, tasks, n, separator, nthreads, nstreams, kernel,
-                                      per_task);
+                                      choice, per_task);
       tot = r.total;
       gt = r.gpu_tasks;
     }

@@ -313,12 +313,13 @@ void bbtc_batch_destroy(bbtc_batch* b);
  * only; tasks [separator, n) are shared, the device taking batches from the
  * front (one persistent launch per batch, up to nstreams batches in flight)
  * while nthreads-1 host workers take single tasks from the back. ctx and g
- * NULL: every task on the host (--ngpu 0). gpu_tasks_out (may be NULL)
- * receives how many leading tasks the device counted. */
+ * NULL: every task on the host (--ngpu 0). choice (may be NULL) gives
+ * per-task device forms as in bbtc_plan_create_tuned. gpu_tasks_out (may be
+ * NULL) receives how many leading tasks the device counted. */
 bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* L,
                               const bbtc_task* tasks, uint64_t n, uint64_t separator, int nthreads,
-                              int nstreams, bbtc_kernel kernel, uint64_t* per_task,
-                              uint64_t* total, uint64_t* gpu_tasks_out);
+                              int nstreams, bbtc_kernel kernel, const uint8_t* choice,
+                              uint64_t* per_task, uint64_t* total, uint64_t* gpu_tasks_out);
 
 /* ------------------------------------------------------------------------- */
 /* Serving: count requests for many graphs on one device with continuous      */

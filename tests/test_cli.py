@@ -1,0 +1,81 @@
+"""The bbtc command line on the CPU path (--ngpu 0): flags, validation and
+report lines of the reference driver (src/bbtc.cu:15-90, utils.hpp:21-154),
+counts against the golden vectors."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import bbtc_amd as B
+from tests.conftest import graph_from_spec, load_golden
+
+
+def run(args, timeout=120):
+    return subprocess.run([B.CLI_PATH] + args, capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.fixture(scope="module")
+def c1(tmp_path_factory):
+    d = load_golden("c1_rmat_s16_ef8")[0]
+    path = str(tmp_path_factory.mktemp("cli") / "c1.bbtc")
+    graph_from_spec(d["graph"]).save(path)
+    return d, path
+
+
+def test_cpu_path_counts_and_report(c1, tmp_path):
+    d, path = c1
+    for r in d["runs"]:
+        out = tmp_path / f"pt_{r['partitioner']}{r['ncut']}.txt"
+        p = run(["--graph", path, "--partitioner", r["partitioner"], "--ncut", str(r["ncut"]), "--ngpu", "0",
+                 "--repeat", "2", "--per-task-out", str(out)])
+        assert p.returncode == 0, p.stderr
+        lines = p.stdout.splitlines()
+        assert f"Graph file: {path}" in lines
+        assert f"Number of Cuts: {r['ncut']}" in lines
+        assert f"Number of Tasks: {len(r['tasks'])}" in lines
+        assert f"Number of Triangles: {r['total']}" in lines
+        assert sum(ln.startswith("Triangle Counting Execution Time (s)") for ln in lines) == 2
+        assert any(ln.startswith("Partitioning and Layout construction (s)") for ln in lines)
+        got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
+        assert got == {tuple(x[:3]): x[3] for x in r["tasks"]}
+
+
+def test_cuts_file(c1, tmp_path):
+    d, path = c1
+    r = d["runs"][0]
+    cf = tmp_path / "cuts.txt"
+    cf.write_text(" ".join(map(str, r["cuts"])))
+    p = run(["--graph", path, "--ncut", str(r["ncut"]), "--cuts-file", str(cf), "--ngpu", "0", "--repeat", "1"])
+    assert p.returncode == 0, p.stderr
+    assert f"Number of Triangles: {r['total']}" in p.stdout
+
+
+def test_flags_are_case_insensitive(c1):
+    d, path = c1
+    p = run(["--GRAPH", path, "--Partitioner", "uni", "--NCUT", "4", "--NGPU", "0", "--Repeat", "1"])
+    assert p.returncode == 0, p.stderr
+    assert f"Number of Triangles: {d['runs'][0]['total']}" in p.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--ncut", "1"], "--ncut has to be set in between"),
+    (["--nstream", "0"], "--nstream has to be set in between"),
+    (["--border", "-2"], "--border has to be set in between"),
+    (["--repeat", "0"], "--repeat has to be set in between"),
+    (["--partitioner", "xyz"], "Wrong partitioning algorithm."),
+    (["--partitioner", "rac"], "rac partitioner is not available"),
+    (["--kernel", "xyz"], "Wrong kernel."),
+    (["--bogus", "1"], "Wrong argument '--bogus' given."),
+])
+def test_validation(c1, args, msg):
+    _, path = c1
+    p = run(["--graph", path, "--ngpu", "0"] + args)
+    assert p.returncode == 2
+    assert msg in p.stdout and "--graph" in p.stdout  # message, then usage
+
+
+def test_missing_graph_and_usage(tmp_path):
+    p = run(["--graph", str(tmp_path / "none.bbtc"), "--ngpu", "0"])
+    assert p.returncode == 2 and "Graph doesn't exist" in p.stdout
+    p = run(["--help"])
+    assert p.returncode == 0 and "--partitioner" in p.stdout

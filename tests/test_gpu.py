@@ -192,7 +192,8 @@ def test_cli_gpu(tmp_path):
     path = str(tmp_path / "c1.bbtc")
     g.save(path)
     for run in d["runs"]:
-        for extra in ([], ["--border", "-1"], ["--border", "7", "--nstream", "2"], ["--copy-first"]):
+        for extra in ([], ["--border", "-1"], ["--border", "7", "--nstream", "2"], ["--copy-first"],
+                      ["--kernel", "tuned", "--border", "-1"], ["--kernel", "rdense"], ["--kernel", "dense"]):
             out = tmp_path / "pt.txt"
             cmd = [B.CLI_PATH, "--graph", path, "--partitioner", run["partitioner"], "--ncut", str(run["ncut"]),
                    "--repeat", "2", "--ngpu", "1", "--per-task-out", str(out)] + extra
