@@ -126,7 +126,7 @@ def _declare(lib):
         "bbtc_layouts_build": (st, [vp, u64, vp, i32, vp]),
         "bbtc_tune_ncut": (st, [vp, vp, u64, vp, u32, i32, i32, vp, vp]),
         "bbtc_rule_ncut": (u32, [vp, u64]),
-        "bbtc_batch_create": (st, [vp, vp, u64, i32, pvp]),
+        "bbtc_batch_create": (st, [vp, vp, u64, i32, i32, pvp]),
         "bbtc_batch_ntasks": (u64, [vp]),
         "bbtc_batch_bytes": (u64, [vp]),
         "bbtc_batch_tasks": (st, [vp, vp, vp]),
@@ -511,13 +511,15 @@ def rule_ncut(rule, m: int) -> int:
 class Batch:
     """Many graphs resident in one arena, counted by one persistent launch."""
 
-    def __init__(self, dev: "Device", layouts, kernel: str = "auto"):
+    ORDER = {"heavy_first": 0, "small_first": 1}
+
+    def __init__(self, dev: "Device", layouts, kernel: str = "auto", order: str = "heavy_first"):
         self.dev = dev
         self.layouts = list(layouts)
         n = len(self.layouts)
         lh = (C.c_void_p * max(n, 1))(*[L._h for L in self.layouts])
         h = C.c_void_p()
-        _check(lib().bbtc_batch_create(dev._h, lh, n, KERNEL[kernel], C.byref(h)))
+        _check(lib().bbtc_batch_create(dev._h, lh, n, KERNEL[kernel], self.ORDER[order], C.byref(h)))
         self._h = h
         self.ngraphs = n
 

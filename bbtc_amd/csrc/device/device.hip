@@ -722,8 +722,10 @@ void batch_enqueue(bbtc_batch* b, hipStream_t s) {
 extern "C" {
 
 bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, uint64_t ngraphs,
-                              bbtc_kernel kernel, bbtc_batch** out) {
+                              bbtc_kernel kernel, bbtc_batch_order order_kind, bbtc_batch** out) {
   DEV_REQUIRE(ctx && out && (layouts || ngraphs == 0), "null argument");
+  DEV_REQUIRE(order_kind == BBTC_BATCH_HEAVY_FIRST || order_kind == BBTC_BATCH_SMALL_GRAPHS_FIRST,
+              "unknown batch order");
   DEV_REQUIRE(bbtc::valid_kernel(kernel), "unknown kernel choice");
   DEV_REQUIRE(ngraphs < 0xFFFFFFFFull, "too many graphs");
   for (uint64_t gi = 0; gi < ngraphs; ++gi) DEV_REQUIRE(layouts[gi], "null layout");
@@ -766,6 +768,7 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
     A->tiles = bbtc::dev_alloc<bbtc::TileDesc>(ctx, std::max<uint64_t>(n_tiles, 1), &A->bytes);
     struct Order {
       float w;
+      uint64_t m;  // edges of the task's graph
       uint64_t q;
     };
     std::vector<Order> order;
@@ -803,14 +806,16 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
             b->task_ijk.insert(b->task_ijk.end(), {i, j, k});
             const uint32_t nnz_ab = uint32_t(L.nnz[L.tile(i, j)]);
             const float w = nnz_ab * std::max(L.deg[L.tile(j, k)], L.deg[L.tile(i, k)]);
-            order.push_back(Order{w, b->refs.size() - 1});
+            order.push_back(Order{w, L.m, b->refs.size() - 1});
           }
     }
     HIP_OK(hipMemcpyAsync(A->tiles, A->htiles.data(), A->htiles.size() * sizeof(bbtc::TileDesc),
                           hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    // heavy tasks of every graph first (NaN weights of empty bands last)
-    std::stable_sort(order.begin(), order.end(), [](const Order& l, const Order& r) {
+    // heavy tasks first (NaN weights of empty bands last), over all graphs or
+    // within each graph with the smallest graphs first
+    std::stable_sort(order.begin(), order.end(), [order_kind](const Order& l, const Order& r) {
+      if (order_kind == BBTC_BATCH_SMALL_GRAPHS_FIRST && l.m != r.m) return l.m < r.m;
       const float a = l.w == l.w ? l.w : -1.f, c = r.w == r.w ? r.w : -1.f;
       return a > c;
     });

@@ -285,8 +285,12 @@ uint32_t bbtc_rule_ncut(const uint32_t* rule, uint64_t m);
 typedef struct bbtc_batch bbtc_batch;
 /* Uploads n layouts into one device arena and plans every task of every
  * graph (heavy tasks of all graphs first) for one persistent launch. */
+typedef enum bbtc_batch_order {
+  BBTC_BATCH_HEAVY_FIRST = 0,        /* tasks of all graphs by weight, heaviest first */
+  BBTC_BATCH_SMALL_GRAPHS_FIRST = 1  /* graphs by edge count, smallest first; heavy tasks first within */
+} bbtc_batch_order;
 bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, uint64_t n,
-                              bbtc_kernel kernel, bbtc_batch** out);
+                              bbtc_kernel kernel, bbtc_batch_order order, bbtc_batch** out);
 uint64_t bbtc_batch_ntasks(const bbtc_batch* b);
 uint64_t bbtc_batch_bytes(const bbtc_batch* b);
 /* The batch's task order: graph index and (i,j,k) of each task (either may be NULL). */

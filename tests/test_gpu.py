@@ -270,6 +270,35 @@ def test_s26_properties(dev, s26):
     assert tot2 == tot_a
 
 
+# ---- batches -----------------------------------------------------------------
+@pytest.mark.parametrize("order", ["heavy_first", "small_first"])
+def test_batch_matches_per_graph(dev, order):
+    from bbtc_amd import workloads as W
+    specs = W.batched_specs(48, 5000)
+    graphs = [W.make_graph(dev, s) for s in specs]
+    ncut = [1, 2, 4, 8] * 12
+    Ls = B.layouts_build(graphs, ncut, "pal")
+    cpu = [L.count_cpu(L.tasks("lex")) for L in Ls]
+    b = B.Batch(dev, Ls, "auto", order=order)
+    gidx, bt = b.tasks()
+    assert len(gidx) == sum(len(c[0]) for c in cpu)
+    # per task of the batch: the CPU count of that graph's task
+    lex_index = [{tuple(x): q for q, x in enumerate(L.tasks("lex").tolist())} for L in Ls]
+    want = np.array([cpu[g][0][lex_index[g][tuple(x)]] for g, x in zip(gidx, bt.tolist())], dtype=np.uint64)
+    for mode in ["auto", "tuned", "captured"]:
+        if mode == "tuned":
+            b.autotune()
+        if mode == "captured":
+            b.capture()
+        for rep in range(3):  # replays, with host work between them
+            b.launch()
+            pg, pt, lat = b.results()
+            assert np.array_equal(pg, np.array([c[1] for c in cpu], dtype=np.uint64)), (mode, rep)
+            assert np.array_equal(pt, want), (mode, rep)
+            assert (lat > 0).all() and np.isfinite(lat).all()
+    b.destroy()
+
+
 # ---- serving -----------------------------------------------------------------
 def _serving_requests():
     """Small requests of uneven size and cut count; per-task CPU counts."""
