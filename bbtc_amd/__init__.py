@@ -80,6 +80,8 @@ def _declare(lib):
         "bbtc_last_error": (C.c_char_p, []),
         "bbtc_hgraph_load": (st, [C.c_char_p, pvp]),
         "bbtc_hgraph_save": (st, [vp, C.c_char_p]),
+        "bbtc_convert_text": (st, [C.c_char_p, C.c_char_p, i32]),
+        "bbtc_hgraph_load_text": (st, [C.c_char_p, pvp]),
         "bbtc_hgraph_from_csr": (st, [u32, u32, u64, vp, vp, pvp]),
         "bbtc_hgraph_from_edges": (st, [u32, u64, vp, vp, pvp]),
         "bbtc_hgraph_gen_rmat": (st, [u32, u32, C.c_double, C.c_double, C.c_double, u64, pvp]),
@@ -197,6 +199,12 @@ class Graph:
     @classmethod
     def load(cls, path: str) -> "Graph":
         return cls._make(lib().bbtc_hgraph_load, path.encode())
+
+    @classmethod
+    def load_text(cls, path: str) -> "Graph":
+        """Edge list or Matrix Market text, converted as the reference does
+        (duplicate entries dropped)."""
+        return cls._make(lib().bbtc_hgraph_load_text, path.encode())
 
     @classmethod
     def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1) -> "Graph":
@@ -327,6 +335,12 @@ class Layout:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.bbtc_layout_free(self._h)
             self._h = None
+
+
+def convert_text(in_path: str, out_path: str, dedup: bool = True):
+    """Writes the .bbtc the reference's driver writes for a text graph
+    (byte for byte with dedup=False)."""
+    _check(lib().bbtc_convert_text(in_path.encode(), out_path.encode(), int(dedup)))
 
 
 def device_count() -> int:

@@ -3,10 +3,13 @@
 // Same flags, defaults, validation messages and report lines as
 // src/bbtc.cu:15-90 with utils::ParseArguments (include/utils/utils.hpp:
 // 21-154) and hooks.hpp:13-52, driving libbbtc.so through its C-ABI:
-//   load .bbtc -> partition (uni|n1d|pal) -> block layout -> task list sorted
-//   by weight, descending -> per repeat: [copy] + count.
+//   load .bbtc (text input is converted first and saved as <path>.bbtc, as
+//   utils.hpp:171-268 does) -> partition (uni|n1d|pal) -> block layout ->
+//   task list sorted by weight, descending -> per repeat: [copy] + count.
 // Differences, all additive or fail-safe:
-//   * input must be a .bbtc file (text ingest is not part of this path);
+//   * text conversion drops duplicate edges unless --keep-duplicates is
+//     given (the counters accept simple graphs only); METIS .graph input is
+//     refused;
 //   * --partitioner rac is refused (not built);
 //   * argument errors exit with status 2 (the reference exits 0);
 //   * "Cuda Initial Memory Allocations and Stream Creation" is reported as
@@ -44,13 +47,14 @@ struct Params {
   int threads = 0;
   std::string cuts_file;
   std::string per_task_out;
+  bool keep_duplicates = false;
 };
 
 void usage() {
   std::cout << "bbTC: A Block Based Triangle Counting Algorithm (MI355X build)\n"
             << "syntax: bbtc --graph <path> [OPTION].. [--help]\n"
             << "Options:                                                           (Defaults:)\n"
-            << "  --graph <path>        Path for the graph file (.bbtc binary CSR).  ()\n"
+            << "  --graph <path>        Path for the graph file (.bbtc, edge list or .mtx)  ()\n"
             << "  --partitioner <str>   Name of the partitioner algorithm.         (pal)\n"
             << "                          Algorithms: [uni | n1d | pal]\n"
             << "                          uni: uniform, checkerboard partitioning\n"
@@ -69,6 +73,7 @@ void usage() {
             << "  --threads <int>       Host worker threads                        (all)\n"
             << "  --cuts-file <path>    Read the p+1 cuts from a text file instead of partitioning\n"
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
+            << "  --keep-duplicates     Keep repeated edges when converting text input\n"
             << "\nExamples:\n"
             << "  bbtc --graph email-EuAll_adj.bbtc --partitioner n1d --ncut 4\n"
             << "  bbtc --graph email-EuAll_adj.bbtc --ncut 4 --copy-first --border 4\n";
@@ -148,6 +153,7 @@ Params parse(int argc, char** argv) {
       check_int("--threads", p.threads);
     } else if (!strcasecmp(a, "--cuts-file")) p.cuts_file = val("--cuts-file");
     else if (!strcasecmp(a, "--per-task-out")) p.per_task_out = val("--per-task-out");
+    else if (!strcasecmp(a, "--keep-duplicates")) p.keep_duplicates = true;
     else if (!strcasecmp(a, "--help")) {
       usage();
       std::exit(0);
@@ -189,8 +195,15 @@ int main(int argc, char** argv) {
   // utils.hpp:168-170: a sibling "<path>.bbtc" is preferred when it exists
   if (file_exists(path + ".bbtc")) path += ".bbtc";
   else if (!ends_with(path, ".bbtc")) {
-    std::cerr << "bbtc: only .bbtc binary CSR input is supported (got " << path << ")" << std::endl;
-    return 1;
+    // utils.hpp:171-268: convert the text graph and save it beside the input
+    std::cerr << "# Saving binary CSR graph.." << std::endl;
+    const bbtc_status cs = bbtc_convert_text(path.c_str(), (path + ".bbtc").c_str(), P.keep_duplicates ? 0 : 1);
+    if (cs) {
+      std::cerr << "bbtc: converting " << path << ": " << bbtc_strerror(cs) << ": " << bbtc_last_error()
+                << std::endl;
+      return 1;
+    }
+    path += ".bbtc";
   }
   bbtc_hgraph* g = nullptr;
   bbtc_status st = bbtc_hgraph_load(path.c_str(), &g);

@@ -16,6 +16,10 @@ void validate_csr(const Csr& g);
 Csr load_bbtc(const std::string& path);
 void save_bbtc(const Csr& g, const std::string& path);
 Csr copy_csr(uint32_t n, uint32_t ncols, uint64_t m, const uint64_t* off, const uint32_t* col);
+// Edge list or Matrix Market text to the canonical upper-triangular CSR, as
+// the reference's driver converts it; dedup drops repeated entries (which
+// the reference keeps).
+Csr csr_from_text(const std::string& path, bool dedup);
 
 // ---- generators -----------------------------------------------------------
 // Canonical input from an undirected pair list: symmetrise, drop self loops

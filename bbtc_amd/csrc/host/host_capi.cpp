@@ -64,6 +64,22 @@ bbtc_status bbtc_hgraph_load(const char* path, bbtc_hgraph** out) {
   BBTC_CATCH
 }
 
+bbtc_status bbtc_convert_text(const char* in_path, const char* out_path, int dedup) {
+  REQUIRE(in_path && out_path, "null argument");
+  BBTC_TRY
+  bbtc::save_bbtc(bbtc::csr_from_text(in_path, dedup != 0), out_path);
+  BBTC_CATCH
+}
+
+bbtc_status bbtc_hgraph_load_text(const char* path, bbtc_hgraph** out) {
+  REQUIRE(path && out, "null argument");
+  BBTC_TRY
+  bbtc::Csr g = bbtc::csr_from_text(path, true);
+  bbtc::validate_csr(g);
+  *out = new bbtc_hgraph{std::move(g)};
+  BBTC_CATCH
+}
+
 bbtc_status bbtc_hgraph_save(const bbtc_hgraph* g, const char* path) {
   REQUIRE(g && path, "null argument");
   BBTC_TRY

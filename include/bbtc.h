@@ -53,6 +53,18 @@ bbtc_status bbtc_hgraph_load(const char* path, bbtc_hgraph** out);
  * utils.hpp:260-268. Writes L=4 and O=4 when m < 2^32, else O=8. */
 bbtc_status bbtc_hgraph_save(const bbtc_hgraph* g, const char* path);
 
+/* Replaces the text branch of utils::ReadGraph (utils.hpp:171-268) with
+ * PIGO's edge-list / Matrix Market reader (coo.impl.hpp:318-466,
+ * pigo.impl.hpp:184-345) and SARMA's degree order (csr_matrix.hpp:503-584):
+ * symmetrise (diagonal dropped), rank by degree ascending with the same
+ * std::sort over (id, degree) pairs, keep the upper triangle. Writes the
+ * .bbtc file the reference writes, byte for byte, when dedup is 0; dedup 1
+ * also drops repeated entries (the reference keeps them, and the counters
+ * here accept only simple graphs). METIS .graph input is refused. */
+bbtc_status bbtc_convert_text(const char* in_path, const char* out_path, int dedup);
+/* The same conversion (with dedup) straight into a host graph. */
+bbtc_status bbtc_hgraph_load_text(const char* path, bbtc_hgraph** out);
+
 /* Copies a caller CSR (validated like bbtc_hgraph_load). */
 bbtc_status bbtc_hgraph_from_csr(uint32_t n, uint32_t ncols, uint64_t m,
                                  const uint64_t* offsets, const uint32_t* cols,
