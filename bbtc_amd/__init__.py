@@ -25,7 +25,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbbtc.so")
 CLI_PATH = os.path.join(_HERE, "bbtc")
 
-PART = {"uni": 0, "n1d": 1, "pal": 2}
+PART = {"uni": 0, "n1d": 1, "pal": 2, "rac": 3}
 ORDER = {"lex": 0, "desc": 1, "asc": 2}
 KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4}
 

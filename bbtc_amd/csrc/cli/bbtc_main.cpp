@@ -10,7 +10,6 @@
 //   * text conversion drops duplicate edges unless --keep-duplicates is
 //     given (the counters accept simple graphs only); METIS .graph input is
 //     refused;
-//   * --partitioner rac is refused (not built);
 //   * argument errors exit with status 2 (the reference exits 0);
 //   * "Cuda Initial Memory Allocations and Stream Creation" is reported as
 //     "HIP Initial Memory Allocations and Stream Creation";
@@ -54,11 +53,14 @@ void usage() {
   std::cout << "bbTC: A Block Based Triangle Counting Algorithm (MI355X build)\n"
             << "syntax: bbtc --graph <path> [OPTION].. [--help]\n"
             << "Options:                                                           (Defaults:)\n"
-            << "  --graph <path>        Path for the graph file (.bbtc, edge list or .mtx)  ()\n"
+            << "  --graph <path>        Path for the graph file.                   ()\n"
+            << "                          Supported formats: edge lists, matrix\n"
+            << "                          market files and binary CSR.\n"
             << "  --partitioner <str>   Name of the partitioner algorithm.         (pal)\n"
             << "                          Algorithms: [uni | n1d | pal]\n"
             << "                          uni: uniform, checkerboard partitioning\n"
             << "                          n1d: Nicol's 1D Partitioning\n"
+            << "                          rac: refine a cut partitioning\n"
             << "                          pal: probe a load partitioning\n"
             << "  --ncut <int>          Number of cuts                             (32)\n"
             << "  --ngpu <int>          Number of GPUs to use                      (all)\n"
@@ -75,8 +77,8 @@ void usage() {
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
             << "  --keep-duplicates     Keep repeated edges when converting text input\n"
             << "\nExamples:\n"
-            << "  bbtc --graph email-EuAll_adj.bbtc --partitioner n1d --ncut 4\n"
-            << "  bbtc --graph email-EuAll_adj.bbtc --ncut 4 --copy-first --border 4\n";
+            << "  bbtc --graph email-EuAll_adj.mtx --partitioner n1d --ncut 4\n"
+            << "  bbtc --graph email-EuAll_adj.txt --ncut 4 --copy-first --border 4\n";
 }
 
 [[noreturn]] void bad(const std::string& msg) {
@@ -141,8 +143,8 @@ Params parse(int argc, char** argv) {
       check_int("--ncut", p.ncut, 1);
     } else if (!strcasecmp(a, "--partitioner")) {
       p.part = val("--partitioner");
-      if (p.part == "rac") bad("The rac partitioner is not available in this build.");
-      if (p.part != "pal" && p.part != "n1d" && p.part != "uni") bad("Wrong partitioning algorithm.");
+      if (p.part != "pal" && p.part != "n1d" && p.part != "uni" && p.part != "rac")
+        bad("Wrong partitioning algorithm.");
     } else if (!strcasecmp(a, "--kernel")) {
       p.kernel = val("--kernel");
       if (p.kernel != "auto" && p.kernel != "merge" && p.kernel != "bitmap" && p.kernel != "dense" &&
@@ -225,7 +227,10 @@ int main(int argc, char** argv) {
         return 1;
       }
   } else {
-    const bbtc_part kind = P.part == "uni" ? BBTC_PART_UNI : P.part == "n1d" ? BBTC_PART_N1D : BBTC_PART_PAL;
+    const bbtc_part kind = P.part == "uni"   ? BBTC_PART_UNI
+                           : P.part == "n1d" ? BBTC_PART_N1D
+                           : P.part == "rac" ? BBTC_PART_RAC
+                                             : BBTC_PART_PAL;
     st = bbtc_partition(g, kind, p, cuts.data());
     if (st) die(st, "partitioning");
   }

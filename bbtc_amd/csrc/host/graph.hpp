@@ -32,6 +32,7 @@ Csr gen_gnm(uint32_t n, uint64_t m, uint64_t seed);
 std::vector<uint32_t> partition_uni(const Csr& g, uint32_t p);
 std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t p);
 std::vector<uint32_t> partition_pal(const Csr& g, uint32_t p);
+std::vector<uint32_t> partition_rac(const Csr& g, uint32_t p);
 // Dispatch + cuts[p] = n + 1, as the reference driver does.
 std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p);
 

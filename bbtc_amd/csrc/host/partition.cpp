@@ -52,19 +52,15 @@ std::vector<u32> n1d_probe(const std::vector<u32>& prefix, u32 L, const std::vec
   return p;
 }
 
-}  // namespace
-
-std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t P) {
-  // The weight of element i is indptr[i] itself (the reference passes the
-  // offset array, utils.hpp:281) and the prefix is accumulated in 32 bits,
-  // wrapping (nicol1d.hpp:211-214).
-  const size_t asz = size_t(g.n) + 1;
+// Optimal 1D cuts of a prefix array (nicol1d.hpp:106-195, one array): prefix
+// has N + 1 entries (prefix[0] = 0) accumulated in 32 bits; returns P + 1
+// cuts over [0, N].
+std::vector<u32> nicol_prefix(std::vector<u32> prefix, u32 P) {
+  const u32 N = u32(prefix.size() - 1);
   // One trailing zero: partition_indices reads prefix[l + 1] with l == N at
   // the last cut (nicol1d.hpp:144), one element past the array; the zero
   // filled pages a large allocation sits in supply 0 there.
-  std::vector<u32> prefix(asz + 2, 0);
-  for (size_t i = 0; i < asz; ++i) prefix[i + 1] = prefix[i] + u32(g.off[i]);
-  const u32 N = u32(asz);  // prefix.size() - 1 in the reference
+  prefix.push_back(0);
   std::vector<u32> UB(size_t(P) + 1, N);
   u32 lower = prefix[N] / P, upper = prefix[N];
   auto cub = [&](u32 l, u32 r) { return u32(prefix[r] - prefix[l]); };
@@ -91,6 +87,94 @@ std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t P) {
     pback = l;
   }
   return n1d_probe(prefix, upper, UB);
+}
+
+}  // namespace
+
+std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t P) {
+  // The weight of element i is indptr[i] itself (the reference passes the
+  // offset array, utils.hpp:281) and the prefix is accumulated in 32 bits,
+  // wrapping (nicol1d.hpp:211-214).
+  const size_t asz = size_t(g.n) + 1;
+  std::vector<u32> prefix(asz + 1, 0);
+  for (size_t i = 0; i < asz; ++i) prefix[i + 1] = prefix[i] + u32(g.off[i]);
+  return nicol_prefix(std::move(prefix), P);
+}
+
+// ---- rac: sarma::refine_a_cut::partition(A, P) (refine_a_cut.hpp:43-223) -------
+namespace {
+
+// Rows of the transpose of an n x n CSR, each sorted.
+void transpose_rows(const Csr& g, std::vector<u32>& tp, std::vector<u32>& ti) {
+  const u32 n = g.n;
+  tp.assign(size_t(n) + 1, 0);
+  for (uint64_t e = 0; e < g.m; ++e) tp[size_t(g.col[e]) + 1]++;
+  for (size_t i = 1; i < tp.size(); ++i) tp[i] += tp[i - 1];
+  ti.assign(g.m, 0);
+  std::vector<u32> pos(tp.begin(), tp.end() - 1);
+  for (u32 r = 0; r < n; ++r)
+    for (uint64_t e = g.off[r]; e < g.off[r + 1]; ++e) ti[pos[g.col[e]]++] = r;  // rows ascending
+}
+
+// refine_a_cut.hpp:43-73: per row, the largest count of its entries inside
+// one interval of prev; prefix over rows in 32 bits; optimal 1D cuts.
+template <class Off>
+std::vector<u32> rac_refine(u32 n, const Off* off, const u32* col, const std::vector<u32>& prev, u32 P) {
+  std::vector<u32> prefix(size_t(n) + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    u32 max_i = 0, max_k = 0, k = 0;
+    for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
+      while (col[j] >= prev[k]) {
+        max_i = std::max(max_i, max_k);
+        max_k = 0;
+        ++k;
+      }
+      max_k += 1;
+    }
+    prefix[size_t(i) + 1] = std::max(max_i, max_k);
+  }
+  for (size_t i = 1; i < prefix.size(); ++i) prefix[i] += prefix[i - 1];
+  return nicol_prefix(std::move(prefix), P);
+}
+
+// Matrix::compute_maxload(c, c) (csr_matrix.hpp:281-322): the largest tile
+// entry count with the same cuts on rows and columns.
+u32 maxload(const Csr& g, const std::vector<u32>& c) {
+  const size_t B = c.size() - 1;
+  std::vector<u32> load(B * B, 0);
+  size_t u = 0;
+  for (u32 i = 0; i < g.n; ++i) {
+    while (u < B && i >= c[u]) ++u;
+    for (uint64_t e = g.off[i]; e < g.off[i + 1]; ++e) {
+      const size_t q = size_t(std::lower_bound(c.begin(), c.end(), g.col[e] + 1) - c.begin()) - 1;
+      load[(u - 1) * B + q] += 1;
+    }
+  }
+  return *std::max_element(load.begin(), load.end());
+}
+
+}  // namespace
+
+std::vector<uint32_t> partition_rac(const Csr& g, uint32_t P) {
+  const u32 n = g.n;
+  std::vector<u32> tp, ti;
+  transpose_rows(g, tp, ti);
+  std::vector<u32> cx(size_t(P) + 1, n), cy(size_t(P) + 1, n);
+  cx[0] = cy[0] = 0;
+  auto refine_x = [&]() { return rac_refine(n, g.off.data(), g.col.data(), cx, P); };
+  auto refine_y = [&]() { return rac_refine(n, tp.data(), ti.data(), cy, P); };
+  // direction with the smaller max tile load after one refinement
+  // (refine_a_cut.hpp:153-166), then max(10, P) refinements along it
+  const u32 max_x = maxload(g, refine_x());
+  const u32 max_y = maxload(g, refine_y());
+  const bool along_x = max_x <= max_y;
+  const int iters = std::max(10, int(P));
+  for (int it = 0; it < iters; ++it) {
+    if (along_x) cx = refine_x();
+    else cy = refine_y();
+  }
+  return along_x ? cx : cy;
 }
 
 // ---- pal: sparsify + sarma::probe_a_load::partition ----------------------------
@@ -253,6 +337,7 @@ std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p) {
     case BBTC_PART_UNI: c = partition_uni(g, p); break;
     case BBTC_PART_N1D: c = partition_n1d(g, p); break;
     case BBTC_PART_PAL: c = partition_pal(g, p); break;
+    case BBTC_PART_RAC: c = partition_rac(g, p); break;
     default: fail(BBTC_ERR_INVALID, "unknown partitioner");
   }
   if (c.size() != size_t(p) + 1) fail(BBTC_ERR_INTERNAL, "partitioner returned a wrong cut count");

@@ -93,13 +93,20 @@ void bbtc_hgraph_free(bbtc_hgraph* g);
 /* ------------------------------------------------------------------------ */
 /* Partitioners.                                                             */
 /* ------------------------------------------------------------------------ */
-typedef enum bbtc_part { BBTC_PART_UNI = 0, BBTC_PART_N1D = 1, BBTC_PART_PAL = 2 } bbtc_part;
+typedef enum bbtc_part {
+  BBTC_PART_UNI = 0,
+  BBTC_PART_N1D = 1,
+  BBTC_PART_PAL = 2,
+  BBTC_PART_RAC = 3
+} bbtc_part;
 
 /* Replaces the partition dispatch of utils.hpp:271-288:
  *   uni -> sarma::uniform::partition (uniform.hpp:20-27, 44-49)
  *   n1d -> sarma::nicol1d::partition(indptr) (nicol1d.hpp:106-216)
  *   pal -> sparsify (csr_matrix.hpp:435-454, tools/utils.hpp:20-25, 81-106)
  *          then sarma::probe_a_load::partition (probe_a_load.hpp:29-89)
+ *   rac -> sarma::refine_a_cut::partition(A, p) on the whole matrix
+ *          (refine_a_cut.hpp:43-73, 129-223; max(10, p) refinements)
  * Writes p+1 cuts into cuts_out exactly as the driver uses them, i.e. with
  * cuts_out[p] overwritten by n+1 (utils.hpp:288). Requires p >= 1. */
 bbtc_status bbtc_partition(const bbtc_hgraph* g, bbtc_part kind, uint32_t p, uint32_t* cuts_out);

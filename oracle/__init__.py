@@ -33,6 +33,7 @@ def lib():
         L.orc_free_csr.argtypes = [C.POINTER(_Csr)]
         L.orc_cuts_uni.argtypes = [u32, u32, vp]
         L.orc_cuts_n1d.argtypes = [vp, u32, u32, vp]
+        L.orc_cuts_rac.argtypes = [vp, vp, u32, u32, vp]
         L.orc_cuts_pal.argtypes = [vp, vp, u32, u32, u64, u32, vp]
         L.orc_bcsr_build.argtypes = [vp, vp, u32, u64, vp, u32]
         L.orc_bcsr_build.restype = vp
@@ -75,7 +76,7 @@ def load_bbtc(path):
 
 
 def cuts(kind, off, col, p, n=None, ncols=None):
-    """Driver cuts (cuts[p] = n+1) for uni / n1d / pal."""
+    """Driver cuts (cuts[p] = n+1) for uni / n1d / pal / rac."""
     off = np.ascontiguousarray(off, dtype=np.uint64)
     col = np.ascontiguousarray(col, dtype=np.uint32)
     n = len(off) - 1 if n is None else n
@@ -87,6 +88,8 @@ def cuts(kind, off, col, p, n=None, ncols=None):
         lib().orc_cuts_n1d(_p(off), n, p, _p(out))
     elif kind == "pal":
         lib().orc_cuts_pal(_p(off), _p(col), n, ncols, int(off[-1]), p, _p(out))
+    elif kind == "rac":
+        lib().orc_cuts_rac(_p(off), _p(col), n, p, _p(out))
     else:
         raise ValueError(kind)
     out[p] = n + 1

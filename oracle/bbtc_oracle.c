@@ -10,6 +10,7 @@
  *   - .bbtc reader            PIGO csr.impl.hpp:475-512
  *   - uni cuts                SARMA algorithms/uniform.hpp:20-27
  *   - n1d cuts                SARMA algorithms/nicol1d.hpp:30-216
+ *   - rac cuts                SARMA algorithms/refine_a_cut.hpp:43-73, 129-223
  *   - pal cuts                SARMA tools/utils.hpp:20-25, 81-106 (sample),
  *                             csr_matrix.hpp:435-454 (sparsify),
  *                             sparse_prefix_sum.hpp:318-387 (loads),
@@ -139,13 +140,10 @@ static void n1d_probe(const uint32_t* prefix, uint32_t L, const uint32_t* UB, ui
   }
 }
 
-ORC_API void orc_cuts_n1d(const uint64_t* off, uint32_t n, uint32_t p, uint32_t* cuts) {
-  const uint64_t asz = (uint64_t)n + 1;
-  /* prefix of the 32-bit offsets, plus one zero past the end (the reference
-   * reads prefix[N+1] at its last cut, nicol1d.hpp:144) */
-  uint32_t* prefix = (uint32_t*)calloc(asz + 2, 4);
-  for (uint64_t i = 0; i < asz; ++i) prefix[i + 1] = prefix[i] + (uint32_t)off[i];
-  const uint32_t N = (uint32_t)asz;
+/* Nicol's optimal 1D cuts (nicol1d.hpp:106-195) of prefix[0..N]; prefix
+ * holds one more zero at N+1 (the reference reads prefix[N+1] at its last
+ * cut, nicol1d.hpp:144). Writes p+1 cuts. */
+static void nicol_core(const uint32_t* prefix, uint32_t N, uint32_t p, uint32_t* cuts) {
   uint32_t* UB = (uint32_t*)malloc(4ull * (p + 1));
   uint32_t* pr = (uint32_t*)malloc(4ull * (p + 1));
   for (uint32_t i = 0; i <= p; ++i) UB[i] = N;
@@ -172,9 +170,98 @@ ORC_API void orc_cuts_n1d(const uint64_t* off, uint32_t n, uint32_t p, uint32_t*
     pback = l;
   }
   n1d_probe(prefix, upper, UB, p + 1, cuts);
-  free(prefix);
   free(UB);
   free(pr);
+}
+
+ORC_API void orc_cuts_n1d(const uint64_t* off, uint32_t n, uint32_t p, uint32_t* cuts) {
+  const uint64_t asz = (uint64_t)n + 1;
+  /* prefix of the 32-bit offsets (plus the zero past the end) */
+  uint32_t* prefix = (uint32_t*)calloc(asz + 2, 4);
+  for (uint64_t i = 0; i < asz; ++i) prefix[i + 1] = prefix[i] + (uint32_t)off[i];
+  nicol_core(prefix, (uint32_t)asz, p, cuts);
+  free(prefix);
+}
+
+/* ------------------------------------------------------------------------ */
+/* rac: refine a cut (refine_a_cut.hpp:43-73, 129-223) on the whole matrix    */
+/* ------------------------------------------------------------------------ */
+/* One refinement: per row the largest number of its entries inside one
+ * interval of prev, a 32-bit prefix over rows, then Nicol's cuts. */
+static void rac_step(const uint64_t* off, const uint32_t* col, uint32_t n, const uint32_t* prev,
+                     uint32_t p, uint32_t* out) {
+  uint32_t* prefix = (uint32_t*)calloc((size_t)n + 2, 4);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t best = 0, run = 0, k = 0;
+    for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
+      while (col[j] >= prev[k]) {
+        if (run > best) best = run;
+        run = 0;
+        ++k;
+      }
+      ++run;
+    }
+    prefix[i + 1] = run > best ? run : best;
+  }
+  for (uint32_t i = 1; i <= n; ++i) prefix[i] += prefix[i - 1];
+  nicol_core(prefix, n, p, out);
+  free(prefix);
+}
+
+/* Largest tile entry count with the same cuts on rows and columns
+ * (csr_matrix.hpp:281-322). */
+static uint32_t rac_maxload(const uint64_t* off, const uint32_t* col, uint32_t n, const uint32_t* c,
+                            uint32_t p) {
+  uint32_t* load = (uint32_t*)calloc((size_t)p * p, 4);
+  uint32_t rb = 0, mx = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    while (rb < p && i >= c[rb + 1]) ++rb;  /* band of row i: c[rb] <= i < c[rb+1] */
+    for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
+      uint32_t cb = 0;
+      while (cb + 1 < p && col[j] >= c[cb + 1]) ++cb;
+      const uint32_t v = ++load[(size_t)rb * p + cb];
+      if (v > mx) mx = v;
+    }
+  }
+  free(load);
+  return mx;
+}
+
+ORC_API void orc_cuts_rac(const uint64_t* off, const uint32_t* col, uint32_t n, uint32_t p, uint32_t* cuts) {
+  const uint64_t m = off[n];
+  /* the transpose, rows ascending */
+  uint64_t* toff = (uint64_t*)calloc((size_t)n + 1, 8);
+  uint32_t* tcol = (uint32_t*)malloc(4ull * (m ? m : 1));
+  for (uint64_t e = 0; e < m; ++e) toff[col[e] + 1]++;
+  for (uint32_t i = 1; i <= n; ++i) toff[i] += toff[i - 1];
+  uint64_t* fill = (uint64_t*)malloc(8ull * (n ? n : 1));
+  for (uint32_t i = 0; i < n; ++i) fill[i] = toff[i];
+  for (uint32_t r = 0; r < n; ++r)
+    for (uint64_t e = off[r]; e < off[r + 1]; ++e) tcol[fill[col[e]]++] = r;
+  free(fill);
+  uint32_t* cx = (uint32_t*)malloc(4ull * (p + 1));
+  uint32_t* cy = (uint32_t*)malloc(4ull * (p + 1));
+  uint32_t* t = (uint32_t*)malloc(4ull * (p + 1));
+  for (uint32_t i = 0; i <= p; ++i) cx[i] = cy[i] = n;
+  cx[0] = cy[0] = 0;
+  rac_step(off, col, n, cx, p, t);
+  const uint32_t mx = rac_maxload(off, col, n, t, p);
+  rac_step(toff, tcol, n, cy, p, t);
+  const uint32_t my = rac_maxload(off, col, n, t, p);
+  const int iters = p > 10 ? (int)p : 10;
+  uint32_t* c = mx <= my ? cx : cy;
+  for (int it = 0; it < iters; ++it) {
+    if (mx <= my) rac_step(off, col, n, c, p, t);
+    else rac_step(toff, tcol, n, c, p, t);
+    memcpy(c, t, 4ull * (p + 1));
+  }
+  memcpy(cuts, c, 4ull * (p + 1));
+  cuts[p] = n + 1;  /* the driver, utils.hpp:288 */
+  free(cx);
+  free(cy);
+  free(t);
+  free(toff);
+  free(tcol);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -33,25 +33,25 @@ RMAT = dict(a=0.57, b=0.19, c=0.19)
 # (name, graph spec, [(partitioner, ncut, cuts_only)])
 FIXTURES = [
     ("c1_rmat_s16_ef8", dict(kind="rmat", scale=16, ef=8, seed=1, **RMAT),
-     [("uni", 4, False), ("n1d", 4, False), ("pal", 4, False)]),
+     [("uni", 4, False), ("n1d", 4, False), ("pal", 4, False), ("rac", 4, False)]),
     ("c2_gnm_n22_m26", dict(kind="gnm", n=1 << 22, m=1 << 26, seed=2),
      [("pal", 16, False)]),
     ("small_rmat_s8", dict(kind="rmat", scale=8, ef=8, seed=11, **RMAT),
-     [("uni", 2, False), ("n1d", 3, False), ("pal", 5, False), ("pal", 8, False)]),
+     [("uni", 2, False), ("n1d", 3, False), ("pal", 5, False), ("pal", 8, False), ("rac", 3, False)]),
     ("small_rmat_s10", dict(kind="rmat", scale=10, ef=16, seed=12, **RMAT),
-     [("uni", 3, False), ("n1d", 4, False), ("pal", 4, False), ("pal", 8, False)]),
+     [("uni", 3, False), ("n1d", 4, False), ("pal", 4, False), ("pal", 8, False), ("rac", 8, False)]),
     ("small_rmat_s12", dict(kind="rmat", scale=12, ef=24, seed=13, **RMAT),
-     [("uni", 8, False), ("n1d", 8, False), ("pal", 2, False), ("pal", 8, False)]),
+     [("uni", 8, False), ("n1d", 8, False), ("pal", 2, False), ("pal", 8, False), ("rac", 5, False)]),
     ("small_gnm_2k", dict(kind="gnm", n=2000, m=16000, seed=14),
-     [("uni", 5, False), ("n1d", 2, False), ("pal", 6, False)]),
+     [("uni", 5, False), ("n1d", 2, False), ("pal", 6, False), ("rac", 6, False)]),
     ("small_gnm_dense", dict(kind="gnm", n=600, m=60000, seed=15),
-     [("uni", 4, False), ("n1d", 4, False), ("pal", 4, False)]),
+     [("uni", 4, False), ("n1d", 4, False), ("pal", 4, False), ("rac", 4, False)]),
     ("batch_rmat_s11", dict(kind="rmat", scale=11, ef=20, seed=1003, **RMAT),
      [("pal", 2, False), ("pal", 4, False), ("pal", 8, False)]),
     ("batch_gnm_4k", dict(kind="gnm", n=4096, m=4096 * 20, seed=1007),
      [("pal", 2, False), ("pal", 4, False), ("pal", 8, False)]),
     ("rmat_s18_ef16", dict(kind="rmat", scale=18, ef=16, seed=7, **RMAT),
-     [("pal", 16, False), ("n1d", 16, True), ("uni", 16, True)]),
+     [("pal", 16, False), ("n1d", 16, True), ("uni", 16, True), ("rac", 16, True)]),
 ]
 S26 = ("c3_rmat_s26_ef16_cuts", dict(kind="rmat", scale=26, ef=16, seed=3, **RMAT),
        [("pal", 32, True)])

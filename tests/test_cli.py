@@ -40,6 +40,18 @@ def test_cpu_path_counts_and_report(c1, tmp_path):
         assert got == {tuple(x[:3]): x[3] for x in r["tasks"]}
 
 
+def test_rac_partitioner(c1, tmp_path):
+    d, path = c1
+    r = next(x for x in d["runs"] if x["partitioner"] == "rac")
+    out = tmp_path / "pt.txt"
+    p = run(["--graph", path, "--partitioner", "rac", "--ncut", str(r["ncut"]), "--ngpu", "0", "--repeat", "1",
+             "--per-task-out", str(out)])
+    assert p.returncode == 0, p.stderr
+    assert f"Number of Triangles: {r['total']}" in p.stdout
+    got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
+    assert got == {tuple(x[:3]): x[3] for x in r["tasks"]}
+
+
 def test_cuts_file(c1, tmp_path):
     d, path = c1
     r = d["runs"][0]
@@ -63,7 +75,6 @@ def test_flags_are_case_insensitive(c1):
     (["--border", "-2"], "--border has to be set in between"),
     (["--repeat", "0"], "--repeat has to be set in between"),
     (["--partitioner", "xyz"], "Wrong partitioning algorithm."),
-    (["--partitioner", "rac"], "rac partitioner is not available"),
     (["--kernel", "xyz"], "Wrong kernel."),
     (["--bogus", "1"], "Wrong argument '--bogus' given."),
 ])
