@@ -419,17 +419,17 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       }
       wave_lds_sync();
       wedges(wlo, hi, uint32_t(lane) * T.wpr, lds);
-      // restore the all-zero slice
+      // restore the all-zero slice: the bitmap rows of the staged rows (a
+      // contiguous lane range), 16 bytes per store, without reading ac(u)
+      // from memory again
       wave_lds_sync();
-      for (uint32_t s = 0; s < A; s += kWave) {
-        const uint32_t e = s + lane;
-        const int q = owner_of(inclA, e);
-        const uint32_t q_ac0 = __shfl(ac0, q, kWave);
-        const uint32_t q_ex = __shfl(exclA, q, kWave);
-        if (e < A) {
-          const uint32_t x = ac_c[q_ac0 + (e - q_ex)];
-          lds[uint32_t(q) * T.wpr + (x >> 5)] = 0;
-        }
+      const uint64_t staged = __ballot(dact != 0);
+      if (staged) {
+        const uint32_t r0 = uint32_t(__ffsll((long long)staged) - 1);
+        const uint32_t r1 = uint32_t(64 - __clzll(staged));
+        uint4* z = reinterpret_cast<uint4*>(lds + r0 * T.wpr);
+        const uint32_t nz = (r1 - r0) * T.wpr / 4;
+        for (uint32_t x = uint32_t(lane); x < nz; x += kWave) z[x] = make_uint4(0, 0, 0, 0);
       }
     } else {
       // Sorted ac(u) lists, a window of consecutive rows at a time: as many

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -462,7 +463,10 @@ void server_free(bbtc_server* S) {
   for (auto* p : S->seg_dev) (void)hipFree(p);
   for (auto* p : S->seg_host) (void)hipHostFree(p);
   for (auto e : S->event_pool) (void)hipEventDestroy(e);
-  if (S->heap_base) (void)hipFree(S->heap_base);
+  if (S->heap_base) {
+    const hipError_t e = hipFree(S->heap_base);
+    if (e != hipSuccess) std::fprintf(stderr, "bbtc: freeing the server heap failed: %s\n", hipGetErrorString(e));
+  }
   if (S->ctl) (void)hipFree(S->ctl);
   if (S->spill) (void)hipFree(S->spill);
   if (S->spill_base) (void)hipFree(S->spill_base);
@@ -513,7 +517,8 @@ bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc
     if (ctx->budget && ctx->in_use + heap > ctx->budget)
       bbtc::fail(BBTC_ERR_BUDGET, "server heap exceeds the context's device budget");
     if (heap + (1ull << 30) > free_b)
-      bbtc::fail(BBTC_ERR_BUDGET, "server heap of " + std::to_string(heap) + " bytes exceeds free device memory");
+      bbtc::fail(BBTC_ERR_BUDGET, "server heap of " + std::to_string(heap) + " bytes exceeds free device memory (" +
+                                      std::to_string(free_b) + " of " + std::to_string(total_b) + " bytes free)");
     S->quantum = opts && opts->quantum_items ? opts->quantum_items : bbtc::kDefaultQuantum;
     S->max_inflight = opts && opts->max_inflight ? opts->max_inflight : bbtc::kDefaultInflight;
     S->max_segments = opts && opts->max_segments ? opts->max_segments : bbtc::kDefaultSegments;

@@ -217,11 +217,14 @@ def run_plain(dev: Device, layouts, tasks, arrivals=None):
     return tot, lat, time.perf_counter() - t0, per
 
 
-def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200 << 30, **opts):
-    """Every request through one continuous-batching server. Returns (totals,
-    latencies s, makespan s, per-task counts, server statistics)."""
+def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200 << 30, srv=None, **opts):
+    """Every request through one continuous-batching server (`srv`, or a new
+    one destroyed afterwards). Returns (totals, latencies s, makespan s,
+    per-task counts, server statistics of this run)."""
     n = len(layouts)
-    srv = Server(dev, heap_bytes=heap_bytes, **opts)
+    own = srv is None
+    if own:
+        srv = Server(dev, heap_bytes=heap_bytes, **opts)
     try:
         srv.reset_clock()
         ids = [srv.submit(layouts[x], tasks[x], 0.0 if arrivals is None else float(arrivals[x]))
@@ -229,7 +232,8 @@ def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200
         res = {r.id: r for r in srv.drain()}
         stats = srv.stats()
     finally:
-        srv.destroy()
+        if own:
+            srv.destroy()
     rs = [res[i] for i in ids]
     bad = [r.status for r in rs if r.status != 0]
     if bad:
@@ -254,7 +258,16 @@ def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, qua
     p_tot, _, p_make, p_per = run_plain(dev, layouts, tasks)
     p_cap = n / p_make
     log(f"plain saturated: {p_cap:.2f} graphs/s ({p_make:.2f}s for {n})")
-    s_tot, _, s_make, s_per, st = run_server(dev, layouts, tasks, quantum_items=quantum_items)
+    # one server (one heap reservation) for both server runs
+    srv = Server(dev, heap_bytes=200 << 30, quantum_items=quantum_items)
+    try:
+        return _serving_runs(dev, layouts, tasks, srv, n, p_tot, p_per, p_cap, rate, load, log)
+    finally:
+        srv.destroy()
+
+
+def _serving_runs(dev, layouts, tasks, srv, n, p_tot, p_per, p_cap, rate, load, log):
+    s_tot, _, s_make, s_per, st = run_server(dev, layouts, tasks, srv=srv)
     s_cap = n / s_make
     log(f"server saturated: {s_cap:.2f} graphs/s ({s_make:.2f}s); {st}")
     if not np.array_equal(p_tot, s_tot) or not all(np.array_equal(a, b) for a, b in zip(p_per, s_per)):
@@ -263,7 +276,7 @@ def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, qua
     lam = rate if rate else load * p_cap
     arr = poisson_arrivals(n, lam)
     p_tot2, p_lat, p_make2, _ = run_plain(dev, layouts, tasks, arr)
-    s_tot2, s_lat, s_make2, _, st2 = run_server(dev, layouts, tasks, arr, quantum_items=quantum_items)
+    s_tot2, s_lat, s_make2, _, st2 = run_server(dev, layouts, tasks, arr, srv=srv)
     if not (np.array_equal(p_tot2, p_tot) and np.array_equal(s_tot2, p_tot)):
         raise RuntimeError("counts at the Poisson rate differ from the saturated runs")
     res = dict(
