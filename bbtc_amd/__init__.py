@@ -98,6 +98,8 @@ def _declare(lib):
         "bbtc_layout_cuts": (vp, [vp]),
         "bbtc_layout_tile": (st, [vp, u32, u32, C.POINTER(bbtc_tile_info)]),
         "bbtc_layout_device_bytes": (u64, [vp]),
+        "bbtc_layout_save_bcsr": (st, [vp, C.c_char_p]),
+        "bbtc_layout_load_bcsr": (st, [C.c_char_p, pvp, pvp]),
         "bbtc_layout_free": (None, [vp]),
         "bbtc_task_count": (u64, [u32]),
         "bbtc_tasks_build": (st, [vp, i32, vp, u64, C.POINTER(u64)]),
@@ -299,6 +301,22 @@ class Layout:
         self._h = h
         self.p = len(c) - 1
         self.cuts = c.copy()
+
+    @classmethod
+    def load_bcsr(cls, path: str) -> "Layout":
+        """A stored block layout (.bcsr) and its graph, with the stored cuts."""
+        gh, lh = C.c_void_p(), C.c_void_p()
+        _check(lib().bbtc_layout_load_bcsr(path.encode(), C.byref(gh), C.byref(lh)))
+        L = cls.__new__(cls)
+        L.graph = Graph(gh)
+        L._h = lh
+        L.p = int(lib().bbtc_layout_ncut(lh))
+        ptr = C.cast(lib().bbtc_layout_cuts(lh), C.POINTER(C.c_uint32))
+        L.cuts = np.array([ptr[x] for x in range(L.p + 1)], dtype=np.uint32)
+        return L
+
+    def save_bcsr(self, path: str):
+        _check(lib().bbtc_layout_save_bcsr(self._h, path.encode()))
 
     def tile(self, i, k) -> TileInfo:
         t = bbtc_tile_info()

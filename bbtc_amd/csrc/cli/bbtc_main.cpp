@@ -14,7 +14,8 @@
 //   * "Cuda Initial Memory Allocations and Stream Creation" is reported as
 //     "HIP Initial Memory Allocations and Stream Creation";
 //   * extra flags: --kernel auto|merge|bitmap|dense|rdense|tuned, --threads n, --cuts-file path,
-//     --per-task-out path.
+//     --per-task-out path, --keep-duplicates, --save-bcsr path (a .bcsr
+//     --graph is read back with its own cuts).
 #include <omp.h>
 #include <strings.h>
 
@@ -47,6 +48,7 @@ struct Params {
   std::string cuts_file;
   std::string per_task_out;
   bool keep_duplicates = false;
+  std::string save_bcsr;
 };
 
 void usage() {
@@ -76,6 +78,7 @@ void usage() {
             << "  --cuts-file <path>    Read the p+1 cuts from a text file instead of partitioning\n"
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
             << "  --keep-duplicates     Keep repeated edges when converting text input\n"
+            << "  --save-bcsr <path>    Write the block layout (.bcsr) for later runs\n"
             << "\nExamples:\n"
             << "  bbtc --graph email-EuAll_adj.mtx --partitioner n1d --ncut 4\n"
             << "  bbtc --graph email-EuAll_adj.txt --ncut 4 --copy-first --border 4\n";
@@ -156,6 +159,7 @@ Params parse(int argc, char** argv) {
     } else if (!strcasecmp(a, "--cuts-file")) p.cuts_file = val("--cuts-file");
     else if (!strcasecmp(a, "--per-task-out")) p.per_task_out = val("--per-task-out");
     else if (!strcasecmp(a, "--keep-duplicates")) p.keep_duplicates = true;
+    else if (!strcasecmp(a, "--save-bcsr")) p.save_bcsr = val("--save-bcsr");
     else if (!strcasecmp(a, "--help")) {
       usage();
       std::exit(0);
@@ -194,9 +198,11 @@ bool ends_with(const std::string& s, const std::string& t) {
 int main(int argc, char** argv) {
   Params P = parse(argc, argv);
   std::string path = P.file;
-  // utils.hpp:168-170: a sibling "<path>.bbtc" is preferred when it exists
-  if (file_exists(path + ".bbtc")) path += ".bbtc";
-  else if (!ends_with(path, ".bbtc")) {
+  const bool bcsr = ends_with(path, ".bcsr");  // utils.hpp:300-301
+  if (bcsr) {
+  } else if (file_exists(path + ".bbtc")) {  // utils.hpp:168-170: a sibling "<path>.bbtc" is preferred
+    path += ".bbtc";
+  } else if (!ends_with(path, ".bbtc")) {
     // utils.hpp:171-268: convert the text graph and save it beside the input
     std::cerr << "# Saving binary CSR graph.." << std::endl;
     const bbtc_status cs = bbtc_convert_text(path.c_str(), (path + ".bbtc").c_str(), P.keep_duplicates ? 0 : 1);
@@ -208,36 +214,52 @@ int main(int argc, char** argv) {
     path += ".bbtc";
   }
   bbtc_hgraph* g = nullptr;
-  bbtc_status st = bbtc_hgraph_load(path.c_str(), &g);
-  if (st) die(st, "reading " + path);
+  bbtc_layout* L = nullptr;
+  bbtc_status st = BBTC_OK;
+  Fields F;
   uint32_t n = 0, ncols = 0;
   uint64_t m = 0;
-  bbtc_hgraph_info(g, &n, &ncols, &m);
-  std::cerr << "# Graph read.. N: " << n << ", NNZ: " << m << std::endl;
-
-  Fields F;
-  F.begin("Partitioning and Layout construction (s)");
-  const uint32_t p = uint32_t(P.ncut);
-  std::vector<uint32_t> cuts(p + 1);
-  if (!P.cuts_file.empty()) {
-    std::ifstream cf(P.cuts_file);
-    for (uint32_t b = 0; b <= p; ++b)
-      if (!(cf >> cuts[b])) {
-        std::cerr << "bbtc: --cuts-file needs " << p + 1 << " integers" << std::endl;
-        return 1;
-      }
+  uint32_t p = uint32_t(P.ncut);
+  if (bcsr) {
+    // a stored block layout: its own cuts, no partitioning (block_csr.cpp:215-255)
+    F.begin("Partitioning and Layout construction (s)");
+    st = bbtc_layout_load_bcsr(path.c_str(), &g, &L);
+    if (st) die(st, "reading " + path);
+    F.end();
+    bbtc_hgraph_info(g, &n, &ncols, &m);
+    p = bbtc_layout_ncut(L);
+    std::cerr << "# Graph read.. N: " << n << ", NNZ: " << m << std::endl;
   } else {
-    const bbtc_part kind = P.part == "uni"   ? BBTC_PART_UNI
-                           : P.part == "n1d" ? BBTC_PART_N1D
-                           : P.part == "rac" ? BBTC_PART_RAC
-                                             : BBTC_PART_PAL;
-    st = bbtc_partition(g, kind, p, cuts.data());
-    if (st) die(st, "partitioning");
+    st = bbtc_hgraph_load(path.c_str(), &g);
+    if (st) die(st, "reading " + path);
+    bbtc_hgraph_info(g, &n, &ncols, &m);
+    std::cerr << "# Graph read.. N: " << n << ", NNZ: " << m << std::endl;
+
+    F.begin("Partitioning and Layout construction (s)");
+    std::vector<uint32_t> cuts(p + 1);
+    if (!P.cuts_file.empty()) {
+      std::ifstream cf(P.cuts_file);
+      for (uint32_t b = 0; b <= p; ++b)
+        if (!(cf >> cuts[b])) {
+          std::cerr << "bbtc: --cuts-file needs " << p + 1 << " integers" << std::endl;
+          return 1;
+        }
+    } else {
+      const bbtc_part kind = P.part == "uni"   ? BBTC_PART_UNI
+                             : P.part == "n1d" ? BBTC_PART_N1D
+                             : P.part == "rac" ? BBTC_PART_RAC
+                                               : BBTC_PART_PAL;
+      st = bbtc_partition(g, kind, p, cuts.data());
+      if (st) die(st, "partitioning");
+    }
+    st = bbtc_layout_build(g, cuts.data(), p, &L);
+    if (st) die(st, "building the block layout");
+    F.end();
   }
-  bbtc_layout* L = nullptr;
-  st = bbtc_layout_build(g, cuts.data(), p, &L);
-  if (st) die(st, "building the block layout");
-  F.end();
+  if (!P.save_bcsr.empty()) {
+    st = bbtc_layout_save_bcsr(L, P.save_bcsr.c_str());
+    if (st) die(st, "writing " + P.save_bcsr);
+  }
 
   uint64_t ntasks = 0;
   std::vector<bbtc_task> tasks(bbtc_task_count(p));

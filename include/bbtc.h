@@ -135,6 +135,17 @@ const uint32_t* bbtc_layout_cuts(const bbtc_layout* L);
 /* Replaces the BlockCSR getters GetNNZ/GetHeight/GetWidth/GetTileDeg
  * (block_csr.cpp:87-186). Lower tiles (i > k) report zero entries. */
 bbtc_status bbtc_layout_tile(const bbtc_layout* L, uint32_t i, uint32_t k, bbtc_tile_info* out);
+/* Block CSR files (.bcsr): replaces BlockCSR::DeSerialize
+ * (block_csr.cpp:215-255) and supplies the writer the reference lacks. Same
+ * fields and widths (uint32 N, E, P, cuts[P+1], sparse_tiles[P*P+1],
+ * rows[N*P], cols[E], tiles row-major over the full grid), written with
+ * cuts[P] = N so the N*P row slots are exactly the tiles' rows; the end of
+ * the last row is E. Loading rebuilds the graph and the layout with the
+ * stored cuts (cuts[P] set to N+1 as the driver does), skipping
+ * partitioning and the CSR-to-tile pass. */
+bbtc_status bbtc_layout_save_bcsr(const bbtc_layout* L, const char* path);
+bbtc_status bbtc_layout_load_bcsr(const char* path, bbtc_hgraph** graph_out, bbtc_layout** layout_out);
+
 /* Bytes the layout occupies once uploaded. */
 uint64_t bbtc_layout_device_bytes(const bbtc_layout* L);
 void bbtc_layout_free(bbtc_layout* L);

@@ -1,0 +1,99 @@
+"""Block CSR files (.bcsr): the reference's serialised BlockCSR fields
+(block_csr.cpp:215-255) written with cuts[P] = N. A layout written and read
+back keeps its graph, its cuts and every per-task count (pinned by the golden
+vectors the reference's own code wrote); the file's arrays follow the
+reference's definitions, checked here against a direct construction."""
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import bbtc_amd as B
+from tests.conftest import graph_from_spec, load_golden
+
+GOLDEN = load_golden()
+RUNS = [(d, r) for d in GOLDEN for r in d["runs"] if "tasks" in r and d["fingerprint"]["m"] < 3_000_000]
+
+
+@pytest.mark.parametrize("d,r", RUNS, ids=[f"{d['name']}-{r['partitioner']}{r['ncut']}" for d, r in RUNS])
+def test_round_trip_counts(tmp_path, d, r):
+    g = graph_from_spec(d["graph"])
+    L = B.Layout(g, np.array(r["cuts"], dtype=np.uint32))
+    path = str(tmp_path / "x.bcsr")
+    L.save_bcsr(path)
+    L2 = B.Layout.load_bcsr(path)
+    assert np.array_equal(L2.cuts, np.array(r["cuts"], dtype=np.uint32))
+    assert np.array_equal(L2.graph.offsets(), g.offsets()) and np.array_equal(L2.graph.cols(), g.cols())
+    ref = np.array(r["tasks"], dtype=np.uint64)
+    per, tot = L2.count_cpu(ref[:, :3].astype(np.uint32))
+    assert tot == r["total"] and np.array_equal(per, ref[:, 3])
+
+
+def test_file_layout(tmp_path):
+    # the arrays as block_csr.cpp:286-376 defines them, with cuts[P] = N
+    g = B.Graph.rmat(9, 8, seed=4)
+    off, col = g.offsets().astype(np.int64), g.cols().astype(np.int64)
+    n, m = g.n, g.m
+    cuts = g.partition("uni", 3)
+    L = B.Layout(g, cuts)
+    path = tmp_path / "x.bcsr"
+    L.save_bcsr(str(path))
+    raw = np.frombuffer(path.read_bytes(), dtype=np.uint32)
+    P = 3
+    N_, E_, P_ = raw[:3]
+    assert (N_, E_, P_) == (n, m, P)
+    c = raw[3:3 + P + 1]
+    assert np.array_equal(c[:P], cuts[:P]) and c[P] == n
+    tiles = raw[3 + P + 1: 3 + P + 1 + P * P + 1]
+    rows = raw[3 + P + 1 + P * P + 1: 3 + P + 1 + P * P + 1 + n * P]
+    cols = raw[3 + P + 1 + P * P + 1 + n * P:]
+    assert len(cols) == m
+    h = np.diff(c.astype(np.int64))
+    assert np.array_equal(tiles, np.concatenate([[0], np.cumsum(np.repeat(h, P))]))
+    # entries of tile (y, x) in row-major tile order, each row's entries in column order
+    band = np.searchsorted(c[1:], np.arange(n), side="right")
+    want_rows, want_cols, at = [], [], 0
+    for y in range(P):
+        for x in range(P):
+            for u in range(c[y], c[y + 1]):
+                want_rows.append(at)
+                s = col[off[u]:off[u + 1]]
+                s = s[(s >= c[x]) & (s < c[x + 1])] - c[x]
+                want_cols.extend(s.tolist())
+                at += len(s)
+    assert np.array_equal(rows, np.array(want_rows)) and np.array_equal(cols, np.array(want_cols))
+    assert band.max() == P - 1
+
+
+def test_malformed_files(tmp_path):
+    g = B.Graph.rmat(8, 8, seed=2)
+    L = B.Layout(g, g.partition("pal", 2))
+    good = tmp_path / "g.bcsr"
+    L.save_bcsr(str(good))
+    data = good.read_bytes()
+    for name, blob in [("trunc", data[:-8]), ("badp", struct.pack("<III", 10, 0, 0)),
+                       ("cuts", data[:12] + struct.pack("<I", 5) + data[16:])]:
+        p = tmp_path / f"{name}.bcsr"
+        p.write_bytes(blob)
+        with pytest.raises(B.BbtcError):
+            B.Layout.load_bcsr(str(p))
+
+
+def test_cli_save_and_read(tmp_path):
+    d = load_golden("c1_rmat_s16_ef8")[0]
+    r = next(x for x in d["runs"] if x["partitioner"] == "pal")
+    path = str(tmp_path / "c1.bbtc")
+    graph_from_spec(d["graph"]).save(path)
+    cache = str(tmp_path / "c1.bcsr")
+    p1 = subprocess.run([B.CLI_PATH, "--graph", path, "--partitioner", "pal", "--ncut", str(r["ncut"]),
+                         "--ngpu", "0", "--repeat", "1", "--save-bcsr", cache], capture_output=True, text=True,
+                        timeout=120)
+    assert p1.returncode == 0, p1.stderr
+    out = tmp_path / "pt.txt"
+    p2 = subprocess.run([B.CLI_PATH, "--graph", cache, "--ngpu", "0", "--repeat", "1", "--per-task-out", str(out)],
+                        capture_output=True, text=True, timeout=120)
+    assert p2.returncode == 0, p2.stderr
+    assert f"Number of Cuts: {r['ncut']}" in p2.stdout and f"Number of Triangles: {r['total']}" in p2.stdout
+    got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
+    assert got == {tuple(x[:3]): x[3] for x in r["tasks"]}
