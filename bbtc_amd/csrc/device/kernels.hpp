@@ -66,7 +66,32 @@ __device__ __forceinline__ int owner_of(uint32_t incl, uint32_t x) {
   return q > kWave - 1 ? kWave - 1 : q;
 }
 
-__device__ __forceinline__ bool sorted_contains(const uint32_t* s, uint32_t n, uint32_t x) {
+// Pointers into global memory (address space 1). The graph arrays are
+// reached through descriptors loaded from memory, so without the cast the
+// compiler cannot tell them from LDS and emits flat loads, which wait on
+// both the vector-memory and the LDS counters.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p) {
+  return (gptr<T>)p;
+}
+// A descriptor struct (a multiple of 8 bytes, 8-byte aligned) read from
+// global memory word by word.
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+  static_assert(sizeof(T) % 8 == 0, "descriptor size");
+  T v;
+  const gptr<uint64_t> q = (gptr<uint64_t>)p;
+  uint64_t* d = reinterpret_cast<uint64_t*>(&v);
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 8); ++i) d[i] = q[i];
+  return v;
+}
+
+template <class Ptr>
+__device__ __forceinline__ bool sorted_contains(Ptr s, uint32_t n, uint32_t x) {
   uint32_t lo = 0, len = n;
   while (len) {
     const uint32_t half = len >> 1;
@@ -81,7 +106,7 @@ __device__ __forceinline__ bool sorted_contains(const uint32_t* s, uint32_t n, u
 }
 
 // Largest t with prefix[t] <= item (prefix[n] > item).
-__device__ __forceinline__ uint32_t find_task(const uint64_t* __restrict__ prefix, uint32_t n,
+__device__ __forceinline__ uint32_t find_task(gptr<uint64_t> prefix, uint32_t n,
                                               uint64_t item) {
   uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
@@ -147,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       if (acc) atomicAdd(S.per_task + t_cur, acc);
       if (acc_cyc) atomicAdd(S.cycles + t_cur, acc_cyc);
       if (a.left && run) {
-        const uint32_t gi = S.tasks[t_cur].graph;
+        const uint32_t gi = gload(S.tasks + t_cur).graph;
         if (atomicSub(a.left + gi, run) == run) a.done[gi] = wall_clock64();
       }
     }
@@ -194,12 +219,12 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       uint32_t lo = 0, hi = nrec;  // record holding chunk c: last base <= c
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (a.spill_base[mid] <= c) lo = mid;
+        if (as_global(a.spill_base)[mid] <= c) lo = mid;
         else hi = mid;
       }
-      const SpillRec R = a.spill[lo];
+      const SpillRec R = gload(a.spill + lo);
       it = R.item;
-      wlo = R.w0 + uint32_t(c - a.spill_base[lo]) * R.wc;
+      wlo = R.w0 + uint32_t(c - as_global(a.spill_base)[lo]) * R.wc;
       whi = R.w1 - wlo > R.wc ? wlo + R.wc : R.w1;
       may_defer = false;
     }
@@ -218,34 +243,35 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       s_cur = __builtin_amdgcn_readfirstlane(sg);
       const SegDesc& S = a.segs[s_cur];
       const uint64_t pi = S.item0 + (it - S.launch0);
-      t_cur = find_task(S.item_prefix, S.ntasks, pi);
-      t_first = S.item_prefix[t_cur];
-      const uint64_t e = S.item_prefix[t_cur + 1];
+      const gptr<uint64_t> ip = as_global(S.item_prefix);
+      t_cur = find_task(ip, S.ntasks, pi);
+      t_first = ip[t_cur];
+      const uint64_t e = ip[t_cur + 1];
       t_lo = S.launch0 + ((t_first > S.item0 ? t_first : S.item0) - S.item0);
       t_hi = S.launch0 + ((e < S.item1 ? e : S.item1) - S.item0);
     }
     const SegDesc& S = a.segs[s_cur];
     const unsigned long long t_start = S.cycles ? clock64() : 0;
-    const TaskDesc T = S.tasks[t_cur];
-    const TileDesc AB = S.tiles[T.ab];
-    const TileDesc BC = S.tiles[T.bc];
-    const TileDesc AC = S.tiles[T.ac];
-    const uint32_t* __restrict__ ab_rp = S.rowptr + AB.rp;
-    const uint32_t* __restrict__ ac_rp = S.rowptr + AC.rp;
-    const uint32_t* __restrict__ bc_rp = S.rowptr + BC.rp;
-    const uint32_t* __restrict__ ab_c = S.cols + AB.col;
-    const uint32_t* __restrict__ ac_c = S.cols + AC.col;
-    const uint32_t* __restrict__ bc_c = S.cols + BC.col;
+    const TaskDesc T = gload(S.tasks + t_cur);
+    const TileDesc AB = gload(S.tiles + T.ab);
+    const TileDesc BC = gload(S.tiles + T.bc);
+    const TileDesc AC = gload(S.tiles + T.ac);
+    const gptr<uint32_t> ab_rp = as_global(S.rowptr + AB.rp);
+    const gptr<uint32_t> ac_rp = as_global(S.rowptr + AC.rp);
+    const gptr<uint32_t> bc_rp = as_global(S.rowptr + BC.rp);
+    const gptr<uint32_t> ab_c = as_global(S.cols + AB.col);
+    const gptr<uint32_t> ac_c = as_global(S.cols + AC.col);
+    const gptr<uint32_t> bc_c = as_global(S.cols + BC.col);
     const bool dense = T.kind == kKindDense;
     const bool rdense = T.kind == kKindRdense;
     const bool bitmap = T.kind == kKindBitmap || dense;  // ac(u) staged as bitmaps
-    const uint32_t* __restrict__ bc_d = S.dense + BC.dense;  // bc bitmap rows (dense, rdense)
+    const gptr<uint32_t> bc_d = as_global(S.dense + BC.dense);  // bc bitmap rows (dense, rdense)
 
     // one row per lane
     const uint64_t z = (S.item0 + (it - S.launch0) - t_first) * T.rows + uint64_t(lane);
     uint32_t ab0 = 0, dab = 0, ac0 = 0, dac = 0;
     if (uint32_t(lane) < T.rows && z < AB.nzr) {
-      const uint32_t r = S.nzrows[AB.nz + z];
+      const uint32_t r = as_global(S.nzrows)[AB.nz + z];
       ab0 = ab_rp[r];
       dab = ab_rp[r + 1] - ab0;
       ac0 = ac_rp[r];
@@ -270,8 +296,10 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
     uint32_t steps = 0, iters = 0;
     bool stop = false;  // the rest of the unit was spilled
 
-    // Wedges [w_from, w_to) against ac(u) staged at seg (per row) in base.
-    auto wedges = [&](uint32_t w_from, uint32_t w_to, uint32_t seg, const uint32_t* base) {
+    // Wedges [w_from, w_to) against ac(u) staged at seg (per row) in base:
+    // the wave's LDS slice, or the ac tile's columns in HBM (one
+    // instantiation each, so every load has a known address space).
+    auto wedges = [&](uint32_t w_from, uint32_t w_to, uint32_t seg, auto base) {
       for (uint32_t wb = w_from; wb < w_to; wb += kWave) {
         const uint32_t w = wb + lane;
         const bool valid = w < w_to;
@@ -285,13 +313,13 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
           // lane per wedge, 16 bytes per step from LDS and from HBM.
           if (valid) {
             const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
-            const uint4* __restrict__ brow = reinterpret_cast<const uint4*>(bc_d + uint64_t(v) * T.wpr);
-            const uint4* arow = reinterpret_cast<const uint4*>(lds + q_seg);
+            const gptr<u32x4> brow = (gptr<u32x4>)(bc_d + uint64_t(v) * T.wpr);
+            const u32x4* arow = reinterpret_cast<const u32x4*>(lds + q_seg);
             uint32_t c = 0;
 #pragma unroll 2
             for (uint32_t j = 0; j < T.wpr / 4; ++j) {
-              const uint4 bv = brow[j];
-              const uint4 av = arow[j];
+              const u32x4 bv = brow[j];
+              const u32x4 av = arow[j];
               c += __popc(av.x & bv.x) + __popc(av.y & bv.y) + __popc(av.z & bv.z) + __popc(av.w & bv.w);
             }
             lcnt += c;
@@ -456,8 +484,9 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
         wave_lds_sync();
         const uint32_t f_ex = __shfl(exclW, first, kWave);
         const uint32_t l_in = __shfl(inclW, last, kWave);
-        wedges(f_ex > wlo ? f_ex : wlo, l_in < hi ? l_in : hi, in_hbm ? ac0 : exclA - a0,
-               in_hbm ? ac_c : lds);
+        const uint32_t w_from = f_ex > wlo ? f_ex : wlo, w_to = l_in < hi ? l_in : hi;
+        if (in_hbm) wedges(w_from, w_to, ac0, ac_c);
+        else wedges(w_from, w_to, exclA - a0, lds);
         wave_lds_sync();
         a0 = a1;
       }
