@@ -10,10 +10,11 @@
 //
 // The scheduler thread keeps up to `max_inflight` launches queued on the
 // compute stream. Each launch is one count_probe launch pair over a budget of
-// `quantum_items` work items, filled with slices of the resident requests in
-// order of remaining work (shortest first). A request that arrives while
-// others run joins the next launch and shares the machine with them; a large
-// request takes what the small ones leave and the whole budget when alone. A
+// `quantum_items` work items: the oldest resident request with work left
+// keeps half of it, and the rest is filled with slices of the resident
+// requests in order of remaining work (shortest first). A request that
+// arrives while others run joins the next launch and shares the machine with
+// them; a large request is slowed at most about twofold by later arrivals. A
 // request finishes with the launch that holds its last slice. Its counters
 // are read back and its heap block returns to the heap, which lets the next
 // arrival in.
@@ -381,25 +382,39 @@ void sched_loop(bbtc_server* S) {
         Launch l;
         l.ring = ring;
         const double t = S->now();
-        // shortest remaining work first, within the launch's item budget
-        std::vector<Request*> cand;
+        // The oldest request with work left keeps half of the launch's item
+        // budget (so a large request is slowed at most about twofold by the
+        // arrivals behind it); the rest goes shortest remaining work first.
+        std::vector<Request*> cand;  // admission order
         for (auto& r : act)
           if (r->next < r->nitems) cand.push_back(r.get());
-        std::stable_sort(cand.begin(), cand.end(), [](const Request* x, const Request* y) {
-          return x->nitems - x->next < y->nitems - y->next;
-        });
+        std::vector<uint64_t> take(cand.size(), 0);
         uint64_t budget = S->sequential ? ~0ull : S->quantum;
-        for (Request* r : cand) {
-          if (nseg == S->max_segments || budget == 0) break;
-          const uint64_t take = std::min(r->nitems - r->next, budget);
-          budget -= take;
+        if (!S->sequential && !cand.empty()) {
+          take[0] = std::min(cand[0]->nitems - cand[0]->next, budget / 2);
+          budget -= take[0];
+        }
+        std::vector<size_t> by_rem(cand.size());
+        for (size_t x = 0; x < cand.size(); ++x) by_rem[x] = x;
+        std::stable_sort(by_rem.begin(), by_rem.end(), [&](size_t x, size_t y) {
+          return cand[x]->nitems - cand[x]->next < cand[y]->nitems - cand[y]->next;
+        });
+        for (size_t x : by_rem) {
+          const uint64_t more = std::min(cand[x]->nitems - cand[x]->next - take[x], budget);
+          take[x] += more;
+          budget -= more;
+        }
+        for (size_t x : by_rem) {
+          Request* r = cand[x];
+          if (nseg == S->max_segments) break;
+          if (take[x] == 0) continue;
           SegDesc d = r->seg;
           d.item0 = r->next;
-          d.item1 = r->next + take;
+          d.item1 = r->next + take[x];
           d.launch0 = total;
           sh[nseg++] = d;
-          total += take;
-          r->next += take;
+          total += take[x];
+          r->next += take[x];
           r->launches++;
           if (r->start_s < 0) r->start_s = t;
           if (r->next == r->nitems)

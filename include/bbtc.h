@@ -361,9 +361,9 @@ bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* 
 /* graph: requests are admitted in arrival order while their device footprint */
 /* (block CSR, bitmap rows of dense tiles, plan tables) fits the server heap; */
 /* an upload thread copies them in while earlier requests count; each launch  */
-/* fills a budget of work items with slices of the resident requests, least   */
-/* remaining work first, so arrivals join the next launch and share the      */
-/* machine with requests already running.                                     */
+/* fills a budget of work items: half kept by the oldest resident request,    */
+/* the rest least remaining work first, so arrivals join the next launch and  */
+/* share the machine with requests already running.                           */
 /* ------------------------------------------------------------------------- */
 typedef struct bbtc_server bbtc_server;
 
