@@ -15,6 +15,8 @@ Prints one JSON line:
   roofline      algorithmic bytes per step / mean event time vs 8 TB/s
   cpu_baseline  the CPU oracle (restated reference CPU path) on a sample of
                 the same tasks, scaled by algorithmic bytes
+  c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), tuned forms:
+                edges/s and roofline fraction of the same count
   batched       4,096 uneven graphs (70% RMAT s10-18, 30% G(n,m)) resident,
                 one captured persistent launch vs the plain per-graph launch:
                 graphs/s, per-graph p50/p99, bit-exact per-graph counts
@@ -96,6 +98,33 @@ def cpu_baseline(g, cuts, tasks, per_task_gpu, bytes_per_task, total_bytes, m, b
     }
 
 
+def count_c2(B, dev, steps):
+    """G(n=2^22, m=2^26) seed 2, pal 16, tuned forms: one resident count."""
+    t0 = time.time()
+    g = B.Graph.gnm_device(dev, 1 << 22, 1 << 26, 2)
+    cuts = g.partition("pal", 16)
+    L = B.Layout(g, cuts)
+    tasks = L.tasks("desc")
+    _, total_bytes, _ = L.task_bytes(tasks, 4)
+    dg = dev.upload(L)
+    choice, _ = dg.autotune(tasks)
+    plan = dg.plan(tasks, "auto", choice)
+    prep = time.time() - t0
+    plan.time(1)
+    ms = plan.time(steps)
+    per, total = plan.results()
+    plan.destroy()
+    dg.release()
+    mean_s = float(np.mean(ms)) * 1e-3
+    log(f"c2: G(n=2^22, m=2^26) pal 16: {total} triangles, events ms {np.round(ms, 3).tolist()} "
+        f"(prepared in {prep:.1f}s)")
+    return {"workload": "gnm-n22-m26 pal ncut16", "metric": "edges/s", "value": g.m / mean_s,
+            "ms_per_count": mean_s * 1e3, "triangles": int(total), "tasks": int(len(tasks)),
+            "algorithmic_bytes": int(total_bytes), "offset_bytes": 4,
+            "roofline_frac": total_bytes / mean_s / 1e9 / HBM_PEAK_GBS,
+            "forms_merge_bitmap_dense_rdense": np.bincount(choice, minlength=5)[1:].tolist()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +138,7 @@ def main():
     ap.add_argument("--kernel", default="tuned", help="tuned | auto | merge | bitmap | dense | rdense")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the G(n=2^22, m=2^26) count")
     ap.add_argument("--batched-graphs", type=int, default=4096, help="0: skip the batched workload")
     ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
     args = ap.parse_args()
@@ -207,6 +237,9 @@ def main():
 
     def rounded(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+    if not args.no_c2:
+        out["c2"] = rounded(count_c2(B, dev, steps=3))
 
     if args.batched_graphs:
         res = W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log)

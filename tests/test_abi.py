@@ -1,0 +1,62 @@
+"""The C-ABI boundary: libbbtc.so loads without a GPU and exports every
+function include/bbtc.h declares; the declarations stay plain C."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import bbtc_amd as B
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+HEADER = os.path.join(ROOT, "include", "bbtc.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    names = re.findall(r"\b(bbtc_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert len(names) > 60
+    for must in ["bbtc_hgraph_load", "bbtc_partition", "bbtc_layout_build", "bbtc_count", "bbtc_count_hybrid",
+                 "bbtc_batch_create", "bbtc_server_create", "bbtc_convert_text", "bbtc_layout_save_bcsr"]:
+        assert must in names
+
+
+def test_every_declared_function_is_exported():
+    lib = C.CDLL(B.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_exported_symbols_are_c_names():
+    out = subprocess.run(["nm", "-D", "--defined-only", B.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = {ln.split()[-1] for ln in out.stdout.splitlines() if " T " in ln}
+    for n in declared_functions():
+        assert n in exported, n  # unmangled: extern "C"
+
+
+def test_header_is_plain_c(tmp_path):
+    src = tmp_path / "use.c"
+    src.write_text('#include "bbtc.h"\nint main(void) { bbtc_status s = BBTC_OK; (void)s; return 0; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
+                        "-o", str(tmp_path / "use.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_status_strings_and_errors_without_gpu():
+    lib = B.lib()
+    for s in range(9):
+        assert lib.bbtc_strerror(s)
+    with pytest.raises(B.BbtcError) as e:
+        B.Graph.load("/nonexistent/graph.bbtc")
+    assert e.value.status == 2  # BBTC_ERR_IO
+    g = B.Graph.rmat(8, 4, seed=1)
+    with pytest.raises(B.BbtcError) as e:
+        g.partition("pal", 0)
+    assert e.value.status == 1  # BBTC_ERR_INVALID
