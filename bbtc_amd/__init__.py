@@ -27,7 +27,7 @@ CLI_PATH = os.path.join(_HERE, "bbtc")
 
 PART = {"uni": 0, "n1d": 1, "pal": 2, "rac": 3}
 ORDER = {"lex": 0, "desc": 1, "asc": 2}
-KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4}
+KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4, "lane": 5}
 
 
 class BbtcError(RuntimeError):
@@ -51,7 +51,7 @@ class bbtc_opts(C.Structure):
 
 class bbtc_server_opts(C.Structure):
     _fields_ = [("heap_bytes", C.c_uint64), ("quantum_items", C.c_uint64), ("max_inflight", C.c_uint32),
-                ("max_segments", C.c_uint32), ("sequential", C.c_uint32), ("reserved", C.c_uint32)]
+                ("max_segments", C.c_uint32), ("sequential", C.c_uint32), ("no_lane_form", C.c_uint32)]
 
 
 class bbtc_server_result(C.Structure):
@@ -110,6 +110,7 @@ def _declare(lib):
         "bbtc_ctx_destroy": (None, [vp]),
         "bbtc_ctx_bytes_in_use": (u64, [vp]),
         "bbtc_ctx_synchronize": (st, [vp]),
+        "bbtc_ctx_read_bandwidth": (st, [vp, u64, i32, vp]),
         "bbtc_graph_upload": (st, [vp, vp, pvp]),
         "bbtc_graph_bytes": (u64, [vp]),
         "bbtc_graph_release": (None, [vp]),
@@ -383,6 +384,12 @@ class Device:
     def synchronize(self):
         _check(lib().bbtc_ctx_synchronize(self._h))
 
+    def read_bandwidth(self, nbytes: int = 8 << 30, iters: int = 5) -> np.ndarray:
+        """GB/s of a streaming read over nbytes of device memory, per pass."""
+        out = np.zeros(iters, dtype=np.float64)
+        _check(lib().bbtc_ctx_read_bandwidth(self._h, nbytes, iters, _ptr(out)))
+        return out
+
     def upload(self, layout: Layout) -> "DeviceGraph":
         return DeviceGraph(self, layout)
 
@@ -408,10 +415,11 @@ class DeviceGraph:
         return lib().bbtc_graph_bytes(self._h)
 
     def autotune(self, tasks):
-        """Per-task form choice from measured clocks: (choice[n], cycles[n,4])."""
+        """Per-task form choice from measured clocks: (choice[n], cycles[n,5]),
+        forms in the order merge, bitmap, dense, rdense, lane."""
         t = tasks_array(tasks)
         choice = np.zeros(len(t), dtype=np.uint8)
-        cyc = np.zeros((len(t), 4), dtype=np.uint64)
+        cyc = np.zeros((len(t), 5), dtype=np.uint64)
         _check(lib().bbtc_autotune(self.dev._h, self._h, _ptr(t), len(t), _ptr(choice), _ptr(cyc)))
         return choice, cyc
 
@@ -626,8 +634,9 @@ class Server:
     """Continuous-batching request loop on one device (bbtc_server_*)."""
 
     def __init__(self, dev: "Device", heap_bytes: int = 0, quantum_items: int = 0, max_inflight: int = 0,
-                 max_segments: int = 0, sequential: bool = False):
-        o = bbtc_server_opts(heap_bytes, quantum_items, max_inflight, max_segments, int(sequential), 0)
+                 max_segments: int = 0, sequential: bool = False, no_lane_form: bool = False):
+        o = bbtc_server_opts(heap_bytes, quantum_items, max_inflight, max_segments, int(sequential),
+                             int(no_lane_form))
         h = C.c_void_p()
         _check(lib().bbtc_server_create(dev._h, C.byref(o), C.byref(h)))
         self._h = h

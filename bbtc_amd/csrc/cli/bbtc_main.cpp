@@ -13,7 +13,7 @@
 //   * argument errors exit with status 2 (the reference exits 0);
 //   * "Cuda Initial Memory Allocations and Stream Creation" is reported as
 //     "HIP Initial Memory Allocations and Stream Creation";
-//   * extra flags: --kernel auto|merge|bitmap|dense|rdense|tuned, --threads n, --cuts-file path,
+//   * extra flags: --kernel auto|merge|bitmap|dense|rdense|lane|tuned, --threads n, --cuts-file path,
 //     --per-task-out path, --keep-duplicates, --save-bcsr path (a .bcsr
 //     --graph is read back with its own cuts).
 #include <omp.h>
@@ -73,7 +73,7 @@ void usage() {
             << "                          Note: if set to -1, GPUs process all.\n"
             << "  --copy-first          If set then copy graph to GPUs first       (false)\n"
             << "  --repeat <int>        Run bbTC algorithm '--repeat' times        (5)\n"
-            << "  --kernel <str>        auto | merge | bitmap | dense | rdense | tuned (auto)\n"
+            << "  --kernel <str>        auto | merge | bitmap | dense | rdense | lane | tuned (auto)\n"
             << "  --threads <int>       Host worker threads                        (all)\n"
             << "  --cuts-file <path>    Read the p+1 cuts from a text file instead of partitioning\n"
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
@@ -151,7 +151,7 @@ Params parse(int argc, char** argv) {
     } else if (!strcasecmp(a, "--kernel")) {
       p.kernel = val("--kernel");
       if (p.kernel != "auto" && p.kernel != "merge" && p.kernel != "bitmap" && p.kernel != "dense" &&
-          p.kernel != "rdense" && p.kernel != "tuned")
+          p.kernel != "rdense" && p.kernel != "lane" && p.kernel != "tuned")
         bad("Wrong kernel.");
     } else if (!strcasecmp(a, "--threads")) {
       p.threads = to_int(val("--threads"), "--threads");
@@ -285,6 +285,7 @@ int main(int argc, char** argv) {
                            : P.kernel == "bitmap" ? BBTC_KERNEL_BITMAP
                            : P.kernel == "dense"  ? BBTC_KERNEL_DENSE
                            : P.kernel == "rdense" ? BBTC_KERNEL_RDENSE
+                           : P.kernel == "lane"   ? BBTC_KERNEL_LANE
                                                   : BBTC_KERNEL_AUTO;
   // --kernel tuned: per-task forms measured once on the first upload
   std::vector<uint8_t> choice;

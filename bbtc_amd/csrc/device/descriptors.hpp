@@ -13,7 +13,21 @@ constexpr int kLdsWords = 1792;  // bitmap / sorted-list slice of a wave
 constexpr int kMarkWords = 4 * 64;  // per-step wedge marks of a wave
 constexpr int kWaveLds = kLdsWords + kMarkWords;  // 8 KiB of LDS per wave
 constexpr int kMaxSlots = 64;    // concurrent range launches of one plan
-constexpr int kItemBatch = 16;   // work items a wave claims per atomic
+constexpr int kItemBatch = 8;    // work items a wave claims per atomic
+// First-phase items are dealt to kQueues parts in chunks of consecutive
+// items (chunk c to part c % kQueues); each part has its own head counter in
+// its own 128-byte line and is claimed in its own order. A block starts on
+// part blockIdx % kQueues (its XCD when blocks are dealt to the 8 XCDs in
+// turn) and moves to the next part when its own is drained. Eight counters
+// take the claims of 256 CUs instead of one, and each XCD works through
+// chunks of its own (the tiles its L2 holds). One graph's launch uses one
+// chunk per part (each XCD on its own eighth of the task list: the least
+// sharing of tiles between L2s); batches and the server use chunks of
+// kListChunkItems, so the parts advance through the item list together and
+// items early in the list (small graphs, old requests) finish early.
+constexpr int kQueues = 8;
+constexpr uint64_t kListChunkItems = 4096;
+constexpr int kCtlStride = 16;   // 64-bit words per counter line
 // Splitting of heavy items. A first-phase item that has spent more than
 // kDeferSteps wave steps (one step: 64 probes, one 64-wedge batch, or one
 // 16-byte bitmap word per lane) hands its remaining wedges to the second phase
@@ -30,6 +44,11 @@ constexpr uint32_t kKindSearch = 0;
 constexpr uint32_t kKindBitmap = 1;
 constexpr uint32_t kKindDense = 2;
 constexpr uint32_t kKindRdense = 3;
+constexpr uint32_t kKindLane = 4;
+// The lane form (one lane per row u, lists merged in HBM) applies to tasks
+// whose three tiles hold at most this many entries in any row, which bounds
+// a lane's work to kLaneMaxRow * 2 * kLaneMaxRow merge steps.
+constexpr uint32_t kLaneMaxRow = 32;
 
 struct TileDesc {
   uint64_t rp;     // offset of the tile's row offsets in rowptr
@@ -39,13 +58,13 @@ struct TileDesc {
   uint32_t nzr;    // number of non-empty rows
   uint32_t h;      // band height
   uint32_t dwpr;   // bitmap words per row (multiple of 4)
-  uint32_t pad;
+  uint32_t maxd;   // entries in the tile's longest row
 };
 static_assert(sizeof(TileDesc) == 48, "TileDesc layout");
 
 struct TaskDesc {
   uint32_t ab, bc, ac;  // tile ids i*P+j, j*P+k, i*P+k
-  uint32_t kind;        // kKindSearch, kKindBitmap, kKindDense or kKindRdense
+  uint32_t kind;        // kKindSearch, kKindBitmap, kKindDense, kKindRdense or kKindLane
   uint32_t rows;        // rows of the ab non-empty list per item (1..64)
   uint32_t wpr;         // bitmap words per row (bitmap and dense kinds, multiple of 4)
   uint32_t graph;        // index of the task's graph in a batch (0 otherwise)
@@ -55,11 +74,55 @@ static_assert(sizeof(TaskDesc) == 32, "TaskDesc layout");
 
 // Control words of one launch slot, reset before each launch.
 struct SlotCtl {
-  unsigned long long head;   // next first-phase item
+  // part q: next position at head[q * kCtlStride], positions [0, end[q]);
+  // position x of part q is item base + ((x / C) * kQueues + q) * C + x % C
+  // with C = kChunkItems
+  unsigned long long head[kQueues * kCtlStride];
+  unsigned long long end[(kQueues + kCtlStride - 1) / kCtlStride * kCtlStride];
+  unsigned long long base;   // first item of the launch
+  unsigned long long chunk;  // items per chunk (a multiple of kItemBatch)
+  unsigned long long shift;  // log2(chunk), or 63 with one chunk per part
   unsigned long long spill;  // (records << kSpillShift) | chunks reserved
   unsigned long long head2;  // next second-phase chunk
-  unsigned long long pad;
+  unsigned long long pad[kCtlStride - 5];
 };
+static_assert(sizeof(SlotCtl) % 128 == 0, "SlotCtl layout");
+static_assert(kListChunkItems % kItemBatch == 0, "claims must not straddle chunks");
+static_assert((kListChunkItems & (kListChunkItems - 1)) == 0, "list chunks are a power of two");
+
+// Chunk size of a launch of len items: `chunk` (a power of two and a
+// multiple of kItemBatch), or for chunk == 0 one chunk per part, rounded up
+// to whole claims; the shift that splits a part position into chunk and
+// offset (63 with one chunk per part, where every position is below the
+// chunk). Then the positions of part q, and the item at position x of part q
+// (constexpr: used by the device and checked on the host).
+constexpr unsigned long long part_chunk(unsigned long long len, unsigned long long chunk) {
+  if (chunk) return chunk;
+  const unsigned long long per = (len + kQueues - 1) / kQueues;
+  const unsigned long long c = (per + kItemBatch - 1) / kItemBatch * kItemBatch;
+  return c ? c : kItemBatch;
+}
+constexpr unsigned part_shift(unsigned long long chunk) {
+  unsigned s = 0;
+  while (chunk > 1) {
+    chunk >>= 1;
+    ++s;
+  }
+  return s;
+}
+constexpr unsigned part_shift(unsigned long long len, unsigned long long chunk) {
+  return chunk ? part_shift(chunk) : 63u;
+}
+constexpr unsigned long long part_positions(unsigned long long len, unsigned q, unsigned long long C) {
+  const unsigned long long nch = (len + C - 1) / C;
+  const unsigned long long mine = nch > q ? (nch - 1 - q) / kQueues + 1 : 0;
+  const unsigned long long last = len - (nch ? nch - 1 : 0) * C;
+  return mine == 0 ? 0 : (nch - 1) % kQueues == q ? (mine - 1) * C + last : mine * C;
+}
+constexpr unsigned long long part_item(unsigned long long base, unsigned q, unsigned long long x,
+                                       unsigned long long C, unsigned shift) {
+  return base + ((x >> shift) * kQueues + q) * C + (x & ((1ull << shift) - 1));
+}
 
 // Wedges [w0, w1) of launch-local item `item` (in its flattened wedge order),
 // counted by the second phase in chunks of wc wedges.
@@ -93,8 +156,8 @@ struct CountArgs {
   const SegDesc* __restrict__ segs;  // launch0 increasing
   uint32_t nseg;
   uint32_t phase;             // 0: items from ctl->head; 1: chunks from ctl->head2
-  uint64_t item_end;          // launch-local items [ctl->head at launch, item_end)
-  SlotCtl* ctl;               // head set to the first item of the launch beforehand
+  uint64_t item_end;          // launch-local items [first item, item_end)
+  SlotCtl* ctl;               // heads and part ends set by reset_slot beforehand
   SpillRec* spill;            // spill_cap records (first phase writes, second reads)
   uint64_t* spill_base;       // first chunk of each record (increasing)
   uint32_t spill_cap;         // 0: no splitting

@@ -23,7 +23,7 @@ void set_last_error(const std::string& s);
 
 bool valid_kernel(bbtc_kernel k) {
   return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
-         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE;
+         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE || k == BBTC_KERNEL_LANE;
 }
 
 std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n) {
@@ -50,6 +50,7 @@ std::vector<TileDesc> tile_table(const Layout& L) {
       d.nz = L.nz_base[t];
       d.nzr = L.nz_rows[t];
       d.dense = L.dense_base[t];
+      d.maxd = L.max_row[t];
     }
     out[t] = d;
   }
@@ -62,10 +63,18 @@ void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s) {
   HIP_OK(hipGetLastError());
 }
 
-void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, unsigned blocks, hipStream_t s) {
-  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, i0);
+void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t items, bool lane_only,
+                   uint64_t chunk, hipStream_t s) {
+  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, i0, i0 + items, (unsigned long long)chunk);
   HIP_OK(hipGetLastError());
   a.phase = 0;
+  const uint64_t want = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+  const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(ctx, lane_only))));
+  if (lane_only) {
+    hipLaunchKernelGGL(count_probe_lanes, dim3(blocks), dim3(kBlock), 0, s, a);
+    HIP_OK(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(count_probe, dim3(blocks), dim3(kBlock), 0, s, a);
   HIP_OK(hipGetLastError());
   // spilled chunks of heavy items, over the whole machine
@@ -75,12 +84,24 @@ void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, unsigned b
 }
 
 PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<float>& deg,
-                       const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice) {
+                       const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice,
+                       bool allow_auto_lanes) {
   const uint64_t n = refs.size();
   PlanDesc P;
   P.td.assign(std::max<uint64_t>(n, 1), TaskDesc{});
   P.pref.assign(n + 1, 0);
   P.hkind.reserve(n);
+  auto short_rows = [&](const TaskRef& x) {
+    return std::max(tiles[x.ab].maxd, std::max(tiles[x.bc].maxd, tiles[x.ac].maxd)) <= kLaneMaxRow;
+  };
+  auto has_work = [&](const TaskRef& x) {
+    return !(deg[x.bc] == 0 || deg[x.ac] == 0 || deg[x.ab] == 0) && tiles[x.ab].nzr > 0;
+  };
+  // BBTC_KERNEL_AUTO takes the lane form for every task when every task with
+  // work admits it (short rows throughout: the lane kernel then runs alone)
+  bool auto_lanes = allow_auto_lanes && kernel == BBTC_KERNEL_AUTO && n > 0;
+  for (uint64_t q = 0; q < n && auto_lanes; ++q)
+    if (has_work(refs[q]) && !short_rows(refs[q])) auto_lanes = false;
   for (uint64_t q = 0; q < n; ++q) {
     const TaskRef& x = refs[q];
     TaskDesc d{};
@@ -98,6 +119,8 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
     else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
     else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
     else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
+    else if (choice && choice[q] == 5) k = BBTC_KERNEL_LANE;
+    else if (auto_lanes && !(choice && choice[q])) k = BBTC_KERNEL_LANE;
     uint32_t kind = kKindSearch;
     if (k == BBTC_KERNEL_BITMAP) {
       if (fit >= 1) kind = kKindBitmap;
@@ -106,6 +129,8 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
       else if (fit >= 1) kind = kKindBitmap;
     } else if (k == BBTC_KERNEL_RDENSE) {
       if (has_dense) kind = kKindRdense;
+    } else if (k == BBTC_KERNEL_LANE) {
+      if (short_rows(x)) kind = kKindLane;
     } else if (k == BBTC_KERNEL_AUTO) {
       // without measurements: the ac(u) list against the bc(v) bitmap row
       // where the bc tile has one, else the ac(u) bitmap where it fits
@@ -116,12 +141,13 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
     d.kind = kind;
     P.hkind.push_back(kind + 1);
     d.rows = use_bitmap ? std::min<uint32_t>(fit, kWave) : uint32_t(kWave);
-    d.wpr = kind == kKindSearch ? 0 : wpr;
+    d.wpr = kind == kKindSearch || kind == kKindLane ? 0 : wpr;
     P.td[q] = d;
     uint64_t items = 0;
     if (!(deg[d.bc] == 0 || deg[d.ac] == 0 || deg[d.ab] == 0)) {
       items = (uint64_t(tiles[d.ab].nzr) + d.rows - 1) / d.rows;
-      if (kind != kKindSearch) P.bitmap_tasks++;
+      if (kind != kKindSearch && kind != kKindLane) P.bitmap_tasks++;
+      if (items && kind != kKindLane) P.lane_only = false;
     }
     P.pref[q + 1] = P.pref[q] + items;
   }
@@ -182,9 +208,7 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   a.spill_cap = p->spill_cap;
   a.left = p->track_left;
   a.done = p->track_done;
-  const uint64_t want_blocks = (i1 - i0 + kWavesPerBlock - 1) / kWavesPerBlock;
-  const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want_blocks, resident_blocks(p->ctx))));
-  enqueue_count(p->ctx, a, i0, blocks, s);
+  enqueue_count(p->ctx, a, i0, i1 - i0, p->lane_only, p->chunk, s);
 }
 
 void launch(bbtc_plan* p, hipStream_t s) {
@@ -212,6 +236,7 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
     std::vector<uint64_t>& pref = D.pref;
     p->hkind = std::move(D.hkind);
     p->bitmap_tasks = D.bitmap_tasks;
+    p->lane_only = D.lane_only;
     p->nitems = pref[n];
     p->hprefix = pref;
     HIP_OK(hipSetDevice(ctx->device));
@@ -258,20 +283,22 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
 // replaces the fixed kernel threshold of bb_tc.cpp:98.
 void autotune_refs(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& refs, uint8_t* choice_out,
                    uint64_t* cycles_out) {
-  constexpr int kForms = 4;
+  constexpr int kForms = 5;
   const bbtc_kernel forced[kForms] = {BBTC_KERNEL_MERGE, BBTC_KERNEL_BITMAP, BBTC_KERNEL_DENSE,
-                                      BBTC_KERNEL_RDENSE};
+                                      BBTC_KERNEL_RDENSE, BBTC_KERNEL_LANE};
   const uint64_t n = refs.size();
   const uint64_t none = ~0ull;
   std::vector<uint64_t> best(n, none);
   std::vector<uint8_t> choice(n, 1);
   if (cycles_out)
     for (uint64_t x = 0; x < kForms * n; ++x) cycles_out[x] = none;
+  bool all_lane = false;  // the lane form applies to every task with work
   for (int f = 0; f < kForms; ++f) {
     std::vector<uint32_t> kinds;
     {
       bbtc_plan* p = make_plan(ctx, g, refs, forced[f], nullptr);
       kinds = p->hkind;
+      if (forced[f] == BBTC_KERNEL_LANE) all_lane = p->lane_only && n > 0;
       if (f == 0) {  // warm caches and code once
         try {
           launch(p, ctx->stream);
@@ -299,11 +326,28 @@ void autotune_refs(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& re
     for (size_t x = 0; x < idx.size(); ++x) {
       const uint64_t q = idx[x];
       if (cycles_out) cycles_out[kForms * q + f] = cyc[x];
-      if (cyc[x] < best[q]) {
+      // The lane form runs in its own kernel only when every task of a
+      // launch uses it, so it is weighed as a whole plan below rather than
+      // per task.
+      if (forced[f] != BBTC_KERNEL_LANE && cyc[x] < best[q]) {
         best[q] = cyc[x];
         choice[q] = uint8_t(f + 1);
       }
     }
+  }
+  if (all_lane) {
+    // the per-task choice against every task in the lane form, by launch time
+    auto fastest_ms = [&](bbtc_kernel k, const uint8_t* ch) {
+      bbtc_plan* p = make_plan(ctx, g, refs, k, ch);
+      float ms[3] = {0.f, 0.f, 0.f};
+      const bbtc_status st = bbtc_plan_time(p, 3, ms);
+      release_plan(p);
+      if (st != BBTC_OK) fail(st, bbtc_last_error());
+      return std::min(ms[1], ms[2]);  // the first launch warms caches
+    };
+    const float per_task = fastest_ms(BBTC_KERNEL_AUTO, choice.data());
+    const float lanes = fastest_ms(BBTC_KERNEL_LANE, nullptr);
+    if (lanes < per_task) std::fill(choice.begin(), choice.end(), uint8_t(5));
   }
   std::copy(choice.begin(), choice.end(), choice_out);
 }
@@ -413,6 +457,9 @@ bbtc_status bbtc_ctx_create(int device, const bbtc_opts* opts, bbtc_ctx** out) {
     HIP_OK(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->blocks_per_cu, bbtc::count_probe, bbtc::kBlock, 0));
     if (c->blocks_per_cu < 1) c->blocks_per_cu = 1;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->blocks_per_cu_lane, bbtc::count_probe_lanes,
+                                                        bbtc::kBlock, 0));
+    if (c->blocks_per_cu_lane < 1) c->blocks_per_cu_lane = 1;
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   } catch (...) {
     delete c;
@@ -444,6 +491,51 @@ bbtc_status bbtc_ctx_synchronize(bbtc_ctx* ctx) {
   DEV_TRY
   HIP_OK(hipSetDevice(ctx->device));
   HIP_OK(hipDeviceSynchronize());
+  DEV_CATCH
+}
+
+bbtc_status bbtc_ctx_read_bandwidth(bbtc_ctx* ctx, uint64_t bytes, int iters, double* gbps_out) {
+  DEV_REQUIRE(ctx && iters > 0 && gbps_out, "bad argument");
+  DEV_REQUIRE(bytes >= (1ull << 20), "read at least 1 MiB");
+  DEV_TRY
+  HIP_OK(hipSetDevice(ctx->device));
+  const uint64_t words = bytes / 16;
+  const unsigned blocks = unsigned(std::max(1, ctx->num_cu) * 64);
+  uint64_t acct = 0;
+  auto* buf = bbtc::dev_alloc<bbtc::u32x4>(ctx, words, &acct);
+  unsigned* sink = nullptr;
+  std::vector<hipEvent_t> ev(2 * size_t(iters) + 2, nullptr);
+  auto cleanup = [&] {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (sink) (void)hipFree(sink);
+    bbtc::dev_free(ctx, buf);
+    ctx->in_use -= std::min(ctx->in_use, acct);
+  };
+  try {
+    HIP_OK(hipMalloc(&sink, size_t(blocks) * sizeof(unsigned)));
+    hipStream_t s = ctx->stream;
+    HIP_OK(hipMemsetAsync(buf, 0, words * 16, s));
+    for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+    // one untimed pass, then the timed ones
+    hipLaunchKernelGGL(bbtc::stream_read, dim3(blocks), dim3(bbtc::kBlock), 0, s, buf, words, sink);
+    HIP_OK(hipGetLastError());
+    for (int i = 0; i < iters; ++i) {
+      HIP_OK(hipEventRecord(ev[2 * i], s));
+      hipLaunchKernelGGL(bbtc::stream_read, dim3(blocks), dim3(bbtc::kBlock), 0, s, buf, words, sink);
+      HIP_OK(hipEventRecord(ev[2 * i + 1], s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    for (int i = 0; i < iters; ++i) {
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      gbps_out[i] = double(words) * 16.0 / (double(ms) * 1e-3) / 1e9;
+    }
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
   DEV_CATCH
 }
 
@@ -695,6 +787,7 @@ void batch_replan(bbtc_batch* b, bbtc_kernel kernel, const uint8_t* choice) {
     b->plan = nullptr;
   }
   b->plan = make_plan(b->ctx, b->arena, b->refs, kernel, choice);
+  b->plan->chunk = kListChunkItems;  // graphs early in the list finish early
   b->hleft0.assign(std::max<uint64_t>(b->ngraphs, 1), 0);
   for (uint64_t q = 0; q < b->refs.size(); ++q)
     b->hleft0[b->refs[q].graph] += b->plan->hprefix[q + 1] - b->plan->hprefix[q];

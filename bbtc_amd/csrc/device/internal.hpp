@@ -45,24 +45,35 @@ std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n);
 std::vector<TileDesc> tile_table(const Layout& L);
 
 // Host side of a plan: the form and rows per item of each task (kernel rule,
-// or choice[q] = 1 merge, 2 bitmap, 3 dense, 4 rdense) and the item prefix.
+// or choice[q] = 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane) and the item
+// prefix.
 struct PlanDesc {
   std::vector<TaskDesc> td;       // max(n, 1) entries
   std::vector<uint64_t> pref;     // n + 1 entries
   std::vector<uint32_t> hkind;    // form per task, 1-based as in choice
   uint64_t bitmap_tasks = 0;
+  bool lane_only = true;          // every task with work items uses the lane form
 };
+// auto_lanes: BBTC_KERNEL_AUTO may take the lane form for every task.
 PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<float>& deg,
-                       const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice);
+                       const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice,
+                       bool auto_lanes = true);
 
-// Resident blocks of count_probe on the context's device.
-inline uint64_t resident_blocks(const bbtc_ctx* ctx) {
-  return std::max<uint64_t>(1, uint64_t(ctx->num_cu) * uint64_t(ctx->blocks_per_cu));
+// Resident blocks of count_probe (all forms, or lane form only) on the
+// context's device.
+inline uint64_t resident_blocks(const bbtc_ctx* ctx, bool lane_only = false) {
+  return std::max<uint64_t>(1, uint64_t(ctx->num_cu) *
+                                   uint64_t(lane_only ? ctx->blocks_per_cu_lane : ctx->blocks_per_cu));
 }
 
 // Enqueues one counting launch on s: reset of the slot (first item i0), the
-// first phase over `blocks` blocks, the second over the whole machine.
-void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, unsigned blocks, hipStream_t s);
+// first phase over at most `blocks` blocks, the second over the whole
+// machine. lane_only: every item of the launch is of the lane form, which
+// runs in its own kernel instantiation (no LDS, more resident waves) and
+// never spills, so there is no second phase.
+// chunk: items per chunk of the first-phase parts (0: one chunk per part).
+void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t items, bool lane_only,
+                   uint64_t chunk, hipStream_t s);
 
 // Enqueues p[0..n) = 0 on s.
 void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s);

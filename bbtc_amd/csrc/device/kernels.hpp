@@ -48,6 +48,12 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
   return x;
 }
 
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t x) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
@@ -105,6 +111,32 @@ __device__ __forceinline__ bool sorted_contains(Ptr s, uint32_t n, uint32_t x) {
   return lo < n && s[lo] == x;
 }
 
+// |A[a, ae) ∩ B[b, be)| for sorted lists of distinct entries, four entries
+// of each list at a time: the eight loads of a step are in flight together,
+// then all 16 pairs are compared, and the block with the smaller last entry
+// moves on (both when equal). A block past its list's end is padded with
+// values no entry takes (ids are below 2^32 - 2), distinct between the lists.
+__device__ __forceinline__ uint32_t intersect_blocked(gptr<uint32_t> A, uint32_t a, uint32_t ae,
+                                                      gptr<uint32_t> B, uint32_t b, uint32_t be) {
+  uint32_t n = 0;
+  while (a < ae && b < be) {
+    uint32_t x[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      x[k] = a + k < ae ? A[a + k] : 0xFFFFFFFFu;
+      y[k] = b + k < be ? B[b + k] : 0xFFFFFFFEu;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) n += x[i] == y[j] ? 1u : 0u;
+    const uint32_t xl = x[3], yl = y[3];
+    if (xl <= yl) a += 4;
+    if (yl <= xl) b += 4;
+  }
+  return n;
+}
+
 // Largest t with prefix[t] <= item (prefix[n] > item).
 __device__ __forceinline__ uint32_t find_task(gptr<uint64_t> prefix, uint32_t n,
                                               uint64_t item) {
@@ -137,26 +169,39 @@ __global__ void zero_counts(unsigned long long* __restrict__ p, uint64_t n) {
 }
 
 // Resets a launch slot: first-phase items from i0, no spilled work.
-__global__ void reset_slot(SlotCtl* __restrict__ c, unsigned long long i0) {
-  c->head = i0;
+__global__ void reset_slot(SlotCtl* __restrict__ c, unsigned long long i0, unsigned long long i1,
+                           unsigned long long chunk) {
+  const unsigned long long len = i1 > i0 ? i1 - i0 : 0;
+  const unsigned long long C = part_chunk(len, chunk);
+  for (unsigned q = 0; q < unsigned(kQueues); ++q) {
+    c->head[q * kCtlStride] = 0;
+    c->end[q] = part_positions(len, q, C);
+  }
+  c->base = i0;
+  c->chunk = C;
+  c->shift = part_shift(len, chunk);
   c->spill = 0;
   c->head2 = 0;
 }
 
-// One persistent launch per phase. Phase 0 drains items [ctl->head,
-// item_end); a unit is an item's whole wedge range. Phase 1 drains the
+// One persistent launch per phase. Phase 0 drains the first-phase items
+// (kQueues parts with their own heads); a unit is an item's whole wedge range. Phase 1 drains the
 // chunks that phase 0 spilled; a unit is one chunk of an item's wedges.
-__global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
-  __shared__ __attribute__((aligned(16))) uint32_t lds_all[kWavesPerBlock * kWaveLds];
+// kAll: every form (count_probe); otherwise the lane form only
+// (count_probe_lanes: no LDS, fewer registers, more resident waves).
+template <bool kAll>
+__device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_all) {
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t* lds = lds_all + (threadIdx.x / kWave) * kWaveLds;
+  uint32_t* lds = kAll ? lds_all + (threadIdx.x / kWave) * kWaveLds : nullptr;
   uint32_t* mk_d = lds + kLdsWords;  // x-index base of the wedge starting here
   uint32_t* mk_s = mk_d + kWave;     // its row's membership slot
   uint32_t* mk_n = mk_s + kWave;     // its row's list length (search form)
   uint32_t* mk_f = mk_n + kWave;     // a wedge starts here
-  // bitmap slices are all-zero between units
-  for (int x = lane; x < kLdsWords; x += kWave) lds[x] = 0;
-  wave_lds_sync();
+  if constexpr (kAll) {
+    // bitmap slices are all-zero between units
+    for (int x = lane; x < kLdsWords; x += kWave) lds[x] = 0;
+    wave_lds_sync();
+  }
 
   // current segment and task; [t_lo, t_hi) are t_cur's launch-local items
   uint32_t s_cur = 0;
@@ -192,21 +237,38 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   }
 
   unsigned long long item = 0, batch_end = 0;
+  uint32_t part = blockIdx.x % kQueues, parts_left = kQueues;  // current part; parts not yet drained
+  const unsigned long long base = a.phase == 0 ? a.ctl->base : 0;
+  const unsigned long long chunk = a.phase == 0 ? a.ctl->chunk : 1;
+  const unsigned shift = a.phase == 0 ? unsigned(a.ctl->shift) : 0u;
   for (;;) {
     uint64_t it;
     uint32_t wlo, whi;
     bool may_defer;
     if (a.phase == 0) {
-      // Items are claimed kItemBatch at a time: one device-scope atomic per
-      // batch instead of per item.
+      // Items are claimed kItemBatch at a time from the current part (one
+      // device-scope atomic per batch); a drained part sends the wave on to
+      // the next one.
       if (item >= batch_end) {
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&a.ctl->head, (unsigned long long)kItemBatch);
-        item = uniform64(__shfl(b, 0, kWave));
-        batch_end = item + kItemBatch;
-        if (batch_end > a.item_end) batch_end = a.item_end;
+        unsigned long long old = 0, pe = 0;
+        if (lane == 0) {
+          while (parts_left) {
+            pe = a.ctl->end[part];
+            old = atomicAdd(&a.ctl->head[part * kCtlStride], (unsigned long long)kItemBatch);
+            if (old < pe) break;
+            part = part + 1 == kQueues ? 0 : part + 1;
+            --parts_left;
+          }
+        }
+        parts_left = __builtin_amdgcn_readfirstlane(parts_left);
+        if (!parts_left) break;
+        part = __builtin_amdgcn_readfirstlane(part);
+        old = uniform64(__shfl(old, 0, kWave));
+        pe = uniform64(__shfl(pe, 0, kWave));
+        // a claim stays inside one chunk (kItemBatch divides the chunk)
+        item = part_item(base, part, old, chunk, shift);
+        batch_end = item + ((old + kItemBatch < pe ? old + kItemBatch : pe) - old);
       }
-      if (item >= a.item_end) break;
       it = item++;
       wlo = 0;
       whi = 0xffffffffu;
@@ -278,6 +340,31 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
       dac = ac_rp[r + 1] - ac0;
       if (dac == 0) dab = 0;
       if (dab == 0) dac = 0;
+    }
+    if (!kAll || T.kind == kKindLane) {
+      // Lane form: one lane per wedge (u, v), v in ab(u), intersecting bc(v)
+      // with ac(u) straight from HBM (rows of at most kLaneMaxRow entries, so
+      // an item is bounded work and is never split).
+      const uint32_t inclW = wave_incl_scan(dab, lane);
+      const uint32_t exclW = inclW - dab;
+      const uint32_t W = __builtin_amdgcn_readlane(inclW, kWave - 1);
+      uint32_t c = 0;
+      for (uint32_t wb = 0; wb < W; wb += kWave) {
+        const uint32_t w = wb + lane;
+        const int q = owner_of(inclW, w);
+        const uint32_t q_ab0 = __shfl(ab0, q, kWave);
+        const uint32_t q_ex = __shfl(exclW, q, kWave);
+        const uint32_t q_ac0 = __shfl(ac0, q, kWave);
+        const uint32_t q_dac = __shfl(dac, q, kWave);
+        if (w < W) {
+          const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
+          c += intersect_blocked(ac_c, q_ac0, q_ac0 + q_dac, bc_c, bc_rp[v], bc_rp[v + 1]);
+        }
+      }
+      acc += wave_sum32(c);
+      if (S.cycles) acc_cyc += clock64() - t_start;
+      ++run;
+      continue;
     }
     // rows -> wedges (u, v), v in ab(u); the unit covers wedges [wlo, hi)
     const uint32_t inclW = wave_incl_scan(dab, lane);
@@ -501,6 +588,15 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   flush();
 }
 
+__global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
+  __shared__ __attribute__((aligned(16))) uint32_t lds_all[kWavesPerBlock * kWaveLds];
+  count_units<true>(a, lds_all);
+}
+
+__global__ __launch_bounds__(kBlock) void count_probe_lanes(CountArgs a) {
+  count_units<false>(a, nullptr);
+}
+
 // Start of a batch launch: stamp the device wall clock and reset each
 // graph's outstanding item count (graphs without items finish at once).
 __global__ void batch_begin(const unsigned long long* __restrict__ left0, uint64_t ngraphs,
@@ -513,6 +609,33 @@ __global__ void batch_begin(const unsigned long long* __restrict__ left0, uint64
     left[g] = left0[g];
     done[g] = left0[g] ? 0ull : now;
   }
+}
+
+// Streaming read of n 16-byte words: the achievable HBM read bandwidth that
+// the counting kernels are compared with. Each block reads one contiguous
+// range, each lane four independent 16-byte nontemporal loads per pass (the
+// fastest of the shapes measured on MI355X: grid-stride or blocked ranges,
+// 2/4/8 loads in flight, cached or nontemporal, 1K-32K blocks). The XOR of
+// everything read decides a store that the data never triggers (the buffer
+// holds zeros), so no load can be dropped.
+__global__ __launch_bounds__(kBlock) void stream_read(const u32x4* __restrict__ words, uint64_t n,
+                                                      unsigned* __restrict__ sink) {
+  const gptr<u32x4> p = as_global(words);
+  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t b0 = uint64_t(blockIdx.x) * per;
+  const uint64_t b1 = b0 + per < n ? b0 + per : n;
+  uint64_t i = b0 + threadIdx.x;
+  u32x4 acc = {0u, 0u, 0u, 0u};
+  for (; i + 3 * kBlock < b1; i += 4 * kBlock) {
+    const u32x4 w0 = __builtin_nontemporal_load(&p[i]);
+    const u32x4 w1 = __builtin_nontemporal_load(&p[i + kBlock]);
+    const u32x4 w2 = __builtin_nontemporal_load(&p[i + 2 * kBlock]);
+    const u32x4 w3 = __builtin_nontemporal_load(&p[i + 3 * kBlock]);
+    acc ^= w0 ^ w1 ^ w2 ^ w3;
+  }
+  for (; i < b1; i += kBlock) acc ^= p[i];
+  const unsigned x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (x == 0x9E3779B9u) sink[blockIdx.x] = x;
 }
 
 }  // namespace bbtc

@@ -24,6 +24,7 @@ struct bbtc_ctx {
   int device = 0;
   int num_cu = 0;
   int blocks_per_cu = 1;  // resident blocks of count_probe per CU
+  int blocks_per_cu_lane = 1;  // the same for its lane-form instantiation
   hipStream_t stream = nullptr;          // slot 0
   std::vector<hipStream_t> slots;         // slot i > 0, created on first use
   uint64_t budget = 0;
@@ -57,7 +58,9 @@ struct bbtc_plan {
   uint64_t* item_prefix = nullptr;
   bbtc::SegDesc* segs = nullptr;        // the whole plan: [0] plain, [1] with clock counters
   std::vector<uint64_t> hprefix;        // host copy of item_prefix
-  std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense)
+  std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense, 5 lane)
+  bool lane_only = false;               // every task with items uses the lane form
+  uint64_t chunk = 0;                   // first-phase chunk (0: one per part; batches: list order)
   unsigned long long* per_task = nullptr;
   bbtc::SlotCtl* ctl = nullptr;         // one control block per launch slot
   struct Spill {

@@ -9,10 +9,12 @@
 // on a copy stream while the device counts earlier requests.
 //
 // The scheduler thread keeps up to `max_inflight` launches queued on the
-// compute stream. Each launch is one count_probe launch pair over a budget of
-// `quantum_items` work items: the oldest resident request with work left
-// keeps half of it, and the rest is filled with slices of the resident
-// requests in order of remaining work (shortest first). A request that
+// compute stream. Each launch counts a budget of `quantum_items` work items
+// of one kind of request: those whose tasks all use the lane form (the lane
+// kernel) or the others (count_probe), the kinds taking turns while both
+// have work. Within the kind, the oldest resident request with work left
+// keeps half of the budget, and the rest is filled with slices of the
+// resident requests in order of remaining work (shortest first). A request that
 // arrives while others run joins the next launch and shares the machine with
 // them; a large request is slowed at most about twofold by later arrivals. A
 // request finishes with the launch that holds its last slice. Its counters
@@ -130,6 +132,8 @@ struct bbtc_server {
   uint64_t quantum = 0;
   uint32_t max_inflight = 0, max_segments = 0;
   bool sequential = false;
+  bool lane_turn = false;  // kind of the next launch when both kinds have work
+  bool no_lane_form = false;
 
   char* heap_base = nullptr;
   bbtc::Heap heap;
@@ -379,6 +383,7 @@ void sched_loop(bbtc_server* S) {
         SegDesc* sh = S->seg_host[ring];
         uint32_t nseg = 0;
         uint64_t total = 0;
+        bool lane_only = true;  // every segment's request uses the lane form only
         Launch l;
         l.ring = ring;
         const double t = S->now();
@@ -388,6 +393,18 @@ void sched_loop(bbtc_server* S) {
         std::vector<Request*> cand;  // admission order
         for (auto& r : act)
           if (r->next < r->nitems) cand.push_back(r.get());
+        // A launch holds requests of one kind: all in the lane form (run by
+        // the lane kernel) or none; the two kinds take turns while both have
+        // work.
+        bool kinds[2] = {false, false};
+        for (Request* r : cand) kinds[r->plan.lane_only ? 1 : 0] = true;
+        if (kinds[0] && kinds[1]) {
+          const bool lane_turn = S->lane_turn;
+          S->lane_turn = !S->lane_turn;
+          cand.erase(std::remove_if(cand.begin(), cand.end(),
+                                    [&](Request* r) { return r->plan.lane_only != lane_turn; }),
+                     cand.end());
+        }
         std::vector<uint64_t> take(cand.size(), 0);
         uint64_t budget = S->sequential ? ~0ull : S->quantum;
         if (!S->sequential && !cand.empty()) {
@@ -414,6 +431,7 @@ void sched_loop(bbtc_server* S) {
           d.launch0 = total;
           sh[nseg++] = d;
           total += take[x];
+          lane_only = lane_only && r->plan.lane_only;
           r->next += take[x];
           r->launches++;
           if (r->start_s < 0) r->start_s = t;
@@ -433,9 +451,7 @@ void sched_loop(bbtc_server* S) {
           a.spill = S->spill;
           a.spill_base = S->spill_base;
           a.spill_cap = kSpillCap;
-          const uint64_t want = (total + kWavesPerBlock - 1) / kWavesPerBlock;
-          const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(S->ctx))));
-          enqueue_count(S->ctx, a, 0, blocks, s);
+          enqueue_count(S->ctx, a, 0, total, lane_only, kListChunkItems, s);
           if (S->event_pool.empty()) {
             hipEvent_t e;
             HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -538,6 +554,7 @@ bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc
     S->max_inflight = opts && opts->max_inflight ? opts->max_inflight : bbtc::kDefaultInflight;
     S->max_segments = opts && opts->max_segments ? opts->max_segments : bbtc::kDefaultSegments;
     S->sequential = opts && opts->sequential;
+    S->no_lane_form = opts && opts->no_lane_form;
     if (S->sequential) S->max_inflight = 1;
     HIP_OK(hipMalloc(&S->heap_base, heap));
     S->heap.init(heap);
@@ -604,7 +621,8 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
   auto r = std::make_shared<bbtc::Request>();
   r->L = &L->L;
   r->tiles = bbtc::tile_table(L->L);
-  r->plan = bbtc::describe_plan(r->tiles, L->L.deg, bbtc::refs_of(P, tasks, ntasks), kernel, nullptr);
+  r->plan = bbtc::describe_plan(r->tiles, L->L.deg, bbtc::refs_of(P, tasks, ntasks), kernel, nullptr,
+                                !S->no_lane_form);
   r->ntasks = ntasks;
   r->nitems = r->plan.pref[ntasks];
   r->per_task_out = per_task_out;

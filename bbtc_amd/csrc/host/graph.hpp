@@ -66,6 +66,7 @@ struct Layout {
   std::vector<uint64_t> nz_base;   // P*P
   std::vector<uint64_t> nnz;       // P*P
   std::vector<uint32_t> nz_rows;   // P*P
+  std::vector<uint32_t> max_row;   // P*P, entries in the tile's longest row
   std::vector<float> deg;          // P*P, (float)nnz / (float)h
   std::vector<uint64_t> dense_base; // P*P, ~0 when the tile has no bitmap form
   std::vector<uint32_t> dwpr;       // P, bitmap words per row of band k

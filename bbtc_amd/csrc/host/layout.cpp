@@ -84,6 +84,7 @@ Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
   L.nz_base.assign(T, ~0ull);
   L.nnz.assign(T, 0);
   L.nz_rows.assign(T, 0);
+  L.max_row.assign(T, 0);
   L.deg.assign(T, 0.f);
 
   uint64_t rp_total = 0;
@@ -133,16 +134,18 @@ Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
     uint32_t* row = rp + L.rp_base[t];
     const uint32_t h = L.height(i);
     uint64_t acc = 0;
-    uint32_t nzr = 0;
+    uint32_t nzr = 0, longest = 0;
     for (uint32_t r = 0; r <= h; ++r) {
       const uint64_t c = row[r];
       nzr += (r > 0 && c > 0) ? 1u : 0u;
+      longest = std::max(longest, uint32_t(c));
       acc += c;
       if (acc > 0xFFFFFFFFull) { overflow = 1; break; }
       row[r] = uint32_t(acc);
     }
     L.nnz[t] = acc;
     L.nz_rows[t] = nzr;
+    L.max_row[t] = longest;
   }
   if (overflow) fail(BBTC_ERR_INVALID, "a tile holds 2^32 or more entries; use more cuts");
 

@@ -12,7 +12,11 @@ launch stream give the kernel time behind the roofline figure.
 
 Prints one JSON line:
   value         edges/s = m / (time per step), m = upper-triangle entries
-  roofline      algorithmic bytes per step / mean event time vs 8 TB/s
+  roofline      algorithmic bytes per step / mean event time vs 8 TB/s, and
+                vs the measured streaming-read bandwidth (achievable); HBM
+                traffic from the committed rocprofv3 counters; the
+                compulsory bytes (each task's three tiles read once)
+  end_to_end_s  generate + partition + layout, upload, median count
   cpu_baseline  the CPU oracle (restated reference CPU path) on a sample of
                 the same tasks, scaled by algorithmic bytes
   c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), tuned forms:
@@ -59,6 +63,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def compulsory_bytes(L, tasks, offset_bytes):
+    """B_min: each tile of each counted task read once, as the reference's
+    device tile holds it ((h+1) row offsets and 4-byte column ids); tasks with
+    an empty tile are skipped as in bb_tc.cpp:86-91."""
+    P = len(L.cuts) - 1
+    tile_bytes = np.zeros((P, P), dtype=np.int64)
+    empty = np.zeros((P, P), dtype=bool)
+    for i in range(P):
+        for k in range(i, P):
+            t = L.tile(i, k)
+            tile_bytes[i, k] = (t.height + 1) * offset_bytes + 4 * t.nnz
+            empty[i, k] = t.nnz == 0
+    ti, tj, tk = (tasks[c].astype(np.int64) for c in ("i", "j", "k"))
+    keep = ~(empty[ti, tj] | empty[tj, tk] | empty[ti, tk])
+    return int((tile_bytes[ti, tj] + tile_bytes[tj, tk] + tile_bytes[ti, tk])[keep].sum())
+
+
+def host_info(nthreads):
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"nproc": usable, "omp_threads": nthreads, "cpu_model": model}
+
+
 def cpu_baseline(g, cuts, tasks, per_task_gpu, bytes_per_task, total_bytes, m, budget_s):
     """Time the oracle on a sample of tasks (every s-th in lexicographic order)
     whose algorithmic bytes are about budget_s of CPU work; scale to the whole
@@ -91,6 +129,7 @@ def cpu_baseline(g, cuts, tasks, per_task_gpu, bytes_per_task, total_bytes, m, b
         "unit": "edges/s",
         "cores": nthreads,
         "kind": "port",
+        **host_info(nthreads),
         "sample": (f"oracle (C restatement of the reference CPU path) on {len(chosen)} of {len(tasks)} "
                    f"tasks (every {stride}th in lexicographic order), {dt:.1f}s for "
                    f"{sb / 1e9:.1f} GB algorithmic; scaled by algorithmic bytes to the full task set "
@@ -122,7 +161,7 @@ def count_c2(B, dev, steps):
             "ms_per_count": mean_s * 1e3, "triangles": int(total), "tasks": int(len(tasks)),
             "algorithmic_bytes": int(total_bytes), "offset_bytes": 4,
             "roofline_frac": total_bytes / mean_s / 1e9 / HBM_PEAK_GBS,
-            "forms_merge_bitmap_dense_rdense": np.bincount(choice, minlength=5)[1:].tolist()}
+            "forms_merge_bitmap_dense_rdense_lane": np.bincount(choice, minlength=6)[1:].tolist()}
 
 
 def main():
@@ -135,7 +174,7 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--ncut", type=int, default=32)
     ap.add_argument("--partitioner", default="pal")
-    ap.add_argument("--kernel", default="tuned", help="tuned | auto | merge | bitmap | dense | rdense")
+    ap.add_argument("--kernel", default="tuned", help="tuned | auto | merge | bitmap | dense | rdense | lane")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the G(n=2^22, m=2^26) count")
@@ -152,6 +191,10 @@ def main():
 
     offset_bytes = 8 if args.scale >= 26 else 4
     dev = B.Device(0)
+    read_gbs = dev.read_bandwidth(8 << 30, 5)
+    stream_gbs = float(np.median(read_gbs))
+    log(f"streaming read of 8 GiB: {np.round(read_gbs, 1).tolist()} GB/s (median {stream_gbs:.1f})")
+    t_e2e = time.time()
     t0 = time.time()
     g = B.Graph.rmat_device(dev, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
     n, _, m = g.info
@@ -162,21 +205,25 @@ def main():
     L = B.Layout(g, cuts)
     tasks = L.tasks("desc")
     log(f"partition+layout: {time.time() - t0:.1f}s, {len(tasks)} tasks, layout {L.device_bytes / 1e9:.2f} GB")
+    prep_s = time.time() - t_e2e  # generate, partition, layout
     t0 = time.time()
     bytes_per_task, total_bytes, wedges = L.task_bytes(tasks, offset_bytes)
     log(f"algorithmic bytes: {total_bytes / 1e9:.1f} GB (o={offset_bytes}), wedges {wedges} ({time.time() - t0:.1f}s)")
+    min_bytes = compulsory_bytes(L, tasks, offset_bytes)
+    log(f"compulsory bytes (each task's three tiles once): {min_bytes / 1e9:.1f} GB")
 
     t0 = time.time()
     dg = dev.upload(L)
-    log(f"upload: {time.time() - t0:.1f}s ({dg.bytes / 1e9:.2f} GB resident)")
+    upload_s = time.time() - t0
+    log(f"upload: {upload_s:.1f}s ({dg.bytes / 1e9:.2f} GB resident)")
     tune_s = 0.0
     if args.kernel == "tuned":
         t0 = time.time()
         choice, _ = dg.autotune(tasks)
         tune_s = time.time() - t0
         plan = dg.plan(tasks, "auto", choice)
-        log(f"autotune: {tune_s:.1f}s, forms merge/bitmap/dense/rdense = "
-            f"{np.bincount(choice, minlength=5)[1:].tolist()}")
+        log(f"autotune: {tune_s:.1f}s, forms merge/bitmap/dense/rdense/lane = "
+            f"{np.bincount(choice, minlength=6)[1:].tolist()}")
     else:
         plan = dg.plan(tasks, args.kernel)
     if args.warmup:
@@ -189,6 +236,7 @@ def main():
     per, total = plan.results()
     ms_per_step = wall * 1e3 / args.steps
     ev_mean_s = float(np.mean(ms)) * 1e-3
+    ev_median_ms = float(np.median(ms))
     achieved = total_bytes / ev_mean_s / 1e9
     log(f"count: {total} triangles; events ms {np.round(ms, 3).tolist()}; wall/step {ms_per_step:.3f} ms; "
         f"bitmap tasks {plan.bitmap_tasks}")
@@ -220,12 +268,19 @@ def main():
             "partitioner": args.partitioner, "kernel": args.kernel,
             "triangles": int(total), "autotune_s": round(tune_s, 2),
         },
+        "end_to_end_s": {
+            "generate_partition_layout": round(prep_s, 3), "upload": round(upload_s, 3),
+            "count_median": round(ev_median_ms * 1e-3, 4),
+            "total": round(prep_s + upload_s + ev_median_ms * 1e-3, 3),
+        },
         "triangles_per_s": total / (ms_per_step * 1e-3),
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "algorithmic_bytes": total_bytes, "offset_bytes": offset_bytes,
-            "kernel": "count_probe", "event_ms_mean": ev_mean_s * 1e3,
+            "kernel": "count_probe", "event_ms_mean": ev_mean_s * 1e3, "event_ms_median": ev_median_ms,
+            "achievable": stream_gbs, "frac_achievable": achieved / stream_gbs,
+            "compulsory_bytes": min_bytes,
         },
         "cpu_baseline": cpu,
     }

@@ -200,7 +200,8 @@ typedef struct bbtc_dgraph bbtc_dgraph;
 typedef struct bbtc_plan bbtc_plan;
 
 typedef enum bbtc_kernel {
-  BBTC_KERNEL_AUTO = 0,    /* per task without measurements: RDENSE where the bc tile has a
+  BBTC_KERNEL_AUTO = 0,    /* without measurements: LANE for every task when every task with
+                              work admits it; otherwise per task RDENSE where the bc tile has a
                               bitmap form, else BITMAP where a row of the k band fits the LDS
                               slice, else MERGE (bbtc_autotune measures instead) */
   BBTC_KERNEL_MERGE = 1,   /* bc(v) entries binary-searched in ac(u) (LDS copy) */
@@ -208,8 +209,12 @@ typedef enum bbtc_kernel {
                               band fits the slice */
   BBTC_KERNEL_DENSE = 3,   /* LDS bitmap of ac(u) ANDed with the bitmap row of bc(v) where the
                               bc tile has one, else as BITMAP */
-  BBTC_KERNEL_RDENSE = 4   /* ac(u) entries tested against the bitmap row of bc(v) where the
+  BBTC_KERNEL_RDENSE = 4,  /* ac(u) entries tested against the bitmap row of bc(v) where the
                               bc tile has one, else as MERGE */
+  BBTC_KERNEL_LANE = 5     /* one lane per row u merges bc(v) with ac(u) in HBM, the
+                              per-thread row merge of SparseKernel (gpu_sparse_kernel.cu:6-49),
+                              where no row of the three tiles holds more than 32 entries,
+                              else as MERGE */
 } bbtc_kernel;
 
 typedef struct bbtc_opts {
@@ -228,6 +233,13 @@ void bbtc_ctx_destroy(bbtc_ctx* ctx);
 bbtc_status bbtc_ctx_synchronize(bbtc_ctx* ctx);
 /* Device bytes currently held by graphs and plans of this context. */
 uint64_t bbtc_ctx_bytes_in_use(const bbtc_ctx* ctx);
+/* Achievable HBM read bandwidth of the context's device: a streaming read
+ * kernel over `bytes` (at least 1 MiB, rounded down to 16-byte words) of
+ * zeroed device memory, one untimed pass and then `iters` passes timed with
+ * HIP events; gbps_out[i] = bytes read / time of pass i, in GB/s (1e9 B/s).
+ * The buffer is allocated and freed inside the call. No counterpart in the
+ * reference: it is the measured ceiling reported beside the spec peak. */
+bbtc_status bbtc_ctx_read_bandwidth(bbtc_ctx* ctx, uint64_t bytes, int iters, double* gbps_out);
 
 /* The seeded generators above run on the context's device (radix sorts in
  * HBM) and return the same host CSR, bit for bit, much faster for large
@@ -250,7 +262,8 @@ void bbtc_graph_release(bbtc_dgraph* g);
 bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                              bbtc_kernel kernel, bbtc_plan** out);
 /* As bbtc_plan_create, with a per-task override choice[q]: 0 keeps `kernel`'s
- * rule, 1 forces MERGE, 2 BITMAP, 3 DENSE, 4 RDENSE (each where it applies).
+ * rule, 1 forces MERGE, 2 BITMAP, 3 DENSE, 4 RDENSE, 5 LANE (each where it
+ * applies).
  * Used with bbtc_autotune's output; choice may be NULL. */
 bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks,
                                    uint64_t n, bbtc_kernel kernel, const uint8_t* choice,
@@ -273,14 +286,17 @@ bbtc_status bbtc_plan_time_tasks(bbtc_plan* plan, float* ms_per_task);
  * while all tasks share the device (the autotuner's cost measure). */
 bbtc_status bbtc_plan_profile(bbtc_plan* plan, uint64_t* cycles_per_task);
 /* Autotuner: measures every form on the whole task list (each forced where it
- * applies, per-item clock counters) and writes per task the cheapest form
- * (1 merge, 2 bitmap, 3 dense, 4 rdense) into choice_out[n], ready for
- * bbtc_plan_create_tuned. cycles_out (may be NULL) receives n x 4 clock
- * counts in that order, UINT64_MAX where a form does not apply.
+ * applies, per-item clock counters) and writes per task the cheapest of the
+ * forms 1 merge, 2 bitmap, 3 dense, 4 rdense into choice_out[n], ready for
+ * bbtc_plan_create_tuned. The lane form (5) runs in a kernel of its own only
+ * when every task of a launch uses it, so where it applies to every task with
+ * work, that per-task choice is timed against all tasks in the lane form and
+ * the faster one is written. cycles_out (may be NULL) receives n x 5 clock
+ * counts (forms 1-5 in order), UINT64_MAX where a form does not apply.
  * Replaces the fixed threshold of bb_tc.cpp:98. */
 bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                           uint8_t* choice_out, uint64_t* cycles_out);
-/* Form the plan chose per task: 1 merge, 2 bitmap, 3 dense, 4 rdense. */
+/* Form the plan chose per task: 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane. */
 bbtc_status bbtc_plan_task_kinds(const bbtc_plan* plan, uint8_t* kinds);
 /* Number of tasks the plan routed to an LDS bitmap form (bitmap or dense). */
 uint64_t bbtc_plan_bitmap_tasks(const bbtc_plan* plan);
@@ -373,7 +389,9 @@ typedef struct bbtc_server_opts {
   uint32_t max_inflight;   /* launches queued on the device at once (0: 2) */
   uint32_t max_segments;   /* requests per launch (0: 1024) */
   uint32_t sequential;     /* 1: one request at a time, whole plan per launch */
-  uint32_t reserved;
+  uint32_t no_lane_form;   /* 1: BBTC_KERNEL_AUTO requests never take the lane form, so any
+                              requests share a launch (0: requests whose tasks all take it run
+                              in launches of their own, taking turns with the others) */
 } bbtc_server_opts;
 
 typedef struct bbtc_server_result {

@@ -21,7 +21,7 @@ from tests.test_count_cpu import CLOSED
 
 pytestmark = pytest.mark.gpu
 
-FORMS = ["merge", "bitmap", "dense", "rdense", "auto"]
+FORMS = ["merge", "bitmap", "dense", "rdense", "lane", "auto"]
 GOLDEN = load_golden()
 COUNTED = [(d, r) for d in GOLDEN for r in d["runs"] if "tasks" in r]
 
@@ -186,6 +186,18 @@ def test_budget_is_enforced():
     d.close()
 
 
+def test_read_bandwidth(dev):
+    before = dev.bytes_in_use
+    gbs = dev.read_bandwidth(1 << 30, 3)
+    log("streaming read GB/s", gbs.tolist())
+    # a streaming read of 1 GiB runs well above PCIe rates and below the 8 TB/s spec peak
+    assert gbs.shape == (3,) and np.all(gbs > 500) and np.all(gbs < 8000)
+    assert dev.bytes_in_use == before
+    with pytest.raises(B.BbtcError) as e:
+        dev.read_bandwidth(1024, 1)
+    assert e.value.status == 1  # BBTC_ERR_INVALID
+
+
 def test_cli_gpu(tmp_path):
     d = load_golden("c1_rmat_s16_ef8")[0]
     g = graph_from_spec(d["graph"])
@@ -193,7 +205,8 @@ def test_cli_gpu(tmp_path):
     g.save(path)
     for run in d["runs"]:
         for extra in ([], ["--border", "-1"], ["--border", "7", "--nstream", "2"], ["--copy-first"],
-                      ["--kernel", "tuned", "--border", "-1"], ["--kernel", "rdense"], ["--kernel", "dense"]):
+                      ["--kernel", "tuned", "--border", "-1"], ["--kernel", "rdense"], ["--kernel", "dense"],
+                      ["--kernel", "lane"]):
             out = tmp_path / "pt.txt"
             cmd = [B.CLI_PATH, "--graph", path, "--partitioner", run["partitioner"], "--ncut", str(run["ncut"]),
                    "--repeat", "2", "--ngpu", "1", "--per-task-out", str(out)] + extra
@@ -250,8 +263,10 @@ def test_s26_properties(dev, s26):
     dg = dev.upload(L)
     per_a, tot_a = dg.count(t, "auto")
     per_r, tot_r = dg.count(t, "rdense")
+    per_l, tot_l = dg.count(t, "lane")
     assert int(per_a.sum()) == tot_a
     assert tot_a == tot_r and np.array_equal(per_a, per_r)
+    assert tot_a == tot_l and np.array_equal(per_a, per_l)
     # oracle on every 97th task (lexicographic), bit-exact
     lex = np.lexsort((t["k"], t["j"], t["i"]))[::97]
     R = O.RefLayout(g.offsets(), g.cols(), cuts)
