@@ -241,6 +241,7 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
   const unsigned long long base = a.phase == 0 ? a.ctl->base : 0;
   const unsigned long long chunk = a.phase == 0 ? a.ctl->chunk : 1;
   const unsigned shift = a.phase == 0 ? unsigned(a.ctl->shift) : 0u;
+  constexpr unsigned long long batch = kItemBatch;
   for (;;) {
     uint64_t it;
     uint32_t wlo, whi;
@@ -254,7 +255,7 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
         if (lane == 0) {
           while (parts_left) {
             pe = a.ctl->end[part];
-            old = atomicAdd(&a.ctl->head[part * kCtlStride], (unsigned long long)kItemBatch);
+            old = atomicAdd(&a.ctl->head[part * kCtlStride], (unsigned long long)batch);
             if (old < pe) break;
             part = part + 1 == kQueues ? 0 : part + 1;
             --parts_left;
@@ -265,9 +266,9 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
         part = __builtin_amdgcn_readfirstlane(part);
         old = uniform64(__shfl(old, 0, kWave));
         pe = uniform64(__shfl(pe, 0, kWave));
-        // a claim stays inside one chunk (kItemBatch divides the chunk)
+        // a claim stays inside one chunk (the batch divides the chunk)
         item = part_item(base, part, old, chunk, shift);
-        batch_end = item + ((old + kItemBatch < pe ? old + kItemBatch : pe) - old);
+        batch_end = item + ((old + batch < pe ? old + batch : pe) - old);
       }
       it = item++;
       wlo = 0;

@@ -9,10 +9,13 @@
 // on a copy stream while the device counts earlier requests.
 //
 // The scheduler thread keeps up to `max_inflight` launches queued on the
-// compute stream. Each launch counts a budget of `quantum_items` work items
-// of one kind of request: those whose tasks all use the lane form (the lane
-// kernel) or the others (count_probe), the kinds taking turns while both
-// have work. Within the kind, the oldest resident request with work left
+// compute stream. Each launch counts a budget of work items of one kind of
+// request: those whose tasks all use the lane form (the lane kernel) or the
+// others (count_probe), the kinds taking turns while both have work. The
+// budget is `quantum_items`, or by default follows each kind's measured item
+// rate so that a launch takes about `launch_us` (a launch's fixed costs stay
+// small next to its work, and an arrival waits at most about two launches).
+// Within the kind, the oldest resident request with work left
 // keeps half of the budget, and the rest is filled with slices of the
 // resident requests in order of remaining work (shortest first). A request that
 // arrives while others run joins the next launch and shares the machine with
@@ -48,6 +51,10 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kAlign = 256;
 constexpr uint64_t kDefaultHeap = 200ull << 30;
 constexpr uint64_t kDefaultQuantum = 1ull << 18;
+// adaptive launch budget: limits, and the default target launch time
+constexpr uint64_t kMinQuantum = 1ull << 14;
+constexpr uint64_t kMaxQuantum = 1ull << 24;
+constexpr double kDefaultLaunchMs = 10.0;
 constexpr uint32_t kDefaultInflight = 2;
 constexpr uint32_t kDefaultSegments = 1024;
 
@@ -119,7 +126,9 @@ struct Request {
 };
 
 struct Launch {
-  hipEvent_t done = nullptr;
+  hipEvent_t start = nullptr, done = nullptr;  // around the launch on the compute stream
+  uint64_t items = 0;
+  int kind = 0;  // 1: lane-form requests only
   int ring = 0;
   std::vector<std::shared_ptr<Request>> finishing;
 };
@@ -129,7 +138,9 @@ struct Launch {
 
 struct bbtc_server {
   bbtc_ctx* ctx = nullptr;
-  uint64_t quantum = 0;
+  uint64_t quantum[2] = {0, 0};  // item budget of the next launch of each kind (lane form: [1])
+  bool adaptive = false;         // the budgets follow the measured rates
+  double launch_ms = 0;      // target launch time when adaptive
   uint32_t max_inflight = 0, max_segments = 0;
   bool sequential = false;
   bool lane_turn = false;  // kind of the next launch when both kinds have work
@@ -373,6 +384,20 @@ void sched_loop(bbtc_server* S) {
           finish(S, *r);
           act.erase(std::find(act.begin(), act.end(), r));
         }
+        if (S->adaptive) {
+          // next budget: the items this launch would have counted in the
+          // target time at its measured rate, at most doubling or halving
+          float ms = 0.f;
+          HIP_OK(hipEventElapsedTime(&ms, l.start, l.done));
+          if (ms > 0.01f && l.items) {
+            const double want = double(l.items) * S->launch_ms / double(ms);
+            uint64_t& q = S->quantum[l.kind];
+            const double lo = std::max(double(kMinQuantum), 0.5 * double(q));
+            const double hi = std::min(double(kMaxQuantum), 2.0 * double(q));
+            q = uint64_t(std::min(hi, std::max(lo, want)));
+          }
+        }
+        S->event_pool.push_back(l.start);
         S->event_pool.push_back(l.done);
         ring_free.push_back(l.ring);
         progress = true;
@@ -406,7 +431,8 @@ void sched_loop(bbtc_server* S) {
                      cand.end());
         }
         std::vector<uint64_t> take(cand.size(), 0);
-        uint64_t budget = S->sequential ? ~0ull : S->quantum;
+        l.kind = !cand.empty() && cand[0]->plan.lane_only ? 1 : 0;
+        uint64_t budget = S->sequential ? ~0ull : S->quantum[l.kind];
         if (!S->sequential && !cand.empty()) {
           take[0] = std::min(cand[0]->nitems - cand[0]->next, budget / 2);
           budget -= take[0];
@@ -451,14 +477,18 @@ void sched_loop(bbtc_server* S) {
           a.spill = S->spill;
           a.spill_base = S->spill_base;
           a.spill_cap = kSpillCap;
-          enqueue_count(S->ctx, a, 0, total, lane_only, kListChunkItems, s);
-          if (S->event_pool.empty()) {
+          while (S->event_pool.size() < 2) {
             hipEvent_t e;
-            HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIP_OK(hipEventCreate(&e));
             S->event_pool.push_back(e);
           }
+          l.start = S->event_pool.back();
+          S->event_pool.pop_back();
           l.done = S->event_pool.back();
           S->event_pool.pop_back();
+          l.items = total;
+          HIP_OK(hipEventRecord(l.start, s));
+          enqueue_count(S->ctx, a, 0, total, lane_only, kListChunkItems, s);
           HIP_OK(hipEventRecord(l.done, s));
           inflight.push_back(std::move(l));
           std::lock_guard<std::mutex> lk(S->mu);
@@ -497,6 +527,7 @@ void server_free(bbtc_server* S) {
   if (S->heap_base) {
     const hipError_t e = hipFree(S->heap_base);
     if (e != hipSuccess) std::fprintf(stderr, "bbtc: freeing the server heap failed: %s\n", hipGetErrorString(e));
+    (void)hipDeviceSynchronize();
   }
   if (S->ctl) (void)hipFree(S->ctl);
   if (S->spill) (void)hipFree(S->spill);
@@ -547,16 +578,32 @@ bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc
     if (!heap) bbtc::fail(BBTC_ERR_INVALID, "heap size is zero");
     if (ctx->budget && ctx->in_use + heap > ctx->budget)
       bbtc::fail(BBTC_ERR_BUDGET, "server heap exceeds the context's device budget");
-    if (heap + (1ull << 30) > free_b)
-      bbtc::fail(BBTC_ERR_BUDGET, "server heap of " + std::to_string(heap) + " bytes exceeds free device memory (" +
-                                      std::to_string(free_b) + " of " + std::to_string(total_b) + " bytes free)");
-    S->quantum = opts && opts->quantum_items ? opts->quantum_items : bbtc::kDefaultQuantum;
+    if (heap > total_b)
+      bbtc::fail(BBTC_ERR_BUDGET, "server heap of " + std::to_string(heap) + " bytes exceeds device memory (" +
+                                      std::to_string(total_b) + " bytes)");
+    S->quantum[0] = S->quantum[1] = opts && opts->quantum_items ? opts->quantum_items : bbtc::kDefaultQuantum;
+    S->adaptive = !(opts && opts->quantum_items);
+    S->launch_ms = opts && opts->launch_us ? opts->launch_us * 1e-3 : bbtc::kDefaultLaunchMs;
     S->max_inflight = opts && opts->max_inflight ? opts->max_inflight : bbtc::kDefaultInflight;
     S->max_segments = opts && opts->max_segments ? opts->max_segments : bbtc::kDefaultSegments;
     S->sequential = opts && opts->sequential;
     S->no_lane_form = opts && opts->no_lane_form;
     if (S->sequential) S->max_inflight = 1;
-    HIP_OK(hipMalloc(&S->heap_base, heap));
+    hipError_t he = hipMalloc(&S->heap_base, heap);
+    if (he != hipSuccess) {
+      // memory freed just before (another server's heap) can become
+      // available only once the device has gone idle
+      (void)hipGetLastError();
+      (void)hipDeviceSynchronize();
+      he = hipMalloc(&S->heap_base, heap);
+    }
+    if (he != hipSuccess) {
+      (void)hipGetLastError();
+      S->heap_base = nullptr;
+      bbtc::fail(BBTC_ERR_BUDGET, "server heap of " + std::to_string(heap) + " bytes could not be allocated (" +
+                                      std::to_string(free_b) + " of " + std::to_string(total_b) +
+                                      " bytes free before the call)");
+    }
     S->heap.init(heap);
     ctx->in_use += heap;
     HIP_OK(hipMalloc(&S->ctl, sizeof(bbtc::SlotCtl)));
@@ -706,8 +753,9 @@ void bbtc_server_destroy(bbtc_server* S) {
   if (S->admit_thread.joinable()) S->admit_thread.join();
   if (S->sched_thread.joinable()) S->sched_thread.join();
   (void)hipSetDevice(S->ctx->device);
-  (void)hipStreamSynchronize(S->compute);
-  (void)hipStreamSynchronize(S->copy);
+  // every stream that touched the heap (compute, copy, read-back) must be
+  // idle, or the runtime defers releasing the heap past this call
+  (void)hipDeviceSynchronize();
   bbtc::server_free(S);
 }
 

@@ -385,13 +385,17 @@ typedef struct bbtc_server bbtc_server;
 
 typedef struct bbtc_server_opts {
   uint64_t heap_bytes;     /* device heap for resident requests (0: 200 GiB) */
-  uint64_t quantum_items;  /* work items per launch (0: 2^18) */
+  uint64_t quantum_items;  /* work items per launch (0: adaptive, see launch_us) */
   uint32_t max_inflight;   /* launches queued on the device at once (0: 2) */
   uint32_t max_segments;   /* requests per launch (0: 1024) */
   uint32_t sequential;     /* 1: one request at a time, whole plan per launch */
   uint32_t no_lane_form;   /* 1: BBTC_KERNEL_AUTO requests never take the lane form, so any
                               requests share a launch (0: requests whose tasks all take it run
                               in launches of their own, taking turns with the others) */
+  uint32_t launch_us;      /* with quantum_items 0: the item budget starts at 2^18 and follows
+                              the measured rate of finished launches so that a launch takes
+                              about this long (0: 10000 us) */
+  uint32_t reserved;
 } bbtc_server_opts;
 
 typedef struct bbtc_server_result {
