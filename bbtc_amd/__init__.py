@@ -51,8 +51,8 @@ class bbtc_opts(C.Structure):
 
 class bbtc_server_opts(C.Structure):
     _fields_ = [("heap_bytes", C.c_uint64), ("quantum_items", C.c_uint64), ("max_inflight", C.c_uint32),
-                ("max_segments", C.c_uint32), ("sequential", C.c_uint32), ("no_lane_form", C.c_uint32),
-                ("launch_us", C.c_uint32), ("reserved", C.c_uint32)]
+                ("max_segments", C.c_uint32), ("sequential", C.c_uint32), ("lane_form", C.c_uint32),
+                ("launch_us", C.c_uint32), ("oldest_pct", C.c_uint32)]
 
 
 class bbtc_server_result(C.Structure):
@@ -635,10 +635,10 @@ class Server:
     """Continuous-batching request loop on one device (bbtc_server_*)."""
 
     def __init__(self, dev: "Device", heap_bytes: int = 0, quantum_items: int = 0, max_inflight: int = 0,
-                 max_segments: int = 0, sequential: bool = False, no_lane_form: bool = False,
-                 launch_us: int = 0):
+                 max_segments: int = 0, sequential: bool = False, lane_form: bool = False,
+                 launch_us: int = 0, oldest_pct: int = 0):
         o = bbtc_server_opts(heap_bytes, quantum_items, max_inflight, max_segments, int(sequential),
-                             int(no_lane_form), launch_us, 0)
+                             int(lane_form), launch_us, oldest_pct)
         h = C.c_void_p()
         _check(lib().bbtc_server_create(dev._h, C.byref(o), C.byref(h)))
         self._h = h

@@ -9,20 +9,22 @@
 // on a copy stream while the device counts earlier requests.
 //
 // The scheduler thread keeps up to `max_inflight` launches queued on the
-// compute stream. Each launch counts a budget of work items of one kind of
-// request: those whose tasks all use the lane form (the lane kernel) or the
-// others (count_probe), the kinds taking turns while both have work. The
-// budget is `quantum_items`, or by default follows each kind's measured item
-// rate so that a launch takes about `launch_us` (a launch's fixed costs stay
-// small next to its work, and an arrival waits at most about two launches).
-// Within the kind, the oldest resident request with work left
-// keeps half of the budget, and the rest is filled with slices of the
-// resident requests in order of remaining work (shortest first). A request that
-// arrives while others run joins the next launch and shares the machine with
-// them; a large request is slowed at most about twofold by later arrivals. A
-// request finishes with the launch that holds its last slice. Its counters
-// are read back and its heap block returns to the heap, which lets the next
-// arrival in.
+// compute stream. By default a launch's budget is estimated work: each
+// request's items carry the reference's task weight (tasks.hpp:21-23) spread
+// evenly over the task's items, and the budget follows the measured work
+// rate of finished launches so that a launch takes about `launch_us` (a
+// launch's fixed costs stay small next to its work, and an arrival waits at
+// most about two launches); with `quantum_items` set, the budget is that
+// many items. The oldest resident request with work left keeps `oldest_pct`
+// percent of the budget, and the rest is filled with slices of the resident
+// requests in order of remaining work (shortest first). A request that
+// arrives while others run joins the next launch and shares the machine
+// with them; a large request is slowed by later arrivals by at most about
+// 100 / oldest_pct. With `lane_form`, requests whose tasks all take the lane
+// form run in launches of their own (the lane kernel), the two kinds taking
+// turns while both have work. A request finishes with the launch that holds
+// its last slice. Its counters are read back and its heap block returns to
+// the heap, which lets the next arrival in.
 //
 // This replaces the reference's per-run task loop (bb_tc.cpp:142-181,
 // 200-252), which counts one graph per process.
@@ -30,6 +32,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <condition_variable>
 #include <deque>
@@ -55,6 +58,7 @@ constexpr uint64_t kDefaultQuantum = 1ull << 18;
 constexpr uint64_t kMinQuantum = 1ull << 14;
 constexpr uint64_t kMaxQuantum = 1ull << 24;
 constexpr double kDefaultLaunchMs = 10.0;
+constexpr uint32_t kDefaultOldestPct = 75;
 constexpr uint32_t kDefaultInflight = 2;
 constexpr uint32_t kDefaultSegments = 1024;
 
@@ -119,15 +123,41 @@ struct Request {
   hipEvent_t uploaded = nullptr;
   SegDesc seg{};
   uint64_t next = 0;  // next item to put in a launch
+  // estimated work of items [0, x): wpref[t] before task t (the reference's
+  // task weight, tasks.hpp:21-23, spread evenly over the task's items)
+  std::vector<double> wpref;
   uint32_t launches = 0;
   double admit_s = 0, ready_s = 0, start_s = -1, done_s = 0;
   bbtc_status status = BBTC_OK;
   uint64_t total = 0;
 };
 
+// Estimated work of items [0, x) of r.
+double work_at(const Request& r, uint64_t x) {
+  if (x >= r.nitems) return r.wpref.back();
+  const std::vector<uint64_t>& p = r.plan.pref;
+  const size_t t = size_t(std::upper_bound(p.begin(), p.end(), x) - p.begin()) - 1;
+  const uint64_t n = p[t + 1] - p[t];
+  return r.wpref[t] + (r.wpref[t + 1] - r.wpref[t]) * double(x - p[t]) / double(n);
+}
+
+// Items of r from `from` that hold about `work` (at least one item).
+uint64_t items_for_work(const Request& r, uint64_t from, double work) {
+  const double target = work_at(r, from) + work;
+  if (target >= r.wpref.back()) return r.nitems - from;
+  const std::vector<uint64_t>& p = r.plan.pref;
+  // task t with wpref[t] <= target < wpref[t + 1] (then it has items)
+  const size_t t = size_t(std::upper_bound(r.wpref.begin(), r.wpref.end(), target) - r.wpref.begin()) - 1;
+  const double per = (r.wpref[t + 1] - r.wpref[t]) / double(p[t + 1] - p[t]);
+  uint64_t x = p[t] + uint64_t(std::ceil((target - r.wpref[t]) / per));
+  x = std::min(std::max(x, from + 1), r.nitems);
+  return x - from;
+}
+
 struct Launch {
   hipEvent_t start = nullptr, done = nullptr;  // around the launch on the compute stream
   uint64_t items = 0;
+  double work = 0;  // estimated work of its items
   int kind = 0;  // 1: lane-form requests only
   int ring = 0;
   std::vector<std::shared_ptr<Request>> finishing;
@@ -139,12 +169,14 @@ struct Launch {
 struct bbtc_server {
   bbtc_ctx* ctx = nullptr;
   uint64_t quantum[2] = {0, 0};  // item budget of the next launch of each kind (lane form: [1])
-  bool adaptive = false;         // the budgets follow the measured rates
+  bool adaptive = false;         // budgets in estimated work, following the measured rates
+  double work_budget[2] = {0, 0};  // adaptive: work per launch of each kind (0: not yet measured)
   double launch_ms = 0;      // target launch time when adaptive
   uint32_t max_inflight = 0, max_segments = 0;
   bool sequential = false;
   bool lane_turn = false;  // kind of the next launch when both kinds have work
-  bool no_lane_form = false;
+  bool lane_form = false;  // AUTO requests may take the lane form
+  uint32_t oldest_pct = 75;  // share of each launch's budget kept by the oldest request
 
   char* heap_base = nullptr;
   bbtc::Heap heap;
@@ -385,16 +417,14 @@ void sched_loop(bbtc_server* S) {
           act.erase(std::find(act.begin(), act.end(), r));
         }
         if (S->adaptive) {
-          // next budget: the items this launch would have counted in the
-          // target time at its measured rate, at most doubling or halving
+          // next budget: the work this launch would have done in the target
+          // time at its measured rate, at most doubling or halving
           float ms = 0.f;
           HIP_OK(hipEventElapsedTime(&ms, l.start, l.done));
-          if (ms > 0.01f && l.items) {
-            const double want = double(l.items) * S->launch_ms / double(ms);
-            uint64_t& q = S->quantum[l.kind];
-            const double lo = std::max(double(kMinQuantum), 0.5 * double(q));
-            const double hi = std::min(double(kMaxQuantum), 2.0 * double(q));
-            q = uint64_t(std::min(hi, std::max(lo, want)));
+          if (ms > 0.01f && l.work > 0) {
+            const double want = l.work * S->launch_ms / double(ms);
+            double& w = S->work_budget[l.kind];
+            w = w > 0 ? std::min(2.0 * w, std::max(0.5 * w, want)) : want;
           }
         }
         S->event_pool.push_back(l.start);
@@ -412,9 +442,9 @@ void sched_loop(bbtc_server* S) {
         Launch l;
         l.ring = ring;
         const double t = S->now();
-        // The oldest request with work left keeps half of the launch's item
-        // budget (so a large request is slowed at most about twofold by the
-        // arrivals behind it); the rest goes shortest remaining work first.
+        // The oldest request with work left keeps oldest_pct percent of the
+        // launch's item budget (which bounds how much the arrivals behind it
+        // slow a large request); the rest goes shortest remaining work first.
         std::vector<Request*> cand;  // admission order
         for (auto& r : act)
           if (r->next < r->nitems) cand.push_back(r.get());
@@ -432,20 +462,45 @@ void sched_loop(bbtc_server* S) {
         }
         std::vector<uint64_t> take(cand.size(), 0);
         l.kind = !cand.empty() && cand[0]->plan.lane_only ? 1 : 0;
-        uint64_t budget = S->sequential ? ~0ull : S->quantum[l.kind];
-        if (!S->sequential && !cand.empty()) {
-          take[0] = std::min(cand[0]->nitems - cand[0]->next, budget / 2);
-          budget -= take[0];
-        }
         std::vector<size_t> by_rem(cand.size());
         for (size_t x = 0; x < cand.size(); ++x) by_rem[x] = x;
-        std::stable_sort(by_rem.begin(), by_rem.end(), [&](size_t x, size_t y) {
-          return cand[x]->nitems - cand[x]->next < cand[y]->nitems - cand[y]->next;
-        });
-        for (size_t x : by_rem) {
-          const uint64_t more = std::min(cand[x]->nitems - cand[x]->next - take[x], budget);
-          take[x] += more;
-          budget -= more;
+        if (S->adaptive && !S->sequential && !cand.empty()) {
+          // budget in estimated work; the first launch of a kind takes the
+          // work of kDefaultQuantum items of its oldest request
+          Request* o = cand[0];
+          double& wb = S->work_budget[l.kind];
+          double budget = wb > 0 ? wb
+                                 : work_at(*o, std::min(o->nitems, o->next + kDefaultQuantum)) - work_at(*o, o->next);
+          std::vector<double> rem(cand.size());
+          for (size_t x = 0; x < cand.size(); ++x) rem[x] = cand[x]->wpref.back() - work_at(*cand[x], cand[x]->next);
+          const double mine = std::min(rem[0], budget * S->oldest_pct / 100.0);
+          take[0] = items_for_work(*o, o->next, mine);
+          budget -= mine;
+          std::stable_sort(by_rem.begin(), by_rem.end(), [&](size_t x, size_t y) { return rem[x] < rem[y]; });
+          for (size_t x : by_rem) {
+            if (budget <= 0) break;
+            Request* r = cand[x];
+            const uint64_t from = r->next + take[x];
+            if (from >= r->nitems) continue;
+            const double left = r->wpref.back() - work_at(*r, from);
+            const double w = std::min(left, budget);
+            take[x] += items_for_work(*r, from, w);
+            budget -= w;
+          }
+        } else {
+          uint64_t budget = S->sequential ? ~0ull : S->quantum[l.kind];
+          if (!S->sequential && !cand.empty()) {
+            take[0] = std::min(cand[0]->nitems - cand[0]->next, budget / 100 * S->oldest_pct);
+            budget -= take[0];
+          }
+          std::stable_sort(by_rem.begin(), by_rem.end(), [&](size_t x, size_t y) {
+            return cand[x]->nitems - cand[x]->next < cand[y]->nitems - cand[y]->next;
+          });
+          for (size_t x : by_rem) {
+            const uint64_t more = std::min(cand[x]->nitems - cand[x]->next - take[x], budget);
+            take[x] += more;
+            budget -= more;
+          }
         }
         for (size_t x : by_rem) {
           Request* r = cand[x];
@@ -456,6 +511,7 @@ void sched_loop(bbtc_server* S) {
           d.item1 = r->next + take[x];
           d.launch0 = total;
           sh[nseg++] = d;
+          l.work += work_at(*r, r->next + take[x]) - work_at(*r, r->next);
           total += take[x];
           lane_only = lane_only && r->plan.lane_only;
           r->next += take[x];
@@ -587,7 +643,9 @@ bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc
     S->max_inflight = opts && opts->max_inflight ? opts->max_inflight : bbtc::kDefaultInflight;
     S->max_segments = opts && opts->max_segments ? opts->max_segments : bbtc::kDefaultSegments;
     S->sequential = opts && opts->sequential;
-    S->no_lane_form = opts && opts->no_lane_form;
+    S->lane_form = opts && opts->lane_form;
+    if (opts && opts->oldest_pct > 100) bbtc::fail(BBTC_ERR_INVALID, "oldest_pct above 100");
+    S->oldest_pct = opts && opts->oldest_pct ? opts->oldest_pct : bbtc::kDefaultOldestPct;
     if (S->sequential) S->max_inflight = 1;
     hipError_t he = hipMalloc(&S->heap_base, heap);
     if (he != hipSuccess) {
@@ -669,9 +727,20 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
   r->L = &L->L;
   r->tiles = bbtc::tile_table(L->L);
   r->plan = bbtc::describe_plan(r->tiles, L->L.deg, bbtc::refs_of(P, tasks, ntasks), kernel, nullptr,
-                                !S->no_lane_form);
+                                S->lane_form);
   r->ntasks = ntasks;
   r->nitems = r->plan.pref[ntasks];
+  r->wpref.assign(ntasks + 1, 0.0);
+  for (uint64_t q = 0; q < ntasks; ++q) {
+    const uint64_t items = r->plan.pref[q + 1] - r->plan.pref[q];
+    double w = 0;
+    if (items) {  // nnz(ab) * max(deg(bc), deg(ac)), as the reference sorts tasks
+      const bbtc::TaskDesc& d = r->plan.td[q];
+      w = double(L->L.nnz[d.ab]) * std::max(double(L->L.deg[d.bc]), double(L->L.deg[d.ac]));
+      if (!(w > 0) || !std::isfinite(w)) w = double(items);
+    }
+    r->wpref[q + 1] = r->wpref[q] + w;
+  }
   r->per_task_out = per_task_out;
   r->arrival = arrival_s;
   r->bytes = bbtc::place(*r);

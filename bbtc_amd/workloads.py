@@ -244,7 +244,7 @@ def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200
 
 
 def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, quantum_items: int = 0,
-                no_lane_form: bool = False):
+                lane_form: bool = False):
     """Serving workload: the plain per-graph launch and the server, first
     saturated (every request at time 0: capacity), then at Poisson arrivals of
     `rate` requests/s (default: `load` times the plain capacity). Per-task
@@ -260,7 +260,7 @@ def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, qua
     p_cap = n / p_make
     log(f"plain saturated: {p_cap:.2f} graphs/s ({p_make:.2f}s for {n})")
     # one server (one heap reservation) for both server runs
-    srv = Server(dev, heap_bytes=200 << 30, quantum_items=quantum_items, no_lane_form=no_lane_form)
+    srv = Server(dev, heap_bytes=200 << 30, quantum_items=quantum_items, lane_form=lane_form)
     try:
         return _serving_runs(dev, layouts, tasks, srv, n, p_tot, p_per, p_cap, rate, load, log)
     finally:

@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the G(n=2^22, m=2^26) count")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="only the G(n=2^22, m=2^26) count (--steps launches); prints its object as the line")
     ap.add_argument("--batched-graphs", type=int, default=4096, help="0: skip the batched workload")
     ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
     args = ap.parse_args()
@@ -191,6 +193,9 @@ def main():
 
     offset_bytes = 8 if args.scale >= 26 else 4
     dev = B.Device(0)
+    if args.c2_only:
+        print(json.dumps(count_c2(B, dev, steps=args.steps)), flush=True)
+        return
     read_gbs = dev.read_bandwidth(8 << 30, 5)
     stream_gbs = float(np.median(read_gbs))
     log(f"streaming read of 8 GiB: {np.round(read_gbs, 1).tolist()} GB/s (median {stream_gbs:.1f})")

@@ -389,13 +389,16 @@ typedef struct bbtc_server_opts {
   uint32_t max_inflight;   /* launches queued on the device at once (0: 2) */
   uint32_t max_segments;   /* requests per launch (0: 1024) */
   uint32_t sequential;     /* 1: one request at a time, whole plan per launch */
-  uint32_t no_lane_form;   /* 1: BBTC_KERNEL_AUTO requests never take the lane form, so any
-                              requests share a launch (0: requests whose tasks all take it run
-                              in launches of their own, taking turns with the others) */
-  uint32_t launch_us;      /* with quantum_items 0: the item budget starts at 2^18 and follows
-                              the measured rate of finished launches so that a launch takes
-                              about this long (0: 10000 us) */
-  uint32_t reserved;
+  uint32_t lane_form;      /* 1: BBTC_KERNEL_AUTO requests whose tasks all admit the lane form
+                              take it and run in launches of their own (the lane kernel),
+                              taking turns with the others; 0: they keep the per-task forms,
+                              so any requests share a launch */
+  uint32_t launch_us;      /* with quantum_items 0: launches are composed by estimated work
+                              (the reference's task weight spread over a task's items), and
+                              the work budget follows the measured rate of finished launches
+                              so that a launch takes about this long (0: 10000 us) */
+  uint32_t oldest_pct;     /* percent of each launch's budget the oldest request with work
+                              keeps (0: 75); the rest goes shortest remaining work first */
 } bbtc_server_opts;
 
 typedef struct bbtc_server_result {

@@ -359,6 +359,17 @@ def test_server_alone_vs_mix(dev):
         assert np.array_equal(got[i].per_task, alone[x]), x
         assert got[i].total == int(alone[x].sum())
     assert max(r.launches for r in got.values()) > 1
+    # launches composed by estimated work with a small target time, with and
+    # without lane-form requests in launches of their own
+    for lane_form in (False, True):
+        srv = B.Server(dev, heap_bytes=1 << 30, launch_us=200, lane_form=lane_form, oldest_pct=60)
+        ids = [srv.submit(L, t, 0.0) for L, t, _ in reqs]
+        got = {r.id: r for r in srv.drain()}
+        st = srv.stats()
+        srv.destroy()
+        log("work-budget mix stats", lane_form, st)
+        for x, i in enumerate(ids):
+            assert got[i].status == 0 and np.array_equal(got[i].per_task, alone[x]), (lane_form, x)
     # plain mode (one request at a time, whole plan per launch) agrees too
     srv = B.Server(dev, heap_bytes=1 << 30, sequential=True)
     ids = [srv.submit(L, t, 0.0) for L, t, _ in reqs]
