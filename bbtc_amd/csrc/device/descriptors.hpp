@@ -13,7 +13,8 @@ constexpr int kLdsWords = 1792;  // bitmap / sorted-list slice of a wave
 constexpr int kMarkWords = 4 * 64;  // per-step wedge marks of a wave
 constexpr int kWaveLds = kLdsWords + kMarkWords;  // 8 KiB of LDS per wave
 constexpr int kMaxSlots = 64;    // concurrent range launches of one plan
-constexpr int kItemBatch = 8;    // work items a wave claims per atomic
+constexpr int kItemBatch = 8;    // work items a wave claims per atomic (even: the lane
+                                 // kernel claims half as many for one graph)
 // First-phase items are dealt to kQueues parts in chunks of consecutive
 // items (chunk c to part c % kQueues); each part has its own head counter in
 // its own 128-byte line and is claimed in its own order. A block starts on
@@ -78,13 +79,15 @@ struct SlotCtl {
   // position x of part q is item base + ((x / C) * kQueues + q) * C + x % C
   // with C = kChunkItems
   unsigned long long head[kQueues * kCtlStride];
+  // second phase: the spilled chunks are split into kQueues contiguous
+  // parts too; next chunk of part q (counted from the part's start)
+  unsigned long long head2[kQueues * kCtlStride];
   unsigned long long end[(kQueues + kCtlStride - 1) / kCtlStride * kCtlStride];
   unsigned long long base;   // first item of the launch
   unsigned long long chunk;  // items per chunk (a multiple of kItemBatch)
   unsigned long long shift;  // log2(chunk), or 63 with one chunk per part
   unsigned long long spill;  // (records << kSpillShift) | chunks reserved
-  unsigned long long head2;  // next second-phase chunk
-  unsigned long long pad[kCtlStride - 5];
+  unsigned long long pad[kCtlStride - 4];
 };
 static_assert(sizeof(SlotCtl) % 128 == 0, "SlotCtl layout");
 static_assert(kListChunkItems % kItemBatch == 0, "claims must not straddle chunks");
