@@ -33,8 +33,14 @@ constexpr int kCtlStride = 16;   // 64-bit words per counter line
 // kDeferSteps wave steps (one step: 64 probes, one 64-wedge batch, or one
 // 16-byte bitmap word per lane) hands its remaining wedges to the second phase
 // in chunks of about kChunkSteps steps each; a launch slot holds at most
-// kSpillCap such records.
+// kSpillCap such records. A launch over one graph with at least
+// kManyItemsPerWave items per resident wave splits at kDeferStepsMany steps
+// instead: the machine is kept busy by other items, so fewer splits pay
+// (RMAT s26: 614 vs 646 ms), while batches of small graphs and the server
+// split early, where a heavy item is a graph's or request's critical path.
 constexpr uint32_t kDeferSteps = 512;
+constexpr uint32_t kDeferStepsMany = 1024;
+constexpr uint64_t kManyItemsPerWave = 64;
 constexpr uint32_t kChunkSteps = 256;
 constexpr uint32_t kSpillCap = 1u << 20;
 constexpr int kSpillShift = 36;  // SlotCtl::spill = (records << 36) | chunks
@@ -164,7 +170,7 @@ struct CountArgs {
   SpillRec* spill;            // spill_cap records (first phase writes, second reads)
   uint64_t* spill_base;       // first chunk of each record (increasing)
   uint32_t spill_cap;         // 0: no splitting
-  uint32_t pad;
+  uint32_t defer_steps;       // wave steps after which a first-phase unit spills
   // Per-graph completion (batches; NULL: off): units still to finish, and
   // the device wall clock when a graph's last unit finished.
   unsigned long long* left;

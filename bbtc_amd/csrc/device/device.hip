@@ -68,6 +68,9 @@ void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t i
   hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, i0, i0 + items, (unsigned long long)chunk);
   HIP_OK(hipGetLastError());
   a.phase = 0;
+  a.defer_steps = chunk == 0 && items >= kManyItemsPerWave * resident_blocks(ctx) * kWavesPerBlock
+                      ? kDeferStepsMany
+                      : kDeferSteps;
   const uint64_t want = (items + kWavesPerBlock - 1) / kWavesPerBlock;
   const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(ctx, lane_only))));
   if (lane_only) {

@@ -492,7 +492,7 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
         // batch seen so far.
         // first wedge not yet counted (a window may end inside this batch)
         const uint32_t wn = wb + kWave < w_to ? wb + kWave : w_to;
-        if (may_defer && steps > kDeferSteps && wn < hi) {
+        if (may_defer && steps > a.defer_steps && wn < hi) {
           may_defer = false;
           const uint32_t per = steps / iters;
           uint32_t k = kChunkSteps / (per ? per : 1);
