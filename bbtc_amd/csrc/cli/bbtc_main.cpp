@@ -15,10 +15,15 @@
 //     "HIP Initial Memory Allocations and Stream Creation";
 //   * extra flags: --kernel auto|merge|bitmap|dense|rdense|lane|tuned, --threads n, --cuts-file path,
 //     --per-task-out path, --keep-duplicates, --save-bcsr path (a .bcsr
-//     --graph is read back with its own cuts).
+//     --graph is read back with its own cuts);
+//   * extra modes (the reference counts one graph per run): --batch <list>
+//     counts every graph a list file names in one batched launch, and
+//     --serve <list> runs them as requests (with arrival times) through the
+//     continuous-batching server.
 #include <omp.h>
 #include <strings.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -49,6 +54,8 @@ struct Params {
   std::string per_task_out;
   bool keep_duplicates = false;
   std::string save_bcsr;
+  bool ncut_given = false;
+  std::string batch_list, serve_list;  // list files of the extra modes
 };
 
 void usage() {
@@ -79,6 +86,11 @@ void usage() {
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
             << "  --keep-duplicates     Keep repeated edges when converting text input\n"
             << "  --save-bcsr <path>    Write the block layout (.bcsr) for later runs\n"
+            << "  --batch <list>        Count every graph listed ('path [ncut]' per line) in one\n"
+            << "                          batched launch; without an ncut column or --ncut the\n"
+            << "                          cut count per graph comes from measured sample graphs\n"
+            << "  --serve <list>        Serve the listed graphs ('path [arrival_s]' per line) as\n"
+            << "                          requests through the continuous-batching server\n"
             << "\nExamples:\n"
             << "  bbtc --graph email-EuAll_adj.mtx --partitioner n1d --ncut 4\n"
             << "  bbtc --graph email-EuAll_adj.txt --ncut 4 --copy-first --border 4\n";
@@ -144,6 +156,7 @@ Params parse(int argc, char** argv) {
     else if (!strcasecmp(a, "--ncut")) {
       p.ncut = to_int(val("--ncut"), "--ncut");
       check_int("--ncut", p.ncut, 1);
+      p.ncut_given = true;
     } else if (!strcasecmp(a, "--partitioner")) {
       p.part = val("--partitioner");
       if (p.part != "pal" && p.part != "n1d" && p.part != "uni" && p.part != "rac")
@@ -160,10 +173,19 @@ Params parse(int argc, char** argv) {
     else if (!strcasecmp(a, "--per-task-out")) p.per_task_out = val("--per-task-out");
     else if (!strcasecmp(a, "--keep-duplicates")) p.keep_duplicates = true;
     else if (!strcasecmp(a, "--save-bcsr")) p.save_bcsr = val("--save-bcsr");
+    else if (!strcasecmp(a, "--batch")) p.batch_list = val("--batch");
+    else if (!strcasecmp(a, "--serve")) p.serve_list = val("--serve");
     else if (!strcasecmp(a, "--help")) {
       usage();
       std::exit(0);
     } else bad("Wrong argument '" + std::string(a) + "' given.");
+  }
+  if (!p.batch_list.empty() && !p.serve_list.empty()) bad("--batch and --serve are exclusive");
+  const std::string& list = !p.batch_list.empty() ? p.batch_list : p.serve_list;
+  if (!list.empty()) {
+    if (!file_exists(list)) bad("List file doesn't exist");
+    if (p.ngpu == 0) bad("--batch and --serve need a GPU (--ngpu 1)");
+    return p;
   }
   if (!file_exists(p.file)) bad("Graph doesn't exist");
   return p;
@@ -193,10 +215,253 @@ bool ends_with(const std::string& s, const std::string& t) {
   return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
 }
 
+bbtc_part part_kind(const std::string& s) {
+  return s == "uni" ? BBTC_PART_UNI : s == "n1d" ? BBTC_PART_N1D : s == "rac" ? BBTC_PART_RAC : BBTC_PART_PAL;
+}
+
+bbtc_kernel kernel_kind(const std::string& s) {
+  return s == "merge"    ? BBTC_KERNEL_MERGE
+         : s == "bitmap" ? BBTC_KERNEL_BITMAP
+         : s == "dense"  ? BBTC_KERNEL_DENSE
+         : s == "rdense" ? BBTC_KERNEL_RDENSE
+         : s == "lane"   ? BBTC_KERNEL_LANE
+                         : BBTC_KERNEL_AUTO;
+}
+
+// ---- --batch / --serve -------------------------------------------------------
+
+struct ListEntry {
+  std::string path;
+  double value = 0;  // ncut (--batch) or arrival seconds (--serve)
+  bool has_value = false;
+};
+
+// 'path [number]' per line; blank lines and lines starting with '#' skipped
+std::vector<ListEntry> read_list(const std::string& file) {
+  std::ifstream in(file);
+  std::vector<ListEntry> out;
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    ListEntry e;
+    if (!(ls >> e.path) || e.path[0] == '#') continue;
+    if (ls >> e.value) e.has_value = true;
+    out.push_back(e);
+  }
+  if (out.empty()) {
+    std::cerr << "bbtc: " << file << " names no graph" << std::endl;
+    std::exit(1);
+  }
+  return out;
+}
+
+// A .bbtc file (or its sibling <path>.bbtc), else text converted in memory
+// (duplicate edges dropped).
+bbtc_hgraph* load_listed(const std::string& path) {
+  bbtc_hgraph* g = nullptr;
+  bbtc_status st;
+  if (ends_with(path, ".bcsr")) {
+    std::cerr << "bbtc: " << path << ": block layouts (.bcsr) are not accepted in a list" << std::endl;
+    std::exit(1);
+  }
+  if (ends_with(path, ".bbtc")) st = bbtc_hgraph_load(path.c_str(), &g);
+  else if (file_exists(path + ".bbtc")) st = bbtc_hgraph_load((path + ".bbtc").c_str(), &g);
+  else st = bbtc_hgraph_load_text(path.c_str(), &g);
+  if (st) die(st, "reading " + path);
+  return g;
+}
+
+uint64_t edges_of(const bbtc_hgraph* g) {
+  uint32_t n = 0, nc = 0;
+  uint64_t m = 0;
+  bbtc_hgraph_info(g, &n, &nc, &m);
+  return m;
+}
+
+// Cut count per graph: the list's column (use_column), else --ncut, else a
+// rule from measured device work on sample graphs spread over the sizes.
+std::vector<uint32_t> cut_counts(bbtc_ctx* ctx, const std::vector<bbtc_hgraph*>& g,
+                                 const std::vector<ListEntry>& e, const Params& P, bool use_column) {
+  const size_t n = g.size();
+  std::vector<uint32_t> ncut(n, uint32_t(P.ncut));
+  bool need_rule = false;
+  for (size_t x = 0; x < n; ++x) {
+    if (use_column && e[x].has_value) {
+      if (e[x].value < 1 || e[x].value > 65535) {
+        std::cerr << "bbtc: bad cut count for " << e[x].path << std::endl;
+        std::exit(1);
+      }
+      ncut[x] = uint32_t(e[x].value);
+    } else if (!P.ncut_given) {
+      need_rule = true;
+    }
+  }
+  if (!need_rule) return ncut;
+  std::vector<size_t> by_m(n);
+  for (size_t x = 0; x < n; ++x) by_m[x] = x;
+  std::stable_sort(by_m.begin(), by_m.end(), [&](size_t a, size_t b) { return edges_of(g[a]) < edges_of(g[b]); });
+  std::vector<const bbtc_hgraph*> sample;
+  const size_t step = std::max<size_t>(1, n / 64);
+  for (size_t x = 0; x < n; x += step) sample.push_back(g[by_m[x]]);
+  const uint32_t cands[] = {1, 2, 4, 8};
+  uint32_t rule[64];
+  const bbtc_status st = bbtc_tune_ncut(ctx, sample.data(), sample.size(), cands, 4, part_kind(P.part), 0, rule,
+                                        nullptr);
+  if (st) die(st, "choosing cut counts");
+  for (size_t x = 0; x < n; ++x)
+    if (!(use_column && e[x].has_value)) ncut[x] = bbtc_rule_ncut(rule, edges_of(g[x]));
+  return ncut;
+}
+
+double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  const double pos = q / 100.0 * double(v.size() - 1);
+  const size_t lo = size_t(pos), hi = std::min(v.size() - 1, lo + 1);
+  return v[lo] + (v[hi] - v[lo]) * (pos - double(lo));
+}
+
+int run_batch(const Params& P) {
+  const std::vector<ListEntry> e = read_list(P.batch_list);
+  Fields F;
+  bbtc_ctx* ctx = nullptr;
+  F.begin("HIP Initial Memory Allocations and Stream Creation");
+  bbtc_status st = bbtc_ctx_create(0, nullptr, &ctx);
+  if (st) die(st, "creating the device context");
+  F.end();
+  std::vector<bbtc_hgraph*> g;
+  for (auto& x : e) g.push_back(load_listed(x.path));
+  F.begin("Partitioning and Layout construction (s)");
+  const std::vector<uint32_t> ncut = cut_counts(ctx, g, e, P, true);
+  std::vector<bbtc_layout*> L(g.size(), nullptr);
+  st = bbtc_layouts_build(g.data(), g.size(), ncut.data(), part_kind(P.part), L.data());
+  if (st) die(st, "building the block layouts");
+  F.end();
+  F.begin("Batch upload and plan (s)");
+  bbtc_batch* b = nullptr;
+  st = bbtc_batch_create(ctx, L.data(), L.size(), P.kernel == "tuned" ? BBTC_KERNEL_AUTO : kernel_kind(P.kernel),
+                         BBTC_BATCH_SMALL_GRAPHS_FIRST, &b);
+  if (st) die(st, "creating the batch");
+  if (P.kernel == "tuned" && (st = bbtc_batch_autotune(b, nullptr))) die(st, "autotuning the batch");
+  if ((st = bbtc_batch_capture(b))) die(st, "capturing the batch launch");
+  F.end();
+  uint64_t m = 0;
+  for (auto* x : g) m += edges_of(x);
+  std::cout << "=====================================" << std::endl;
+  std::cout << "Graph list: " << P.batch_list << std::endl;
+  std::cout << "Number of Graphs: " << g.size() << std::endl;
+  std::cout << "Number of Edges: " << m << std::endl;
+  std::cout << "Number of Tasks: " << bbtc_batch_ntasks(b) << std::endl;
+  std::cout << "Number of GPUs: " << P.ngpu << std::endl;
+  std::cout << "=====================================" << std::endl;
+  std::vector<uint64_t> per(g.size(), 0);
+  std::vector<double> lat(g.size(), 0);
+  for (int r = 0; r < P.repeat; ++r) {
+    F.begin("Batch Counting Execution Time (s)");
+    if ((st = bbtc_batch_launch(b))) die(st, "launching the batch");
+    if ((st = bbtc_batch_results(b, per.data(), nullptr, lat.data()))) die(st, "reading the batch results");
+    F.end();
+  }
+  uint64_t total = 0;
+  for (size_t x = 0; x < g.size(); ++x) {
+    std::cout << "Graph " << x << " " << e[x].path << ": cuts " << ncut[x] << ", triangles " << per[x]
+              << ", done after " << std::fixed << std::setprecision(3) << lat[x] << " ms" << std::endl;
+    total += per[x];
+  }
+  std::cout << "Number of Triangles: " << total << std::endl;
+  F.print();
+  bbtc_batch_destroy(b);
+  for (auto* l : L) bbtc_layout_free(l);
+  for (auto* x : g) bbtc_hgraph_free(x);
+  bbtc_ctx_destroy(ctx);
+  return 0;
+}
+
+int run_serve(const Params& P) {
+  std::vector<ListEntry> e = read_list(P.serve_list);
+  // the server takes arrivals in time order
+  std::vector<size_t> order(e.size());
+  for (size_t x = 0; x < e.size(); ++x) order[x] = x;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return e[a].value < e[b].value; });
+  Fields F;
+  bbtc_ctx* ctx = nullptr;
+  F.begin("HIP Initial Memory Allocations and Stream Creation");
+  bbtc_status st = bbtc_ctx_create(0, nullptr, &ctx);
+  if (st) die(st, "creating the device context");
+  bbtc_server* srv = nullptr;
+  if ((st = bbtc_server_create(ctx, nullptr, &srv))) die(st, "creating the server");
+  F.end();
+  std::vector<bbtc_hgraph*> g;
+  for (auto& x : e) g.push_back(load_listed(x.path));
+  F.begin("Partitioning and Layout construction (s)");
+  const std::vector<uint32_t> ncut = cut_counts(ctx, g, e, P, false);
+  std::vector<bbtc_layout*> L(g.size(), nullptr);
+  st = bbtc_layouts_build(g.data(), g.size(), ncut.data(), part_kind(P.part), L.data());
+  if (st) die(st, "building the block layouts");
+  std::vector<std::vector<bbtc_task>> tasks(g.size());
+  for (size_t x = 0; x < g.size(); ++x) {
+    tasks[x].resize(bbtc_task_count(ncut[x]));
+    uint64_t got = 0;
+    if ((st = bbtc_tasks_build(L[x], BBTC_ORDER_DESC, tasks[x].data(), tasks[x].size(), &got)))
+      die(st, "building a task list");
+    tasks[x].resize(got);
+  }
+  F.end();
+  uint64_t m = 0;
+  for (auto* x : g) m += edges_of(x);
+  std::cout << "=====================================" << std::endl;
+  std::cout << "Request list: " << P.serve_list << std::endl;
+  std::cout << "Number of Requests: " << g.size() << std::endl;
+  std::cout << "Number of Edges: " << m << std::endl;
+  std::cout << "Number of GPUs: " << P.ngpu << std::endl;
+  std::cout << "=====================================" << std::endl;
+  const bbtc_kernel kern = P.kernel == "tuned" ? BBTC_KERNEL_AUTO : kernel_kind(P.kernel);
+  std::vector<uint64_t> id(g.size(), 0);
+  std::vector<bbtc_server_result> res(g.size());
+  F.begin("Serving makespan (s)");
+  if ((st = bbtc_server_reset_clock(srv))) die(st, "resetting the server clock");
+  for (size_t x : order)
+    if ((st = bbtc_server_submit(srv, L[x], tasks[x].data(), tasks[x].size(), kern, e[x].value, nullptr, &id[x])))
+      die(st, "submitting " + e[x].path);
+  if ((st = bbtc_server_wait_idle(srv))) die(st, "serving");
+  F.end();
+  std::vector<bbtc_server_result> out(g.size());
+  uint64_t got = 0;
+  if ((st = bbtc_server_poll(srv, out.data(), out.size(), 0, &got))) die(st, "collecting results");
+  uint64_t total = 0;
+  std::vector<double> lat;
+  for (size_t x = 0; x < g.size(); ++x) {
+    const bbtc_server_result* r = nullptr;
+    for (uint64_t k = 0; k < got; ++k)
+      if (out[k].id == id[x]) r = &out[k];
+    if (!r || r->status != BBTC_OK) {
+      std::cerr << "bbtc: request " << x << " (" << e[x].path << ") failed" << std::endl;
+      return 1;
+    }
+    const double ms = (r->done_s - r->arrival_s) * 1e3;
+    lat.push_back(ms);
+    total += r->total;
+    std::cout << "Request " << x << " " << e[x].path << ": cuts " << ncut[x] << ", triangles " << r->total
+              << ", arrival " << std::fixed << std::setprecision(3) << r->arrival_s << " s, latency " << ms
+              << " ms" << std::endl;
+  }
+  std::cout << "Number of Triangles: " << total << std::endl;
+  F.print();
+  std::cout << "Latency p50 (ms) : " << std::fixed << std::setprecision(4) << percentile(lat, 50) << std::endl;
+  std::cout << "Latency p99 (ms) : " << std::fixed << std::setprecision(4) << percentile(lat, 99) << std::endl;
+  bbtc_server_destroy(srv);
+  for (auto* l : L) bbtc_layout_free(l);
+  for (auto* x : g) bbtc_hgraph_free(x);
+  bbtc_ctx_destroy(ctx);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   Params P = parse(argc, argv);
+  if (!P.batch_list.empty()) return run_batch(P);
+  if (!P.serve_list.empty()) return run_serve(P);
   std::string path = P.file;
   const bool bcsr = ends_with(path, ".bcsr");  // utils.hpp:300-301
   if (bcsr) {
@@ -245,11 +510,7 @@ int main(int argc, char** argv) {
           return 1;
         }
     } else {
-      const bbtc_part kind = P.part == "uni"   ? BBTC_PART_UNI
-                             : P.part == "n1d" ? BBTC_PART_N1D
-                             : P.part == "rac" ? BBTC_PART_RAC
-                                               : BBTC_PART_PAL;
-      st = bbtc_partition(g, kind, p, cuts.data());
+      st = bbtc_partition(g, part_kind(P.part), p, cuts.data());
       if (st) die(st, "partitioning");
     }
     st = bbtc_layout_build(g, cuts.data(), p, &L);
@@ -281,12 +542,7 @@ int main(int argc, char** argv) {
   if (P.border == -1 || uint64_t(P.border) > ntasks) separator = ntasks;
   else separator = uint64_t(P.border);
 
-  const bbtc_kernel kern = P.kernel == "merge"    ? BBTC_KERNEL_MERGE
-                           : P.kernel == "bitmap" ? BBTC_KERNEL_BITMAP
-                           : P.kernel == "dense"  ? BBTC_KERNEL_DENSE
-                           : P.kernel == "rdense" ? BBTC_KERNEL_RDENSE
-                           : P.kernel == "lane"   ? BBTC_KERNEL_LANE
-                                                  : BBTC_KERNEL_AUTO;
+  const bbtc_kernel kern = kernel_kind(P.kernel);
   // --kernel tuned: per-task forms measured once on the first upload
   std::vector<uint8_t> choice;
   bbtc_ctx* ctx = nullptr;

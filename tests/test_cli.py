@@ -90,3 +90,18 @@ def test_missing_graph_and_usage(tmp_path):
     assert p.returncode == 2 and "Graph doesn't exist" in p.stdout
     p = run(["--help"])
     assert p.returncode == 0 and "--partitioner" in p.stdout
+
+
+def test_batch_and_serve_modes_validate(c1, tmp_path):
+    # the list modes count on the device; without one they refuse, as they do
+    # for a missing list or both modes at once (exit 2, usage printed)
+    d, path = c1
+    lst = tmp_path / "list.txt"
+    lst.write_text(f"{path}\n")
+    for args, msg in [(["--batch", str(lst), "--ngpu", "0"], "need a GPU"),
+                      (["--serve", str(lst), "--ngpu", "0"], "need a GPU"),
+                      (["--batch", str(tmp_path / "none.txt"), "--ngpu", "0"], "List file doesn't exist"),
+                      (["--batch", str(lst), "--serve", str(lst)], "exclusive")]:
+        p = run(args)
+        assert p.returncode == 2 and msg in p.stdout, (args, p.stdout)
+        assert "--batch <list>" in p.stdout

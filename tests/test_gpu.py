@@ -218,6 +218,38 @@ def test_cli_gpu(tmp_path):
             assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
 
 
+def test_cli_batch_and_serve(tmp_path):
+    # golden graphs (totals written by the reference's code) as list files
+    small = [d for d in GOLDEN if d["fingerprint"]["m"] < 3_000_000][:5]
+    paths, totals = [], []
+    for x, d in enumerate(small):
+        p = str(tmp_path / f"g{x}.bbtc")
+        graph_from_spec(d["graph"]).save(p)
+        paths.append(p)
+        totals.append(d["runs"][0]["total"])
+    batch = tmp_path / "batch.txt"
+    batch.write_text("".join(f"{p} {1 + x % 4}\n" if x % 2 else f"{p}\n" for x, p in enumerate(paths)))
+    for extra in ([], ["--kernel", "tuned"], ["--ncut", "3", "--partitioner", "uni"]):
+        r = subprocess.run([B.CLI_PATH, "--batch", str(batch), "--repeat", "2"] + extra, capture_output=True,
+                           text=True, timeout=300)
+        log("cli batch", extra, r.returncode)
+        assert r.returncode == 0, r.stderr
+        assert f"Number of Graphs: {len(paths)}" in r.stdout
+        assert f"Number of Triangles: {sum(totals)}" in r.stdout
+        for x, t in enumerate(totals):
+            assert f"triangles {t}," in [ln for ln in r.stdout.splitlines() if ln.startswith(f"Graph {x} ")][0]
+        assert r.stdout.count("Batch Counting Execution Time (s)") == 2
+    serve = tmp_path / "serve.txt"
+    serve.write_text("".join(f"{p} {0.01 * (len(paths) - x)}\n" for x, p in enumerate(paths)))  # out of order
+    r = subprocess.run([B.CLI_PATH, "--serve", str(serve)], capture_output=True, text=True, timeout=300)
+    log("cli serve", r.returncode)
+    assert r.returncode == 0, r.stderr
+    assert f"Number of Triangles: {sum(totals)}" in r.stdout
+    for x, t in enumerate(totals):
+        assert f"triangles {t}," in [ln for ln in r.stdout.splitlines() if ln.startswith(f"Request {x} ")][0]
+    assert "Latency p99 (ms)" in r.stdout
+
+
 @pytest.mark.parametrize("spec", [("rmat", 10, 8, 1), ("rmat", 18, 16, 7), ("rmat", 20, 32, 5),
                                   ("gnm", 5000, 40000, 3), ("gnm", 1 << 18, 1 << 21, 9)])
 def test_device_generator_matches_host(dev, spec):
