@@ -82,6 +82,9 @@ def run_batched(dev: Device, specs, log=print, sample_every: int = 64, cands=(1,
     t0 = time.time()
     layouts = layouts_build(graphs, ncut, "pal")
     log(f"batched: {len(layouts)} layouts in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    alg_bytes = sum(L.task_bytes(L.tasks("desc"), 4)[1] for L in layouts)
+    log(f"batched: algorithmic bytes {alg_bytes / 1e9:.1f} GB (o=4) in {time.time() - t0:.1f}s")
 
     # plain per-graph launch: graphs resident, one plan per graph (auto forms),
     # counted one after another
@@ -138,6 +141,7 @@ def run_batched(dev: Device, specs, log=print, sample_every: int = 64, cands=(1,
         batched_s=batched_s, batched_graphs_per_s=len(graphs) / batched_s,
         batched_p50_ms=percentile(lat, 50), batched_p99_ms=percentile(lat, 99),
         speedup=plain_s / batched_s, ncut_tune_s=tune_s, batch_setup_s=setup_s,
+        algorithmic_bytes=int(alg_bytes), roofline_frac=alg_bytes / batched_s / 8e12,
         device_bytes=b.bytes, ntasks=b.ntasks,
     )
     b.destroy()

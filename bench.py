@@ -23,7 +23,8 @@ Prints one JSON line:
                 edges/s and roofline fraction of the same count
   batched       4,096 uneven graphs (70% RMAT s10-18, 30% G(n,m)) resident,
                 one captured persistent launch vs the plain per-graph launch:
-                graphs/s, per-graph p50/p99, bit-exact per-graph counts
+                graphs/s, per-graph p50/p99, bit-exact per-graph counts, and
+                the batch's algorithmic bytes / launch time vs 8 TB/s
   serving       requests (50% RMAT s12-24 ef16, 50% G(n,m) n 2^12-2^22 d 32)
                 through the continuous-batching server vs the plain
                 sequential launch: capacity, and p50/p99 latency at Poisson
