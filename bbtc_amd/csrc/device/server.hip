@@ -58,7 +58,7 @@ constexpr uint64_t kDefaultQuantum = 1ull << 18;
 constexpr uint64_t kMinQuantum = 1ull << 14;
 constexpr uint64_t kMaxQuantum = 1ull << 24;
 constexpr double kDefaultLaunchMs = 10.0;
-constexpr uint32_t kDefaultOldestPct = 75;
+constexpr uint32_t kDefaultOldestPct = 90;
 constexpr uint32_t kDefaultInflight = 2;
 constexpr uint32_t kDefaultSegments = 1024;
 
@@ -176,7 +176,7 @@ struct bbtc_server {
   bool sequential = false;
   bool lane_turn = false;  // kind of the next launch when both kinds have work
   bool lane_form = false;  // AUTO requests may take the lane form
-  uint32_t oldest_pct = 75;  // share of each launch's budget kept by the oldest request
+  uint32_t oldest_pct = 90;  // share of each launch's budget kept by the oldest request
 
   char* heap_base = nullptr;
   bbtc::Heap heap;

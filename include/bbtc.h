@@ -398,7 +398,7 @@ typedef struct bbtc_server_opts {
                               the work budget follows the measured rate of finished launches
                               so that a launch takes about this long (0: 10000 us) */
   uint32_t oldest_pct;     /* percent of each launch's budget the oldest request with work
-                              keeps (0: 75); the rest goes shortest remaining work first */
+                              keeps (0: 90); the rest goes shortest remaining work first */
 } bbtc_server_opts;
 
 typedef struct bbtc_server_result {
