@@ -112,27 +112,34 @@ __device__ __forceinline__ bool sorted_contains(Ptr s, uint32_t n, uint32_t x) {
 }
 
 // |A[a, ae) ∩ B[b, be)| for sorted lists of distinct entries, four entries
-// of each list at a time: the eight loads of a step are in flight together,
-// then all 16 pairs are compared, and the block with the smaller last entry
-// moves on (both when equal). A block past its list's end is padded with
-// values no entry takes (ids are below 2^32 - 2), distinct between the lists.
-__device__ __forceinline__ uint32_t intersect_blocked(gptr<uint32_t> A, uint32_t a, uint32_t ae,
-                                                      gptr<uint32_t> B, uint32_t b, uint32_t be) {
-  uint32_t n = 0;
-  while (a < ae && b < be) {
-    uint32_t x[4], y[4];
+// of each list at a time: all 16 pairs of the two current blocks are
+// compared, and the block with the smaller last entry moves on (both when
+// equal). A block past its list's end is padded with values no entry takes
+// (ids are below 2^32 - 2), distinct between the lists.
+//
+// Block i..i+4 of P (entries at or past e read as pad).
+__device__ __forceinline__ void load4(gptr<uint32_t> P, uint32_t i, uint32_t e, uint32_t pad, uint32_t (&x)[4]) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      x[k] = a + k < ae ? A[a + k] : 0xFFFFFFFFu;
-      y[k] = b + k < be ? B[b + k] : 0xFFFFFFFEu;
-    }
+  for (int k = 0; k < 4; ++k) x[k] = i + k < e ? P[i + k] : pad;
+}
+
+// The intersection from blocks x = A[a, a+4) and y = B[b, b+4) already
+// loaded (a < ae, b < be); only the block that moves on is loaded again.
+__device__ __forceinline__ uint32_t intersect_from(gptr<uint32_t> A, uint32_t a, uint32_t ae, gptr<uint32_t> B,
+                                                   uint32_t b, uint32_t be, uint32_t (&x)[4], uint32_t (&y)[4]) {
+  uint32_t n = 0;
+  for (;;) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) n += x[i] == y[j] ? 1u : 0u;
     const uint32_t xl = x[3], yl = y[3];
-    if (xl <= yl) a += 4;
-    if (yl <= xl) b += 4;
+    const bool na = xl <= yl, nb = yl <= xl;
+    if (na) a += 4;
+    if (nb) b += 4;
+    if (a >= ae || b >= be) break;
+    if (na) load4(A, a, ae, 0xFFFFFFFFu, x);
+    if (nb) load4(B, b, be, 0xFFFFFFFEu, y);
   }
   return n;
 }
@@ -369,7 +376,13 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
         const uint32_t q_dac = __shfl(dac, q, kWave);
         if (w < W) {
           const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
-          c += intersect_blocked(ac_c, q_ac0, q_ac0 + q_dac, bc_c, bc_rp[v], bc_rp[v + 1]);
+          const uint32_t b0 = bc_rp[v], b1 = bc_rp[v + 1];
+          if (b0 < b1) {
+            uint32_t x[4], y[4];
+            load4(ac_c, q_ac0, q_ac0 + q_dac, 0xFFFFFFFFu, x);
+            load4(bc_c, b0, b1, 0xFFFFFFFEu, y);
+            c += intersect_from(ac_c, q_ac0, q_ac0 + q_dac, bc_c, b0, b1, x, y);
+          }
         }
       }
       acc += wave_sum32(c);
@@ -604,7 +617,9 @@ __global__ __launch_bounds__(kBlock) void count_probe(CountArgs a) {
   count_units<true>(a, lds_all);
 }
 
-__global__ __launch_bounds__(kBlock) void count_probe_lanes(CountArgs a) {
+// 8 waves per SIMD: the lane form has no LDS and few registers, and it is
+// latency-bound, so it keeps the full occupancy.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void count_probe_lanes(CountArgs a) {
   count_units<false>(a, nullptr);
 }
 
