@@ -5,11 +5,15 @@
 // front, one launch per task) and CPU workers (from the back, one task at a
 // time) race for [Separator, |T|) with a per-task atomic_flag. Here one host
 // thread feeds the device batches of tasks from the front of the list, each
-// batch one persistent launch sized to keep every CU busy, with up to
-// `nstreams` batches in flight on separate streams; the other OpenMP threads
-// claim single tasks from the back. Both sides claim from one packed 64-bit
-// word (next front task, one past the next back task), so every task is
-// counted exactly once.
+// batch one persistent launch, with up to `nstreams` batches in flight on
+// separate streams; the other OpenMP threads claim single tasks from the
+// back. Both sides claim from one packed 64-bit word (next front task, one
+// past the next back task), so every task is counted exactly once. A batch
+// holds 1/(4 * nstreams) of the items left, and at least enough to keep
+// every CU busy (guided self-scheduling: large launches while the host is
+// far behind, small ones where the two sides meet). When the host may take
+// no task (the whole list is device-only), the device counts it in one
+// launch.
 #include <omp.h>
 
 #include <algorithm>
@@ -68,14 +72,19 @@ HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bb
             f = cur >> 32;
             b = cur & 0xFFFFFFFFull;
             if (f >= b) break;
-            // smallest t1 in (f, b] whose items reach the batch target
-            uint64_t lo = f + 1, hi = b;
-            while (lo < hi) {
-              const uint64_t mid = (lo + hi) / 2;
-              if (dev::plan_items(plan, f, mid) >= target) hi = mid;
-              else lo = mid + 1;
+            if (sep >= b) {
+              t1 = b;  // nothing left the host may take: one launch
+            } else {
+              const uint64_t want = std::max(target, dev::plan_items(plan, f, b) / (4 * uint64_t(S)));
+              // smallest t1 in (f, b] whose items reach the batch size
+              uint64_t lo = f + 1, hi = b;
+              while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (dev::plan_items(plan, f, mid) >= want) hi = mid;
+                else lo = mid + 1;
+              }
+              t1 = lo;
             }
-            t1 = lo;
             if (state.compare_exchange_weak(cur, pack(t1, b))) { got = true; break; }
           }
           if (!got) break;
