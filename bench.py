@@ -186,7 +186,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build counts one graph on one MI355X; "
+        print("bench.py runs on one GPU only: it counts one graph on one MI355X; "
               "the multi-GPU task sharding is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
