@@ -632,7 +632,14 @@ class ServerResult:
 
 
 class Server:
-    """Continuous-batching request loop on one device (bbtc_server_*)."""
+    """Continuous-batching request loop on one device (bbtc_server_*).
+
+    Options (0 / False: the library's defaults): heap_bytes, the device heap
+    (200 GiB); quantum_items, a fixed item budget per launch instead of the
+    default budget in estimated work sized to launch_us (10 ms); oldest_pct,
+    the oldest request's share of each launch (90); lane_form, let AUTO
+    requests whose tasks all admit the lane form run in launches of their
+    own; sequential, one request at a time (the plain path's order)."""
 
     def __init__(self, dev: "Device", heap_bytes: int = 0, quantum_items: int = 0, max_inflight: int = 0,
                  max_segments: int = 0, sequential: bool = False, lane_form: bool = False,
