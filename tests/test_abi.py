@@ -60,3 +60,37 @@ def test_status_strings_and_errors_without_gpu():
     with pytest.raises(B.BbtcError) as e:
         g.partition("pal", 0)
     assert e.value.status == 1  # BBTC_ERR_INVALID
+
+
+STRUCTS = {
+    "bbtc_task": B.bbtc_task, "bbtc_tile_info": B.bbtc_tile_info, "bbtc_opts": B.bbtc_opts,
+    "bbtc_server_opts": B.bbtc_server_opts, "bbtc_server_result": B.bbtc_server_result,
+    "bbtc_server_stats": B.bbtc_server_stats,
+}
+
+
+def test_python_structs_match_the_header(tmp_path):
+    # sizes and field offsets of every struct the ctypes layer mirrors, as the
+    # C compiler lays them out from include/bbtc.h
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "bbtc.h"', "int main(void) {"]
+    for name, py in STRUCTS.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            name, what, val = ln.split()
+            got[(name, what)] = int(val)
+    for name, py in STRUCTS.items():
+        assert got[(name, "size")] == C.sizeof(py), name
+        for field, _ in py._fields_:
+            assert got[(name, field)] == getattr(py, field).offset, (name, field)
