@@ -4,8 +4,9 @@
 // rows [x*R, (x+1)*R) of the ab tile's compact non-empty row list (R <= 64,
 // per task). A launch counts one or more segments (item ranges of plans,
 // possibly of different graphs) numbered consecutively. One wave owns one
-// unit at a time and draws the next from an atomic head counter, so a single
-// persistent launch drains every task of every segment.
+// unit at a time and draws the next from one of eight per-XCD head counters
+// (the launch's items are dealt to eight parts), so a single persistent
+// launch drains every task of every segment.
 //
 // Inside a unit the wave flattens its work twice with wave prefix sums:
 //   rows  -> wedges (u, v), v in ab(u), for rows whose ac(u) is non-empty
@@ -20,15 +21,20 @@
 // spreads over the whole machine, so a hub-core item never leaves one wave
 // running alone at the end of a launch.
 //
-// Membership of w in ac(u) is tested in one of four forms, chosen per task:
+// Membership of w in ac(u) is tested in one of five forms, chosen per task:
 //   search: ac(u) as a sorted list in the wave's LDS slice (a window of rows
 //           at a time); a probe is a binary search.
 //   bitmap: ac(u) as one bit per column of band k in LDS; a probe is one bit
 //           test.
 //   dense:  ac(u) bitmap in LDS AND the bc(v) bitmap row from HBM, popcount.
 //   rdense: ac(u) entries tested against the bc(v) bitmap row in HBM.
-// Matches are counted with ballot + popcount into a wave-uniform 64-bit count
-// and added to the task's slot with one atomic per task change.
+//   lane:   one lane per wedge intersects bc(v) and ac(u) in HBM, four
+//           entries of each at a time (rows of at most 32 entries; no LDS,
+//           no second phase; a launch of lane-form tasks only runs the
+//           count_probe_lanes instantiation at 8 waves per SIMD).
+// Matches are counted with ballot + popcount (or a wave sum) into a
+// wave-uniform 64-bit count and added to the task's slot with one atomic per
+// task change.
 #pragma once
 
 #include <hip/hip_runtime.h>
