@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the G(n=2^22, m=2^26) count")
     ap.add_argument("--c2-only", action="store_true",
                     help="only the G(n=2^22, m=2^26) count (--steps launches); prints its object as the line")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched workload (--batched-graphs graphs); prints its object as the line")
     ap.add_argument("--batched-graphs", type=int, default=4096, help="0: skip the batched workload")
     ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
     args = ap.parse_args()
@@ -196,6 +198,10 @@ def main():
     dev = B.Device(0)
     if args.c2_only:
         print(json.dumps(count_c2(B, dev, steps=args.steps)), flush=True)
+        return
+    if args.batched_only:
+        from bbtc_amd import workloads as W
+        print(json.dumps(W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log)), flush=True)
         return
     read_gbs = dev.read_bandwidth(8 << 30, 5)
     stream_gbs = float(np.median(read_gbs))
