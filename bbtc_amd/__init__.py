@@ -42,7 +42,7 @@ class bbtc_task(C.Structure):
 
 class bbtc_tile_info(C.Structure):
     _fields_ = [("nnz", C.c_uint64), ("height", C.c_uint32), ("width", C.c_uint32),
-                ("nz_rows", C.c_uint32), ("avg_degree", C.c_float)]
+                ("nz_rows", C.c_uint32), ("avg_degree", C.c_float), ("max_row", C.c_uint32)]
 
 
 class bbtc_opts(C.Structure):
@@ -290,6 +290,7 @@ class TileInfo:
     width: int
     nz_rows: int
     avg_degree: float
+    max_row: int
 
 
 class Layout:
@@ -323,7 +324,7 @@ class Layout:
     def tile(self, i, k) -> TileInfo:
         t = bbtc_tile_info()
         _check(lib().bbtc_layout_tile(self._h, i, k, C.byref(t)))
-        return TileInfo(t.nnz, t.height, t.width, t.nz_rows, t.avg_degree)
+        return TileInfo(t.nnz, t.height, t.width, t.nz_rows, t.avg_degree, t.max_row)
 
     @property
     def device_bytes(self) -> int:

@@ -159,6 +159,7 @@ bbtc_status bbtc_layout_tile(const bbtc_layout* L, uint32_t i, uint32_t k, bbtc_
   out->width = L->L.height(k);
   out->nz_rows = L->L.nz_rows[t];
   out->avg_degree = L->L.deg[t];
+  out->max_row = L->L.max_row[t];
   return BBTC_OK;
 }
 

@@ -122,6 +122,7 @@ typedef struct bbtc_tile_info {
   uint32_t width;       /* columns of band k */
   uint32_t nz_rows;     /* rows with at least one entry */
   float avg_degree;     /* (float)nnz / height, as block_csr.cpp:278 (NaN when height is 0) */
+  uint32_t max_row;     /* entries in the tile's longest row (the lane form needs <= 32) */
 } bbtc_tile_info;
 
 /* Replaces BlockCSR(cuts, x_adj, adj) -> CSRtoBCSR + ComputeTileProps

@@ -80,6 +80,9 @@ def test_nonempty_row_counts():
         for k in range(i, 4):
             sel = (bi == i) & (bk == k)
             assert L.tile(i, k).nz_rows == len(np.unique(rows[sel]))
+            # entries of the longest row, which decides where the lane form applies
+            longest = int(np.bincount(rows[sel]).max()) if sel.any() else 0
+            assert L.tile(i, k).max_row == longest
 
 
 @pytest.mark.parametrize("p", [1, 2, 3, 4, 8, 16, 32])
