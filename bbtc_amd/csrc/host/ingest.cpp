@@ -159,15 +159,57 @@ bool ends_with(const std::string& s, const char* suf) {
   return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
 }
 
-// Rows of a matrix with n rows from (row, col) pairs, each row sorted.
+// Rows of a matrix with n rows from (row, col) pairs, each row sorted. Each
+// thread counts its slice of the pairs into a histogram of its own and then
+// scatters that slice to the slots the prefix of all histograms gives it, so
+// no slot is contended; when threads x rows would outgrow the pairs (an id
+// space far sparser than the edges) the slots are claimed with atomics
+// instead. The order inside a row before its sort does not matter.
 void rows_of(uint32_t n, const std::vector<uint32_t>& row, const std::vector<uint32_t>& col,
              std::vector<uint32_t>& ptr, std::vector<uint32_t>& idx) {
+  const int T = std::max(1, omp_get_max_threads());
+  const size_t m = row.size();
   ptr.assign(size_t(n) + 1, 0);
-  for (uint32_t r : row) ptr[size_t(r) + 1]++;
-  for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
-  idx.assign(row.size(), 0);
-  std::vector<uint32_t> pos(ptr.begin(), ptr.end() - 1);
-  for (size_t q = 0; q < row.size(); ++q) idx[pos[row[q]]++] = col[q];
+  idx.resize(m);
+  if (size_t(T) * n <= 2 * m + (size_t(1) << 24)) {
+    std::vector<uint32_t> hist(size_t(T) * n, 0);
+#pragma omp parallel num_threads(T)
+    {
+      const int t = omp_get_thread_num();
+      uint32_t* h = hist.data() + size_t(t) * n;
+      const size_t q0 = m * size_t(t) / size_t(T), q1 = m * size_t(t + 1) / size_t(T);
+      for (size_t q = q0; q < q1; ++q) ++h[row[q]];
+#pragma omp barrier
+#pragma omp for schedule(static)
+      for (int64_t i = 0; i < int64_t(n); ++i) {
+        uint32_t k = 0;
+        for (int u = 0; u < T; ++u) k += hist[size_t(u) * n + size_t(i)];
+        ptr[size_t(i) + 1] = k;
+      }
+#pragma omp single
+      for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
+#pragma omp for schedule(static)
+      for (int64_t i = 0; i < int64_t(n); ++i) {
+        uint32_t at = ptr[size_t(i)];
+        for (int u = 0; u < T; ++u) {
+          uint32_t& c = hist[size_t(u) * n + size_t(i)];
+          const uint32_t k = c;
+          c = at;
+          at += k;
+        }
+      }
+      for (size_t q = q0; q < q1; ++q) idx[h[row[q]]++] = col[q];
+    }
+  } else {
+    uint32_t* cnt = ptr.data() + 1;
+#pragma omp parallel for schedule(static, 1 << 16)
+    for (int64_t q = 0; q < int64_t(m); ++q) __atomic_fetch_add(&cnt[row[q]], 1u, __ATOMIC_RELAXED);
+    for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
+    std::vector<uint32_t> pos(ptr.begin(), ptr.end() - 1);
+    uint32_t* at = pos.data();
+#pragma omp parallel for schedule(static, 1 << 16)
+    for (int64_t q = 0; q < int64_t(m); ++q) idx[__atomic_fetch_add(&at[row[q]], 1u, __ATOMIC_RELAXED)] = col[q];
+  }
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < int64_t(n); ++i) std::sort(idx.begin() + ptr[i], idx.begin() + ptr[i + 1]);
 }
@@ -178,9 +220,13 @@ void rows_of(uint32_t n, const std::vector<uint32_t>& row, const std::vector<uin
 Csr csr_from_text(const std::string& path, bool dedup) {
   if (ends_with(path, ".graph"))
     fail(BBTC_ERR_FORMAT, "METIS .graph input is not supported; convert it to an edge list");
-  std::ifstream f(path, std::ios::binary);
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
   if (!f) fail(BBTC_ERR_IO, "cannot open " + path);
-  std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  const std::streamoff size = f.tellg();
+  if (size < 0) fail(BBTC_ERR_IO, "cannot read " + path);
+  std::vector<char> bytes(static_cast<size_t>(size));
+  f.seekg(0);
+  if (size && !f.read(bytes.data(), size)) fail(BBTC_ERR_IO, "cannot read " + path);
   const char* b = bytes.data();
   const char* e = b + bytes.size();
   if (bytes.size() >= 4 && std::memcmp(b, "PIGO", 4) == 0)

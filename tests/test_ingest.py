@@ -116,3 +116,40 @@ def test_cli_text_input(tmp_path):
     r2 = subprocess.run([B.CLI_PATH, "--graph", src, "--ngpu", "0", "--ncut", "4", "--repeat", "1"],
                         capture_output=True, text=True, timeout=120)
     assert "Saving binary CSR graph" not in r2.stderr and r2.returncode == 0
+
+
+def test_sparse_id_space(tmp_path):
+    # a few thousand edges among ids up to 5M: with several threads, threads x
+    # rows outgrows the edges, so the rows are built with atomic slot counters
+    # rather than per-thread histograms; the graph must be the same up to the
+    # relabeling
+    rng = np.random.default_rng(11)
+    ids = np.sort(rng.choice(5_000_000, 600, replace=False))
+    a, b = ids[rng.integers(0, 600, 4000)], ids[rng.integers(0, 600, 4000)]
+    p = tmp_path / "sparse.txt"
+    p.write_text("".join(f"{x} {y}\n" for x, y in zip(a, b)))
+    g = B.Graph.load_text(str(p))
+    n = g.info[0]
+    assert n == int(max(a.max(), b.max())) + 1
+    off, col = g.offsets().astype(np.int64), g.cols().astype(np.int64)
+    # expected: the simple graph of the input, by closed form trace(A^3)/6
+    keep = a != b
+    used = np.unique(np.concatenate([a[keep], b[keep]]))
+    pos = {int(v): i for i, v in enumerate(used)}
+    A = np.zeros((len(used), len(used)), dtype=np.int64)
+    for x, y in zip(a[keep], b[keep]):
+        A[pos[int(x)], pos[int(y)]] = A[pos[int(y)], pos[int(x)]] = 1
+    # converted graph: keep the rows that carry edges, in order
+    src = np.repeat(np.arange(n), np.diff(off))
+    touched = np.unique(np.concatenate([src, col]))
+    assert len(touched) == len(used)
+    at = {int(v): i for i, v in enumerate(touched)}
+    C = np.zeros_like(A)
+    for x, y in zip(src, col):
+        assert x < y
+        C[at[int(x)], at[int(y)]] = C[at[int(y)], at[int(x)]] = 1
+    assert sorted(C.sum(0)) == sorted(A.sum(0))
+    assert np.trace(C @ C @ C) == np.trace(A @ A @ A)
+    L = B.Layout(g, g.partition("uni", 3))
+    _, tot = L.count_cpu(L.tasks("lex"))
+    assert tot == np.trace(A @ A @ A) // 6
