@@ -522,9 +522,14 @@ int main(int argc, char** argv) {
     if (st) die(st, "writing " + P.save_bcsr);
   }
 
+  // With a GPU the device takes the heaviest tasks from the front of a
+  // descending list (tasks.hpp:25-28); with none, the order of the
+  // reference's CPU build (ascending, tasks.hpp:29-32), so the workers, who
+  // claim from the back, start with the heaviest tasks.
   uint64_t ntasks = 0;
   std::vector<bbtc_task> tasks(bbtc_task_count(p));
-  st = bbtc_tasks_build(L, BBTC_ORDER_DESC, tasks.data(), tasks.size(), &ntasks);
+  st = bbtc_tasks_build(L, P.ngpu > 0 ? BBTC_ORDER_DESC : BBTC_ORDER_ASC, tasks.data(), tasks.size(),
+                        &ntasks);
   if (st) die(st, "building the task list");
 
   std::cout << "=====================================" << std::endl;

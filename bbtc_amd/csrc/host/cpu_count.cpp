@@ -3,13 +3,15 @@
 //
 // Same task semantics as the reference CPU workers (bb_tc.cpp:118-139,
 // 184-197): workers claim tasks from the back of the list with an atomic
-// counter; a task whose tiles include one with average degree > 10000 uses a
-// per-worker stamp table (cpu_dense_kernel.cpp:3-51, merge for rows with
-// d_ab < 4), every other task a sorted merge with early exit
+// counter, here in units of 1,024 non-empty rows so that several workers share
+// a heavy task; a task whose tiles include one with average degree > 10000
+// uses a per-worker stamp table (cpu_dense_kernel.cpp:3-51, merge for rows
+// with d_ab < 4), every other task a sorted merge with early exit
 // (cpu_sparse_kernel.cpp:3-59). Rows come from the compact non-empty list of
 // the ab tile instead of a scan over the whole band.
 #include <omp.h>
 
+#include <algorithm>
 #include <atomic>
 #include <vector>
 
@@ -49,9 +51,10 @@ TaskView view(const Layout& L, const bbtc_task& x) {
                   L.nzrows.data() + L.nz_base[ab], L.nz_rows[ab]};
 }
 
-uint64_t sparse_task(const TaskView& t) {
+// Rows [z0, z1) of the ab tile's non-empty row list.
+uint64_t sparse_rows(const TaskView& t, uint32_t z0, uint32_t z1) {
   uint64_t c = 0;
-  for (uint32_t z = 0; z < t.nnz_rows; ++z) {
+  for (uint32_t z = z0; z < z1; ++z) {
     const uint32_t r = t.nz[z];
     const uint32_t a0 = t.ac_rp[r], a1 = t.ac_rp[r + 1];
     if (a0 == a1) continue;
@@ -63,11 +66,12 @@ uint64_t sparse_task(const TaskView& t) {
   return c;
 }
 
-uint64_t dense_task(const TaskView& t, std::vector<uint32_t>& stamp, uint32_t width) {
-  if (stamp.size() < width) stamp.resize(width);
-  std::fill(stamp.begin(), stamp.begin() + width, 0xFFFFFFFFu);
+// Rows [z0, z1) with the worker's stamp table, which holds no row of
+// [z0, z1) (it is cleared for every task a worker starts): stamp[c] == r
+// marks column c of ac row r.
+uint64_t dense_rows(const TaskView& t, uint32_t z0, uint32_t z1, uint32_t* stamp) {
   uint64_t c = 0;
-  for (uint32_t z = 0; z < t.nnz_rows; ++z) {
+  for (uint32_t z = z0; z < z1; ++z) {
     const uint32_t r = t.nz[z];
     const uint32_t a0 = t.ac_rp[r], a1 = t.ac_rp[r + 1];
     if (a0 == a1) continue;
@@ -88,31 +92,82 @@ uint64_t dense_task(const TaskView& t, std::vector<uint32_t>& stamp, uint32_t wi
   return c;
 }
 
+bool dense_task(const Layout& L, const bbtc_task& x) {
+  return L.deg[L.tile(x.i, x.k)] > kDenseDegree || L.deg[L.tile(x.j, x.k)] > kDenseDegree ||
+         L.deg[L.tile(x.i, x.j)] > kDenseDegree;
+}
+
+void clear_stamp(std::vector<uint32_t>& stamp, uint32_t width) {
+  if (stamp.size() < width) stamp.resize(width);
+  std::fill(stamp.begin(), stamp.begin() + width, 0xFFFFFFFFu);
+}
+
+// Non-empty ab rows per claim unit of the CPU count.
+constexpr uint32_t kRowsPerUnit = 1024;
+
 }  // namespace
 
 uint64_t count_task_cpu(const Layout& L, const bbtc_task& x, std::vector<uint32_t>& stamp) {
   if (task_skipped(L, x)) return 0;
   const TaskView v = view(L, x);
-  const bool dense = L.deg[L.tile(x.i, x.k)] > kDenseDegree ||
-                     L.deg[L.tile(x.j, x.k)] > kDenseDegree ||
-                     L.deg[L.tile(x.i, x.j)] > kDenseDegree;
-  return dense ? dense_task(v, stamp, L.height(x.k)) : sparse_task(v);
+  if (!dense_task(L, x)) return sparse_rows(v, 0, v.nnz_rows);
+  clear_stamp(stamp, L.height(x.k));
+  return dense_rows(v, 0, v.nnz_rows, stamp.data());
 }
 
+// The workers claim units from one counter: the tasks from the back of the
+// list (as the reference's workers claim tasks, bb_tc.cpp:184-197), each task
+// cut into units of kRowsPerUnit non-empty ab rows, so every worker can share
+// a heavy task instead of one worker holding it to the end. A worker adds its
+// sum of a task's units to the task's count when it moves on; the sum of
+// integers is the same whichever worker counts which rows.
 uint64_t count_cpu(const Layout& L, const bbtc_task* tasks, uint64_t n, int nthreads,
                    uint64_t* per_task) {
-  std::atomic<int64_t> next(int64_t(n) - 1);
+  // first[s]: the first unit of the s-th task from the back
+  std::vector<uint64_t> first(n + 1, 0);
+  for (uint64_t s = 0; s < n; ++s) {
+    const bbtc_task& x = tasks[n - 1 - s];
+    const uint64_t rows = task_skipped(L, x) ? 0 : L.nz_rows[L.tile(x.i, x.j)];
+    first[s + 1] = first[s] + std::max<uint64_t>(1, (rows + kRowsPerUnit - 1) / kRowsPerUnit);
+  }
+  const uint64_t units = first[n];
+  if (per_task) std::fill(per_task, per_task + n, 0);
+  std::atomic<uint64_t> next(0);
   std::atomic<uint64_t> total(0);
   const int T = nthreads > 0 ? nthreads : omp_get_max_threads();
 #pragma omp parallel num_threads(T)
   {
     std::vector<uint32_t> stamp;
-    uint64_t mine = 0;
-    for (int64_t q = next.fetch_sub(1); q >= 0; q = next.fetch_sub(1)) {
-      const uint64_t c = count_task_cpu(L, tasks[q], stamp);
-      if (per_task) per_task[q] = c;
-      mine += c;
+    uint64_t mine = 0, acc = 0;
+    uint64_t s = 0, cur = n;  // cur: the task whose rows acc holds (n: none)
+    TaskView v{};
+    bool dense = false, skip = false;
+    auto flush = [&]() {
+      if (cur < n) {
+        if (per_task) __atomic_fetch_add(&per_task[n - 1 - cur], acc, __ATOMIC_RELAXED);
+        mine += acc;
+      }
+      acc = 0;
+    };
+    for (uint64_t u = next.fetch_add(1); u < units; u = next.fetch_add(1)) {
+      while (first[s + 1] <= u) ++s;  // a worker's units only increase
+      const bbtc_task& x = tasks[n - 1 - s];
+      if (s != cur) {
+        flush();
+        cur = s;
+        skip = task_skipped(L, x);
+        if (!skip) {
+          v = view(L, x);
+          dense = dense_task(L, x);
+          if (dense) clear_stamp(stamp, L.height(x.k));
+        }
+      }
+      if (skip) continue;
+      const uint64_t z0 = (u - first[s]) * kRowsPerUnit;
+      const uint32_t z1 = uint32_t(std::min<uint64_t>(z0 + kRowsPerUnit, v.nnz_rows));
+      acc += dense ? dense_rows(v, uint32_t(z0), z1, stamp.data()) : sparse_rows(v, uint32_t(z0), z1);
     }
+    flush();
     total.fetch_add(mine);
   }
   return total.load();

@@ -150,3 +150,27 @@ def test_subset_and_bad_tasks():
         L.count_cpu(np.array([[1, 0, 2]], np.uint32))  # i > j
     with pytest.raises(B.BbtcError):
         L.count_cpu(np.array([[0, 1, 3]], np.uint32))  # k out of range
+
+
+def test_dense_task_split_over_workers():
+    # 1,025 apex rows each joined to all 10,001 vertices of a path: the apex
+    # band's tile to the path has average degree 10,001 > 10,000, so its task
+    # takes the stamp-table form, and its 1,025 non-empty rows are more than
+    # one claim unit; every apex closes one triangle per path edge
+    apex, base = 1025, 10001
+    n = apex + base
+    row_len = np.concatenate([np.full(apex, base), np.ones(base - 1), [0]]).astype(np.uint64)
+    off = np.concatenate([[0], np.cumsum(row_len)]).astype(np.uint64)
+    cols = np.concatenate([np.tile(np.arange(apex, n, dtype=np.uint32), apex),
+                           np.arange(apex + 1, n, dtype=np.uint32)])
+    g = B.Graph.from_csr(off, cols)
+    L = B.Layout(g, np.array([0, apex, n + 1], np.uint32))
+    assert L.tile(0, 1).avg_degree > 10000 and L.tile(0, 1).nz_rows == apex
+    t = L.tasks("desc")
+    expect = apex * (base - 1)
+    per1, tot1 = L.count_cpu(t, 1)
+    per8, tot8 = L.count_cpu(t, 8)
+    assert tot1 == tot8 == expect and np.array_equal(per1, per8)
+    R = O.RefLayout(g.offsets(), g.cols(), np.array([0, apex, n + 1], np.uint32))
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    assert rtot == expect and np.array_equal(rper, per1)
