@@ -105,3 +105,18 @@ def test_batch_and_serve_modes_validate(c1, tmp_path):
         p = run(args)
         assert p.returncode == 2 and msg in p.stdout, (args, p.stdout)
         assert "--batch <list>" in p.stdout
+
+
+def test_cpu_path_task_order(c1, tmp_path):
+    # without a GPU the task list is in the CPU build's ascending weight order
+    # (tasks.hpp:29-32), so the workers, who claim from the back, start with
+    # the heaviest tasks
+    d, path = c1
+    r = next(x for x in d["runs"] if x["partitioner"] == "pal")
+    out = tmp_path / "pt.txt"
+    p = run(["--graph", path, "--partitioner", "pal", "--ncut", str(r["ncut"]), "--ngpu", "0", "--repeat", "1",
+             "--per-task-out", str(out)])
+    assert p.returncode == 0, p.stderr
+    got = np.array([list(map(int, ln.split()[:3])) for ln in out.read_text().splitlines()], dtype=np.uint32)
+    L = B.Layout(graph_from_spec(d["graph"]), np.array(r["cuts"], dtype=np.uint32))
+    assert np.array_equal(B.tasks_array(got), L.tasks("asc"))
