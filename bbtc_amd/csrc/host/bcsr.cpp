@@ -86,6 +86,14 @@ void load_bcsr(const std::string& path, Csr& g, std::vector<uint32_t>& cuts_out)
   get(fc.f, &P, 4, path);
   if (P < 1 || P > 65535) fail(BBTC_ERR_FORMAT, "bad number of cuts in " + path);
   if (uint64_t(N) * P >= (1ull << 32)) fail(BBTC_ERR_FORMAT, "too many row slots in " + path);
+  // the arrays the header announces must be in the file before they are
+  // allocated, so a damaged header fails at once instead of after a large
+  // allocation
+  const uint64_t need = 4 * ((uint64_t(P) + 1) + (uint64_t(P) * P + 1) + uint64_t(N) * P + E);
+  if (std::fseek(fc.f, 0, SEEK_END) != 0) fail(BBTC_ERR_IO, "cannot read " + path);
+  const long size = std::ftell(fc.f);
+  if (size < 0 || std::fseek(fc.f, 12, SEEK_SET) != 0) fail(BBTC_ERR_IO, "cannot read " + path);
+  if (uint64_t(size) - 12 < need) fail(BBTC_ERR_FORMAT, "truncated .bcsr file " + path);
   std::vector<uint32_t> cuts(size_t(P) + 1), tiles(size_t(P) * P + 1), rows(size_t(N) * P), cols(E);
   get(fc.f, cuts.data(), cuts.size() * 4, path);
   get(fc.f, tiles.data(), tiles.size() * 4, path);

@@ -72,8 +72,11 @@ def test_malformed_files(tmp_path):
     good = tmp_path / "g.bcsr"
     L.save_bcsr(str(good))
     data = good.read_bytes()
+    # "huge": a header announcing 2^26 rows in 32 bands (8 GiB of row slots)
+    # over the same small file fails on the file size, before any allocation
     for name, blob in [("trunc", data[:-8]), ("badp", struct.pack("<III", 10, 0, 0)),
-                       ("cuts", data[:12] + struct.pack("<I", 5) + data[16:])]:
+                       ("cuts", data[:12] + struct.pack("<I", 5) + data[16:]),
+                       ("huge", struct.pack("<III", 1 << 26, 0, 32) + data[12:])]:
         p = tmp_path / f"{name}.bcsr"
         p.write_bytes(blob)
         with pytest.raises(B.BbtcError):
