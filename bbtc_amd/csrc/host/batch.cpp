@@ -147,6 +147,8 @@ bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint
   return BBTC_OK;
 }
 
-uint32_t bbtc_rule_ncut(const uint32_t* rule, uint64_t m) { return rule[bbtc::bucket_of(m)]; }
+uint32_t bbtc_rule_ncut(const uint32_t* rule, uint64_t m) {
+  return rule ? rule[bbtc::bucket_of(m)] : 0;
+}
 
 }  // extern "C"

@@ -329,7 +329,7 @@ bbtc_status bbtc_layouts_build(const bbtc_hgraph* const* graphs, uint64_t n, con
 bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint64_t n,
                            const uint32_t* cands, uint32_t ncand, bbtc_part kind, int tune_forms,
                            uint32_t* rule, uint64_t* work_out);
-/* The rule's cut count for a graph with m edges. */
+/* The rule's cut count for a graph with m edges (0 when rule is NULL). */
 uint32_t bbtc_rule_ncut(const uint32_t* rule, uint64_t m);
 
 typedef struct bbtc_batch bbtc_batch;

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -94,3 +95,42 @@ def test_python_structs_match_the_header(tmp_path):
         assert got[(name, "size")] == C.sizeof(py), name
         for field, _ in py._fields_:
             assert got[(name, field)] == getattr(py, field).offset, (name, field)
+
+
+SWEEP = r'''
+import ctypes as C, re, sys
+import bbtc_amd as B
+hdr = re.sub(r"/\*.*?\*/", "", open(sys.argv[1]).read(), flags=re.S)
+lib = C.CDLL(B.LIB_PATH)
+size = int(sys.argv[2])
+for ret, name, params in re.findall(r"^\s*([A-Za-z_][\w\s\*]*?)\b(bbtc_\w+)\s*\(([^)]*)\)\s*;", hdr, flags=re.M):
+    args = []
+    for p in (p.strip() for p in params.split(",")):
+        if not p or p == "void":
+            continue
+        args.append(C.c_void_p(None) if "*" in p else C.c_double(0.0) if "double" in p
+                    else C.c_float(0.0) if "float" in p else C.c_uint64(size))
+    f = getattr(lib, name)
+    status = ret.strip().startswith("bbtc_status")
+    f.restype = C.c_int if status else (C.c_void_p if "*" in ret else C.c_uint64)
+    print(name, end=" ", flush=True)
+    print(int(status), f(*args), flush=True)
+'''
+
+
+@pytest.mark.parametrize("size", [0, 1])
+def test_null_arguments_are_errors(size):
+    """Every entry point, called with NULL pointers and sizes 0 or 1 and no
+    GPU, returns an error status (or, for the getters and destroyers, does
+    nothing) instead of crashing; the sweep runs in a child process so a crash
+    names the function."""
+    r = subprocess.run([sys.executable, "-c", SWEEP, HEADER, str(size)], capture_output=True, text=True,
+                       timeout=120, cwd=ROOT)
+    assert r.returncode == 0, r.stdout[-300:] + r.stderr[-2000:]
+    seen = 0
+    for ln in r.stdout.splitlines():
+        name, status, value = ln.split()
+        seen += 1
+        if status == "1" and not (name == "bbtc_layouts_build" and size == 0):  # n = 0: nothing to build
+            assert int(value) != 0, name
+    assert seen == len(declared_functions())
