@@ -1,7 +1,7 @@
 """ctypes view of the CPU oracle (oracle/liborc.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg. The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check results, and by bench.py's cpu_baseline leg. The product never imports it.
 """
 from __future__ import annotations
 

@@ -1,9 +1,9 @@
 /*
  * bbtc_oracle.c - CPU restatement of the reference block-based triangle count.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/, the smoke
- * entry point and the bench's CPU-baseline leg may load it. The product path
- * (libbbtc.so) never links, calls or falls back to it.
+ * TEST INFRASTRUCTURE ONLY. Results are checked against this file: only
+ * tests/, the smoke entry point and the bench's CPU-baseline leg may load it.
+ * The product path (libbbtc.so) never links, calls or falls back to it.
  *
  * It restates, in plain C with OpenMP, the reference's CPU build (paths
  * relative to the GT-TDAlab/bbTC tree):
