@@ -6,6 +6,7 @@
 // 1091467301 feeding std::uniform_real_distribution<double>).
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
 #include <random>
@@ -104,16 +105,65 @@ std::vector<uint32_t> partition_n1d(const Csr& g, uint32_t P) {
 // ---- rac: sarma::refine_a_cut::partition(A, P) (refine_a_cut.hpp:43-223) -------
 namespace {
 
-// Rows of the transpose of an n x n CSR, each sorted.
+// Row ranges of g with about m / T entries each: T + 1 row bounds.
+std::vector<u32> entry_split(const Csr& g, int T) {
+  std::vector<u32> at(size_t(T) + 1, g.n);
+  at[0] = 0;
+  for (int t = 1; t < T; ++t) {
+    const uint64_t want = g.m * uint64_t(t) / uint64_t(T);
+    at[t] = u32(std::lower_bound(g.off.begin(), g.off.begin() + g.n, want) - g.off.begin());
+  }
+  for (int t = 1; t <= T; ++t) at[t] = std::max(at[t], at[t - 1]);
+  return at;
+}
+
+// Rows of the transpose of an n x n CSR, each sorted. Each thread takes a
+// range of rows, counts its entries per column in a histogram of its own and
+// writes them after those of the threads before it, so every transposed row
+// comes out ascending without a sort. When threads x columns would outgrow
+// the entries, one thread does it.
 void transpose_rows(const Csr& g, std::vector<u32>& tp, std::vector<u32>& ti) {
   const u32 n = g.n;
   tp.assign(size_t(n) + 1, 0);
-  for (uint64_t e = 0; e < g.m; ++e) tp[size_t(g.col[e]) + 1]++;
-  for (size_t i = 1; i < tp.size(); ++i) tp[i] += tp[i - 1];
-  ti.assign(g.m, 0);
-  std::vector<u32> pos(tp.begin(), tp.end() - 1);
-  for (u32 r = 0; r < n; ++r)
-    for (uint64_t e = g.off[r]; e < g.off[r + 1]; ++e) ti[pos[g.col[e]]++] = r;  // rows ascending
+  ti.resize(g.m);
+  const int T = std::max(1, omp_get_max_threads());
+  if (T == 1 || uint64_t(T) * n > 2 * g.m + (uint64_t(1) << 24)) {
+    for (uint64_t e = 0; e < g.m; ++e) tp[size_t(g.col[e]) + 1]++;
+    for (size_t i = 1; i < tp.size(); ++i) tp[i] += tp[i - 1];
+    std::vector<u32> pos(tp.begin(), tp.end() - 1);
+    for (u32 r = 0; r < n; ++r)
+      for (uint64_t e = g.off[r]; e < g.off[r + 1]; ++e) ti[pos[g.col[e]]++] = r;
+    return;
+  }
+  const std::vector<u32> at = entry_split(g, T);
+  std::vector<u32> hist(size_t(T) * n, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    u32* h = hist.data() + size_t(t) * n;
+    for (uint64_t e = g.off[at[t]]; e < g.off[at[t + 1]]; ++e) ++h[g.col[e]];
+#pragma omp barrier
+#pragma omp for schedule(static)
+    for (int64_t c = 0; c < int64_t(n); ++c) {
+      u32 k = 0;
+      for (int v = 0; v < T; ++v) k += hist[size_t(v) * n + size_t(c)];
+      tp[size_t(c) + 1] = k;
+    }
+#pragma omp single
+    for (size_t i = 1; i < tp.size(); ++i) tp[i] += tp[i - 1];
+#pragma omp for schedule(static)
+    for (int64_t c = 0; c < int64_t(n); ++c) {
+      u32 slot = tp[size_t(c)];
+      for (int v = 0; v < T; ++v) {
+        u32& x = hist[size_t(v) * n + size_t(c)];
+        const u32 k = x;
+        x = slot;
+        slot += k;
+      }
+    }
+    for (u32 r = at[t]; r < at[t + 1]; ++r)
+      for (uint64_t e = g.off[r]; e < g.off[r + 1]; ++e) ti[h[g.col[e]]++] = r;
+  }
 }
 
 // refine_a_cut.hpp:43-73: per row, the largest count of its entries inside
@@ -139,19 +189,31 @@ std::vector<u32> rac_refine(u32 n, const Off* off, const u32* col, const std::ve
 }
 
 // Matrix::compute_maxload(c, c) (csr_matrix.hpp:281-322): the largest tile
-// entry count with the same cuts on rows and columns.
+// entry count with the same cuts on rows and columns. Threads count ranges of
+// rows into tile tables of their own, summed at the end (one table when
+// threads x tiles would be large).
 u32 maxload(const Csr& g, const std::vector<u32>& c) {
   const size_t B = c.size() - 1;
-  std::vector<u32> load(B * B, 0);
-  size_t u = 0;
-  for (u32 i = 0; i < g.n; ++i) {
-    while (u < B && i >= c[u]) ++u;
-    for (uint64_t e = g.off[i]; e < g.off[i + 1]; ++e) {
-      const size_t q = size_t(std::lower_bound(c.begin(), c.end(), g.col[e] + 1) - c.begin()) - 1;
-      load[(u - 1) * B + q] += 1;
+  const int T = B * B * size_t(omp_get_max_threads()) <= (size_t(1) << 24) ? omp_get_max_threads() : 1;
+  const std::vector<u32> at = entry_split(g, T);
+  std::vector<u32> load(size_t(T) * B * B, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    u32* mine = load.data() + size_t(t) * B * B;
+    // u: the number of cuts c[0..B) at or below row i (the row band is u - 1)
+    size_t u = size_t(std::upper_bound(c.begin(), c.begin() + B, at[t]) - c.begin());
+    for (u32 i = at[t]; i < at[t + 1]; ++i) {
+      while (u < B && i >= c[u]) ++u;
+      for (uint64_t e = g.off[i]; e < g.off[i + 1]; ++e) {
+        const size_t q = size_t(std::lower_bound(c.begin(), c.end(), g.col[e] + 1) - c.begin()) - 1;
+        mine[(u - 1) * B + q] += 1;
+      }
     }
   }
-  return *std::max_element(load.begin(), load.end());
+  for (int t = 1; t < T; ++t)
+    for (size_t x = 0; x < B * B; ++x) load[x] += load[size_t(t) * B * B + x];
+  return *std::max_element(load.begin(), load.begin() + B * B);
 }
 
 }  // namespace
