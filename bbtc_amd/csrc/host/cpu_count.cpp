@@ -3,12 +3,14 @@
 //
 // Same task semantics as the reference CPU workers (bb_tc.cpp:118-139,
 // 184-197): workers claim tasks from the back of the list with an atomic
-// counter, here in units of 1,024 non-empty rows so that several workers share
-// a heavy task; a task whose tiles include one with average degree > 10000
-// uses a per-worker stamp table (cpu_dense_kernel.cpp:3-51, merge for rows
-// with d_ab < 4), every other task a sorted merge with early exit
-// (cpu_sparse_kernel.cpp:3-59). Rows come from the compact non-empty list of
-// the ab tile instead of a scan over the whole band.
+// counter, here in units of 1,024 non-empty rows so that several workers
+// share a heavy task; a task whose tiles include one with average degree
+// > 10000 uses a per-worker stamp table (cpu_dense_kernel.cpp:3-51, merge for
+// rows with d_ab < 4). Every other task intersects with the reference's
+// sorted merge with early exit (cpu_sparse_kernel.cpp:3-59) for short ac rows,
+// and marks a long ac row once in a per-worker table to probe its bc rows
+// against; the per-task counts are the same. Rows come from the compact
+// non-empty list of the ab tile instead of a scan over the whole band.
 #include <omp.h>
 
 #include <algorithm>
@@ -51,16 +53,49 @@ TaskView view(const Layout& L, const bbtc_task& x) {
                   L.nzrows.data() + L.nz_base[ab], L.nz_rows[ab]};
 }
 
-// Rows [z0, z1) of the ab tile's non-empty row list.
-uint64_t sparse_rows(const TaskView& t, uint32_t z0, uint32_t z1) {
+// Rows [z0, z1) of the ab tile's non-empty row list. An ac row of at least
+// kMarkMin entries, met by two or more ab entries, is marked once in the
+// worker's epoch table, and each bc row is then probed against the marks
+// (one look-up per bc entry instead of a merge through the ac row for every
+// wedge); a bc row more than kMergeRatio times longer than the ac row, and
+// every row of a short ac row, is merged with early exit as the reference
+// does. Both give the number of common entries.
+constexpr uint32_t kMarkMin = 16;
+constexpr uint32_t kMergeRatio = 4;
+
+uint64_t sparse_rows(const TaskView& t, uint32_t z0, uint32_t z1, CpuWorker& w) {
   uint64_t c = 0;
+  uint32_t* mark = w.mark.data();
   for (uint32_t z = z0; z < z1; ++z) {
     const uint32_t r = t.nz[z];
     const uint32_t a0 = t.ac_rp[r], a1 = t.ac_rp[r + 1];
     if (a0 == a1) continue;
-    for (uint32_t y = t.ab_rp[r]; y < t.ab_rp[r + 1]; ++y) {
+    const uint32_t na = a1 - a0;
+    const uint32_t b0 = t.ab_rp[r], b1 = t.ab_rp[r + 1];
+    if (na < kMarkMin || b1 - b0 < 2) {
+      for (uint32_t y = b0; y < b1; ++y) {
+        const uint32_t v = t.ab_c[y];
+        c += merge_count(t.ac_c + a0, na, t.bc_c + t.bc_rp[v], t.bc_rp[v + 1] - t.bc_rp[v]);
+      }
+      continue;
+    }
+    if (w.epoch == 0xFFFFFFFFu) {  // every epoch used: clear and start again
+      std::fill(w.mark.begin(), w.mark.end(), 0u);
+      w.epoch = 0;
+    }
+    const uint32_t e = ++w.epoch;
+    for (uint32_t x = a0; x < a1; ++x) mark[t.ac_c[x]] = e;
+    for (uint32_t y = b0; y < b1; ++y) {
       const uint32_t v = t.ab_c[y];
-      c += merge_count(t.ac_c + a0, a1 - a0, t.bc_c + t.bc_rp[v], t.bc_rp[v + 1] - t.bc_rp[v]);
+      const uint32_t* bc = t.bc_c + t.bc_rp[v];
+      const uint32_t nb = t.bc_rp[v + 1] - t.bc_rp[v];
+      if (nb > kMergeRatio * na) {
+        c += merge_count(t.ac_c + a0, na, bc, nb);
+        continue;
+      }
+      uint32_t k = 0;
+      for (uint32_t x = 0; x < nb; ++x) k += mark[bc[x]] == e;
+      c += k;
     }
   }
   return c;
@@ -102,17 +137,28 @@ void clear_stamp(std::vector<uint32_t>& stamp, uint32_t width) {
   std::fill(stamp.begin(), stamp.begin() + width, 0xFFFFFFFFu);
 }
 
+// The worker's tables ready for task x (columns of band k index them).
+void prepare(const Layout& L, const bbtc_task& x, bool dense, CpuWorker& w) {
+  const uint32_t width = L.height(x.k);
+  if (dense) {
+    clear_stamp(w.stamp, width);
+  } else if (w.mark.size() < width) {
+    w.mark.assign(width, 0u);
+    w.epoch = 0;
+  }
+}
+
 // Non-empty ab rows per claim unit of the CPU count.
 constexpr uint32_t kRowsPerUnit = 1024;
 
 }  // namespace
 
-uint64_t count_task_cpu(const Layout& L, const bbtc_task& x, std::vector<uint32_t>& stamp) {
+uint64_t count_task_cpu(const Layout& L, const bbtc_task& x, CpuWorker& w) {
   if (task_skipped(L, x)) return 0;
   const TaskView v = view(L, x);
-  if (!dense_task(L, x)) return sparse_rows(v, 0, v.nnz_rows);
-  clear_stamp(stamp, L.height(x.k));
-  return dense_rows(v, 0, v.nnz_rows, stamp.data());
+  const bool dense = dense_task(L, x);
+  prepare(L, x, dense, w);
+  return dense ? dense_rows(v, 0, v.nnz_rows, w.stamp.data()) : sparse_rows(v, 0, v.nnz_rows, w);
 }
 
 // The workers claim units from one counter: the tasks from the back of the
@@ -137,7 +183,7 @@ uint64_t count_cpu(const Layout& L, const bbtc_task* tasks, uint64_t n, int nthr
   const int T = nthreads > 0 ? nthreads : omp_get_max_threads();
 #pragma omp parallel num_threads(T)
   {
-    std::vector<uint32_t> stamp;
+    CpuWorker w;
     uint64_t mine = 0, acc = 0;
     uint64_t s = 0, cur = n;  // cur: the task whose rows acc holds (n: none)
     TaskView v{};
@@ -159,13 +205,13 @@ uint64_t count_cpu(const Layout& L, const bbtc_task* tasks, uint64_t n, int nthr
         if (!skip) {
           v = view(L, x);
           dense = dense_task(L, x);
-          if (dense) clear_stamp(stamp, L.height(x.k));
+          prepare(L, x, dense, w);
         }
       }
       if (skip) continue;
       const uint64_t z0 = (u - first[s]) * kRowsPerUnit;
       const uint32_t z1 = uint32_t(std::min<uint64_t>(z0 + kRowsPerUnit, v.nnz_rows));
-      acc += dense ? dense_rows(v, uint32_t(z0), z1, stamp.data()) : sparse_rows(v, uint32_t(z0), z1);
+      acc += dense ? dense_rows(v, uint32_t(z0), z1, w.stamp.data()) : sparse_rows(v, uint32_t(z0), z1, w);
     }
     flush();
     total.fetch_add(mine);

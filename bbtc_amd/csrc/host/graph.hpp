@@ -96,7 +96,17 @@ bool task_skipped(const Layout& L, const bbtc_task& t);
 // ---- CPU count ------------------------------------------------------------
 uint64_t count_cpu(const Layout& L, const bbtc_task* t, uint64_t n, int nthreads,
                    uint64_t* per_task);
-// One task; `stamp` is the worker's reusable stamp table for dense tasks.
-uint64_t count_task_cpu(const Layout& L, const bbtc_task& t, std::vector<uint32_t>& stamp);
+// A CPU worker's tables, kept from task to task. Dense tasks: stamp[c] == r
+// marks column c of ac row r (the table is cleared for each task). Sparse
+// tasks: mark[c] == epoch marks column c of the current long ac row; every
+// such row takes a new epoch, so the table is cleared only when the epoch
+// counter wraps or the table grows.
+struct CpuWorker {
+  std::vector<uint32_t> stamp;
+  std::vector<uint32_t> mark;
+  uint32_t epoch = 0;
+};
+// One task with the worker's tables.
+uint64_t count_task_cpu(const Layout& L, const bbtc_task& t, CpuWorker& w);
 
 }  // namespace bbtc

@@ -97,7 +97,7 @@ HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bb
         state.store(pack(0, 0));  // stop the host workers too
       }
     } else {
-      std::vector<uint32_t> stamp;
+      CpuWorker w;
       for (;;) {
         uint64_t cur = state.load();
         uint64_t q = 0;
@@ -109,7 +109,7 @@ HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bb
           if (state.compare_exchange_weak(cur, pack(f, q))) { got = true; break; }
         }
         if (!got) break;
-        cpu_counts[q] = count_task_cpu(L, tasks[q], stamp);
+        cpu_counts[q] = count_task_cpu(L, tasks[q], w);
       }
     }
   }
