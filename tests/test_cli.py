@@ -120,3 +120,24 @@ def test_cpu_path_task_order(c1, tmp_path):
     got = np.array([list(map(int, ln.split()[:3])) for ln in out.read_text().splitlines()], dtype=np.uint32)
     L = B.Layout(graph_from_spec(d["graph"]), np.array(r["cuts"], dtype=np.uint32))
     assert np.array_equal(B.tasks_array(got), L.tasks("asc"))
+
+
+def test_random_arguments_never_crash(c1, tmp_path):
+    # seeded random flag/value mixes (the GPU left out): every run ends with a
+    # result (0) or the reference's message and usage (2), never a signal
+    import random
+    _, path = c1
+    flags = ["--graph", "--partitioner", "--ncut", "--nstream", "--border", "--copy-first", "--repeat", "--help",
+             "--kernel", "--threads", "--cuts-file", "--per-task-out", "--batch", "--serve", "--save-bcsr",
+             "--keep-duplicates", "--bogus", "-x", ""]
+    vals = [path, str(tmp_path / "nope.bbtc"), "pal", "uni", "n1d", "rac", "xyz", "0", "1", "-1", "-2", "2", "4",
+            "99999999999", "abc", "", "1e3", "0x10", "-0", "4294967296", "lane", "tuned", str(tmp_path / "o.txt")]
+    rng = random.Random(5)
+    for _ in range(120):
+        args = []
+        for _ in range(rng.randint(0, 8)):
+            args.append(rng.choice(flags))
+            if rng.random() < 0.8:
+                args.append(rng.choice(vals))
+        p = run(args + ["--ngpu", "0"])
+        assert p.returncode in (0, 2), (args, p.returncode, p.stderr[-300:])
