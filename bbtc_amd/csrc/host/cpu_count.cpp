@@ -6,11 +6,12 @@
 // counter, here in units of 1,024 non-empty rows so that several workers
 // share a heavy task; a task whose tiles include one with average degree
 // > 10000 uses a per-worker stamp table (cpu_dense_kernel.cpp:3-51, merge for
-// rows with d_ab < 4). Every other task intersects with the reference's
-// sorted merge with early exit (cpu_sparse_kernel.cpp:3-59) for short ac rows,
-// and marks a long ac row once in a per-worker table to probe its bc rows
-// against; the per-task counts are the same. Rows come from the compact
-// non-empty list of the ab tile instead of a scan over the whole band.
+// rows with d_ab < 4). Every other task marks each ac row met by two or more
+// ab entries once in a per-worker table and probes its bc rows against it;
+// the rest take the reference's sorted merge with early exit
+// (cpu_sparse_kernel.cpp:3-59). The per-task counts are the same. Rows come
+// from the compact non-empty list of the ab tile instead of a scan over the
+// whole band.
 #include <omp.h>
 
 #include <algorithm>
@@ -53,15 +54,14 @@ TaskView view(const Layout& L, const bbtc_task& x) {
                   L.nzrows.data() + L.nz_base[ab], L.nz_rows[ab]};
 }
 
-// Rows [z0, z1) of the ab tile's non-empty row list. An ac row of at least
-// kMarkMin entries, met by two or more ab entries, is marked once in the
-// worker's epoch table, and each bc row is then probed against the marks
-// (one look-up per bc entry instead of a merge through the ac row for every
-// wedge); a bc row more than kMergeRatio times longer than the ac row, and
-// every row of a short ac row, is merged with early exit as the reference
-// does. Both give the number of common entries.
-constexpr uint32_t kMarkMin = 16;
-constexpr uint32_t kMergeRatio = 4;
+// Rows [z0, z1) of the ab tile's non-empty row list. An ac row met by two or
+// more ab entries is marked once in the worker's epoch table, and each bc row
+// is then probed against the marks (one look-up per bc entry instead of a
+// merge through the ac row for every wedge); a bc row more than kMergeRatio
+// times longer than the ac row, and the one wedge of a row with a single ab
+// entry, is merged with early exit as the reference does. Both give the
+// number of common entries.
+constexpr uint32_t kMergeRatio = 128;
 
 uint64_t sparse_rows(const TaskView& t, uint32_t z0, uint32_t z1, CpuWorker& w) {
   uint64_t c = 0;
@@ -72,7 +72,7 @@ uint64_t sparse_rows(const TaskView& t, uint32_t z0, uint32_t z1, CpuWorker& w) 
     if (a0 == a1) continue;
     const uint32_t na = a1 - a0;
     const uint32_t b0 = t.ab_rp[r], b1 = t.ab_rp[r + 1];
-    if (na < kMarkMin || b1 - b0 < 2) {
+    if (b1 - b0 < 2) {
       for (uint32_t y = b0; y < b1; ++y) {
         const uint32_t v = t.ab_c[y];
         c += merge_count(t.ac_c + a0, na, t.bc_c + t.bc_rp[v], t.bc_rp[v + 1] - t.bc_rp[v]);

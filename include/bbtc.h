@@ -190,7 +190,7 @@ bbtc_status bbtc_tasks_bytes(const bbtc_layout* L, const bbtc_task* tasks, uint6
  * (bb_tc.cpp:118-139, 184-197; cpu_sparse_kernel.cpp; cpu_dense_kernel.cpp):
  * workers claim tasks from the back of the list, in units of 1,024
  * non-empty rows so that all of them share a heavy task (pass the tasks in
- * ascending weight, BBTC_ORDER_ASC, to start with the heaviest); long ac
+ * ascending weight, BBTC_ORDER_ASC, to start with the heaviest); the ac
  * rows of sparse tasks are marked once and probed rather than merged per
  * wedge, with the same counts. nthreads <= 0 means all OpenMP threads.
  * per_task (may be NULL) receives one count per task, the count of the whole

@@ -174,3 +174,18 @@ def test_dense_task_split_over_workers():
     R = O.RefLayout(g.offsets(), g.cols(), np.array([0, apex, n + 1], np.uint32))
     rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
     assert rtot == expect and np.array_equal(rper, per1)
+
+
+def test_long_bc_row_merges():
+    # vertex 0 joins 1 and 2; vertex 1 joins 2..301. With one band, the ac row
+    # of 0 holds 2 entries and the bc row of 1 holds 300, more than 128 times
+    # as many, so that wedge is merged instead of probed against the marks;
+    # the one triangle is (0, 1, 2)
+    rows = [[1, 2], list(range(2, 302))] + [[] for _ in range(300)]
+    off = np.cumsum([0] + [len(r) for r in rows]).astype(np.uint64)
+    cols = np.array([c for r in rows for c in r], dtype=np.uint32)
+    g = B.Graph.from_csr(off, cols)
+    assert O.brute_force(g.offsets(), g.cols()) == 1
+    for part, p in (("uni", 1), ("uni", 3)):
+        _, _, _, per, tot = count_all(g, part, p)
+        assert tot == 1 == int(per.sum())
