@@ -159,60 +159,90 @@ bool ends_with(const std::string& s, const char* suf) {
   return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
 }
 
-// Rows of a matrix with n rows from (row, col) pairs, each row sorted. Each
-// thread counts its slice of the pairs into a histogram of its own and then
-// scatters that slice to the slots the prefix of all histograms gives it, so
-// no slot is contended; when threads x rows would outgrow the pairs (an id
-// space far sparser than the edges) the slots are claimed with atomics
-// instead. The order inside a row before its sort does not matter.
+// Rows of a matrix with n rows from (row, col) pairs, each row sorted: the
+// slots are claimed with atomic counters (for id spaces too sparse for
+// per-thread histograms), so each row is sorted afterwards.
 void rows_of(uint32_t n, const std::vector<uint32_t>& row, const std::vector<uint32_t>& col,
              std::vector<uint32_t>& ptr, std::vector<uint32_t>& idx) {
-  const int T = std::max(1, omp_get_max_threads());
   const size_t m = row.size();
   ptr.assign(size_t(n) + 1, 0);
   idx.resize(m);
-  if (size_t(T) * n <= 2 * m + (size_t(1) << 24)) {
-    std::vector<uint32_t> hist(size_t(T) * n, 0);
-#pragma omp parallel num_threads(T)
-    {
-      const int t = omp_get_thread_num();
-      uint32_t* h = hist.data() + size_t(t) * n;
-      const size_t q0 = m * size_t(t) / size_t(T), q1 = m * size_t(t + 1) / size_t(T);
-      for (size_t q = q0; q < q1; ++q) ++h[row[q]];
-#pragma omp barrier
-#pragma omp for schedule(static)
-      for (int64_t i = 0; i < int64_t(n); ++i) {
-        uint32_t k = 0;
-        for (int u = 0; u < T; ++u) k += hist[size_t(u) * n + size_t(i)];
-        ptr[size_t(i) + 1] = k;
-      }
-#pragma omp single
-      for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
-#pragma omp for schedule(static)
-      for (int64_t i = 0; i < int64_t(n); ++i) {
-        uint32_t at = ptr[size_t(i)];
-        for (int u = 0; u < T; ++u) {
-          uint32_t& c = hist[size_t(u) * n + size_t(i)];
-          const uint32_t k = c;
-          c = at;
-          at += k;
-        }
-      }
-      for (size_t q = q0; q < q1; ++q) idx[h[row[q]]++] = col[q];
-    }
-  } else {
-    uint32_t* cnt = ptr.data() + 1;
+  uint32_t* cnt = ptr.data() + 1;
 #pragma omp parallel for schedule(static, 1 << 16)
-    for (int64_t q = 0; q < int64_t(m); ++q) __atomic_fetch_add(&cnt[row[q]], 1u, __ATOMIC_RELAXED);
-    for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
-    std::vector<uint32_t> pos(ptr.begin(), ptr.end() - 1);
-    uint32_t* at = pos.data();
+  for (int64_t q = 0; q < int64_t(m); ++q) __atomic_fetch_add(&cnt[row[q]], 1u, __ATOMIC_RELAXED);
+  for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
+  std::vector<uint32_t> pos(ptr.begin(), ptr.end() - 1);
+  uint32_t* at = pos.data();
 #pragma omp parallel for schedule(static, 1 << 16)
-    for (int64_t q = 0; q < int64_t(m); ++q) idx[__atomic_fetch_add(&at[row[q]], 1u, __ATOMIC_RELAXED)] = col[q];
-  }
+  for (int64_t q = 0; q < int64_t(m); ++q) idx[__atomic_fetch_add(&at[row[q]], 1u, __ATOMIC_RELAXED)] = col[q];
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < int64_t(n); ++i) std::sort(idx.begin() + ptr[i], idx.begin() + ptr[i + 1]);
 }
+
+// Whether per-thread histograms over n keys stay within the memory of the
+// m pairs they place (a much sparser id space takes rows_of's atomics).
+bool histograms_fit(uint32_t n, size_t m, int T) {
+  return size_t(T) * n <= 2 * m + (size_t(1) << 24);
+}
+
+// Rows over n keys from m (key, value) pairs in a fixed order, each row
+// keeping that order (a stable counting scatter). The pairs come in T
+// consecutive parts; part(t, f) calls f(key, value) for the pairs of part t
+// in order. Thread t counts its part into a histogram of its own and places
+// it after the parts before it.
+template <class Part>
+void stable_rows(uint32_t n, size_t m, int T, Part part, std::vector<uint32_t>& ptr,
+                 std::vector<uint32_t>& idx) {
+  ptr.assign(size_t(n) + 1, 0);
+  idx.resize(m);
+  std::vector<uint32_t> hist(size_t(T) * n, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    uint32_t* h = hist.data() + size_t(t) * n;
+    part(t, [&](uint32_t key, uint32_t) { ++h[key]; });
+#pragma omp barrier
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+      uint32_t k = 0;
+      for (int u = 0; u < T; ++u) k += hist[size_t(u) * n + size_t(i)];
+      ptr[size_t(i) + 1] = k;
+    }
+#pragma omp single
+    for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+      uint32_t at = ptr[size_t(i)];
+      for (int u = 0; u < T; ++u) {
+        uint32_t& c = hist[size_t(u) * n + size_t(i)];
+        const uint32_t k = c;
+        c = at;
+        at += k;
+      }
+    }
+    part(t, [&](uint32_t key, uint32_t val) { idx[h[key]++] = val; });
+  }
+}
+
+// The pairs of a CSR in row order (key: the entry, value: its row), cut into
+// T parts of about equal entries.
+struct CsrPairs {
+  const std::vector<uint32_t>& ptr;
+  const std::vector<uint32_t>& idx;
+  std::vector<uint32_t> at;  // T + 1 row bounds
+  CsrPairs(const std::vector<uint32_t>& p, const std::vector<uint32_t>& i, int T) : ptr(p), idx(i), at(size_t(T) + 1) {
+    const uint32_t n = uint32_t(ptr.size() - 1);
+    for (int t = 0; t <= T; ++t) {
+      const uint64_t want = uint64_t(idx.size()) * uint64_t(t) / uint64_t(T);
+      at[size_t(t)] = t == T ? n : uint32_t(std::lower_bound(ptr.begin(), ptr.end() - 1, want) - ptr.begin());
+    }
+  }
+  template <class F>
+  void operator()(int t, F f) const {
+    for (uint32_t r = at[size_t(t)]; r < at[size_t(t) + 1]; ++r)
+      for (uint32_t q = ptr[r]; q < ptr[size_t(r) + 1]; ++q) f(idx[q], r);
+  }
+};
 
 }  // namespace
 
@@ -236,11 +266,26 @@ Csr csr_from_text(const std::string& path, bool dedup) {
   const uint32_t n = std::max(c.nrows, c.ncols);
   if (c.x.size() >= (1ull << 32)) fail(BBTC_ERR_FORMAT, "more than 2^32-1 entries");
 
-  // A and A^T as sorted rows over n x n
+  // A and A^T as sorted rows over n x n. With histograms: three stable
+  // scatters, by y, then by x, then by y, sort the rows of both (a two-key
+  // radix sort); otherwise each row is sorted after an atomic scatter.
   std::vector<uint32_t> ap, ai, tp, ti;
-  rows_of(n, c.x, c.y, ap, ai);
-  rows_of(n, c.y, c.x, tp, ti);
-  c = Coo{};
+  const int T = std::max(1, omp_get_max_threads());
+  const size_t m = c.x.size();
+  if (histograms_fit(n, m, T)) {
+    auto coo = [&](int t, auto f) {
+      const size_t q0 = m * size_t(t) / size_t(T), q1 = m * size_t(t + 1) / size_t(T);
+      for (size_t q = q0; q < q1; ++q) f(c.y[q], c.x[q]);
+    };
+    stable_rows(n, m, T, coo, tp, ti);  // rows of y in input order
+    c = Coo{};
+    stable_rows(n, m, T, CsrPairs(tp, ti, T), ap, ai);  // A: rows of x, y ascending
+    stable_rows(n, m, T, CsrPairs(ap, ai, T), tp, ti);  // A^T: rows of y, x ascending
+  } else {
+    rows_of(n, c.x, c.y, ap, ai);
+    rows_of(n, c.y, c.x, tp, ti);
+    c = Coo{};
+  }
 
   // symmetric merge without the diagonal
   std::vector<uint32_t> sp(size_t(n) + 1, 0);
