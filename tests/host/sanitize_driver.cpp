@@ -1,7 +1,8 @@
 // Host-side C-ABI exercise for the sanitizer test (tests/test_host_sanitizers.py):
 // generate, save and load a graph, partition it with every partitioner, build
 // layouts, count on the CPU, compute algorithmic bytes, round-trip .bcsr files,
-// convert a text edge list, and take an error path; everything is released, so
+// convert text edge lists (a dense and a sparse id space), count a wedge
+// that merges a long bc row, and take an error path; everything is released, so
 // AddressSanitizer's leak check sees no leak.
 #include <cstdio>
 #include <cstdlib>
@@ -50,6 +51,31 @@ int main(int argc, char** argv) {
   bbtc_hgraph* gt = nullptr;
   CK(bbtc_hgraph_load_text(tp.c_str(), &gt));
   bbtc_hgraph_free(gt);
+  // an id space far sparser than the edges (rows built with atomic slots)
+  f = std::fopen(tp.c_str(), "w");
+  std::fprintf(f, "0 5000000\n5000000 7\n7 0\n");
+  std::fclose(f);
+  CK(bbtc_hgraph_load_text(tp.c_str(), &gt));
+  bbtc_hgraph_free(gt);
+  // one ac row of 2 entries against a bc row of 300 (merged, not probed)
+  {
+    std::vector<uint64_t> off = {0, 2, 302};
+    std::vector<uint32_t> col = {1, 2};
+    for (uint32_t c = 2; c < 302; ++c) col.push_back(c);
+    for (int i = 0; i < 300; ++i) off.push_back(302);
+    bbtc_hgraph* gl = nullptr;
+    CK(bbtc_hgraph_from_csr(302, 302, col.size(), off.data(), col.data(), &gl));
+    uint32_t cuts[2];
+    CK(bbtc_partition(gl, BBTC_PART_UNI, 1, cuts));
+    bbtc_layout* L = nullptr;
+    CK(bbtc_layout_build(gl, cuts, 1, &L));
+    bbtc_task t{0, 0, 0};
+    uint64_t per = 0, tot = 0;
+    CK(bbtc_count_cpu(L, &t, 1, 2, &per, &tot));
+    if (tot != 1) return 3;
+    bbtc_layout_free(L);
+    bbtc_hgraph_free(gl);
+  }
   // error paths
   bbtc_hgraph* bad = nullptr;
   if (bbtc_hgraph_load("/nonexistent.bbtc", &bad) == BBTC_OK) return 2;
