@@ -86,3 +86,18 @@ def test_pal_balances_load():
     uloads = np.zeros((p, p), dtype=np.int64)
     np.add.at(uloads, (ub, uk), 1)
     assert loads.max() < uloads.max()
+
+
+@pytest.mark.parametrize("n", [3000, 3_000_000])
+def test_rac_parallel_and_serial_transpose(n):
+    # rac's transpose uses per-thread histograms unless threads x rows outgrow
+    # the entries; a few thousand edges over 3M ids take the one-thread path.
+    # Both give the cuts of the oracle's restatement.
+    rng = np.random.default_rng(n)
+    ids = np.sort(rng.choice(n, 400, replace=False)).astype(np.uint32)
+    a, b = ids[rng.integers(0, 400, 3000)], ids[rng.integers(0, 400, 3000)]
+    keep = a != b
+    g = B.Graph.from_edges(n, a[keep], b[keep])
+    for p in (2, 5, 16):
+        got = g.partition("rac", p)
+        assert got.tolist() == O.cuts("rac", g.offsets(), g.cols(), p).tolist(), p
