@@ -89,7 +89,7 @@ uint64_t sparse_rows(const TaskView& t, uint32_t z0, uint32_t z1, CpuWorker& w) 
       const uint32_t v = t.ab_c[y];
       const uint32_t* bc = t.bc_c + t.bc_rp[v];
       const uint32_t nb = t.bc_rp[v + 1] - t.bc_rp[v];
-      if (nb > kMergeRatio * na) {
+      if (uint64_t(nb) > uint64_t(kMergeRatio) * na) {
         c += merge_count(t.ac_c + a0, na, bc, nb);
         continue;
       }
