@@ -346,6 +346,9 @@ class Layout:
         return per, tot.value, wed.value
 
     def count_cpu(self, tasks, nthreads: int = 0):
+        """Per-task and total counts on the host (the --ngpu 0 path). Workers
+        claim from the back of `tasks`, so pass tasks("asc") to start with the
+        heaviest; nthreads 0 means all OpenMP threads."""
         t = tasks_array(tasks)
         per = np.zeros(len(t), dtype=np.uint64)
         tot = C.c_uint64()
