@@ -15,7 +15,8 @@
 //     "HIP Initial Memory Allocations and Stream Creation";
 //   * extra flags: --kernel auto|merge|bitmap|dense|rdense|lane|tuned, --threads n, --cuts-file path,
 //     --per-task-out path, --keep-duplicates, --save-bcsr path (a .bcsr
-//     --graph is read back with its own cuts);
+//     --graph is read back with its own cuts), --forms-out / --forms-in path
+//     (the autotuned per-task forms saved for, and reused by, later runs);
 //   * extra modes (the reference counts one graph per run): --batch <list>
 //     counts every graph a list file names in one batched launch, and
 //     --serve <list> runs them as requests (with arrival times) through the
@@ -54,6 +55,7 @@ struct Params {
   std::string per_task_out;
   bool keep_duplicates = false;
   std::string save_bcsr;
+  std::string forms_out, forms_in;  // autotuned forms written / reused
   bool ncut_given = false;
   std::string batch_list, serve_list;  // list files of the extra modes
 };
@@ -86,6 +88,9 @@ void usage() {
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
             << "  --keep-duplicates     Keep repeated edges when converting text input\n"
             << "  --save-bcsr <path>    Write the block layout (.bcsr) for later runs\n"
+            << "  --forms-out <path>    With --kernel tuned: write the measured per-task forms\n"
+            << "  --forms-in <path>     Count with per-task forms written by --forms-out for\n"
+            << "                          the same cuts (tuned speed, no autotuning)\n"
             << "  --batch <list>        Count every graph listed ('path [ncut]' per line) in one\n"
             << "                          batched launch; without an ncut column or --ncut the\n"
             << "                          cut count per graph comes from measured sample graphs\n"
@@ -173,6 +178,8 @@ Params parse(int argc, char** argv) {
     else if (!strcasecmp(a, "--per-task-out")) p.per_task_out = val("--per-task-out");
     else if (!strcasecmp(a, "--keep-duplicates")) p.keep_duplicates = true;
     else if (!strcasecmp(a, "--save-bcsr")) p.save_bcsr = val("--save-bcsr");
+    else if (!strcasecmp(a, "--forms-out")) p.forms_out = val("--forms-out");
+    else if (!strcasecmp(a, "--forms-in")) p.forms_in = val("--forms-in");
     else if (!strcasecmp(a, "--batch")) p.batch_list = val("--batch");
     else if (!strcasecmp(a, "--serve")) p.serve_list = val("--serve");
     else if (!strcasecmp(a, "--help")) {
@@ -181,6 +188,15 @@ Params parse(int argc, char** argv) {
     } else bad("Wrong argument '" + std::string(a) + "' given.");
   }
   if (!p.batch_list.empty() && !p.serve_list.empty()) bad("--batch and --serve are exclusive");
+  if (!p.forms_out.empty() || !p.forms_in.empty()) {
+    if (!p.batch_list.empty() || !p.serve_list.empty()) bad("--forms-out and --forms-in count one graph");
+    if (p.ngpu == 0) bad("--forms-out and --forms-in need a GPU (--ngpu 1)");
+    if (!p.forms_out.empty() && p.kernel != "tuned") bad("--forms-out needs --kernel tuned");
+    if (!p.forms_in.empty() && p.kernel != "auto" && p.kernel != "tuned")
+      bad("--forms-in sets the forms; --kernel must be auto or tuned");
+    if (!p.forms_in.empty() && !p.forms_out.empty()) bad("--forms-out and --forms-in are exclusive");
+    if (!p.forms_in.empty() && !file_exists(p.forms_in)) bad("Forms file doesn't exist");
+  }
   const std::string& list = !p.batch_list.empty() ? p.batch_list : p.serve_list;
   if (!list.empty()) {
     if (!file_exists(list)) bad("List file doesn't exist");
@@ -213,6 +229,49 @@ struct Fields {
 
 bool ends_with(const std::string& s, const std::string& t) {
   return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
+}
+
+// Forms file: "bbtc-forms 1", then "<ncut> <ntasks>", the ncut+1 cuts, and
+// one form per task in task order (1 merge, 2 bitmap, 3 dense, 4 rdense,
+// 5 lane, 0 the default rule), all as text. The cuts tie it to a layout.
+void write_forms(const std::string& path, const bbtc_layout* L, const std::vector<uint8_t>& choice,
+                 uint64_t ntasks) {
+  const uint32_t p = bbtc_layout_ncut(L);
+  const uint32_t* cuts = bbtc_layout_cuts(L);
+  std::ofstream o(path);
+  o << "bbtc-forms 1\n" << p << " " << ntasks << "\n";
+  for (uint32_t b = 0; b <= p; ++b) o << cuts[b] << (b == p ? "\n" : " ");
+  for (uint64_t q = 0; q < ntasks; ++q) o << int(choice[q]) << (q + 1 == ntasks ? "\n" : " ");
+  if (!o) {
+    std::cerr << "bbtc: cannot write " << path << std::endl;
+    std::exit(1);
+  }
+}
+
+std::vector<uint8_t> read_forms(const std::string& path, const bbtc_layout* L, uint64_t ntasks) {
+  auto fail = [&](const std::string& why) {
+    std::cerr << "bbtc: forms file " << path << ": " << why << std::endl;
+    std::exit(1);
+  };
+  std::ifstream f(path);
+  std::string magic;
+  int version = 0;
+  uint64_t p = 0, nt = 0;
+  if (!(f >> magic >> version) || magic != "bbtc-forms" || version != 1) fail("not a bbtc forms file");
+  if (!(f >> p >> nt)) fail("truncated header");
+  if (p != bbtc_layout_ncut(L) || nt != ntasks) fail("written for another number of cuts or tasks");
+  const uint32_t* cuts = bbtc_layout_cuts(L);
+  for (uint64_t b = 0; b <= p; ++b) {
+    uint64_t c = 0;
+    if (!(f >> c) || c != cuts[b]) fail("written for other cuts");
+  }
+  std::vector<uint8_t> choice(std::max<uint64_t>(ntasks, 1), 0);
+  for (uint64_t q = 0; q < ntasks; ++q) {
+    int c = -1;
+    if (!(f >> c) || c < 0 || c > 5) fail("bad or missing form for task " + std::to_string(q));
+    choice[q] = uint8_t(c);
+  }
+  return choice;
 }
 
 bbtc_part part_kind(const std::string& s) {
@@ -561,7 +620,9 @@ int main(int argc, char** argv) {
   std::vector<uint64_t> per(ntasks, 0);
   uint64_t total = 0;
   bbtc_dgraph* dg = nullptr;
-  if (ctx && P.kernel == "tuned") {
+  if (ctx && !P.forms_in.empty()) {
+    choice = read_forms(P.forms_in, L, ntasks);
+  } else if (ctx && P.kernel == "tuned") {
     F.begin("Autotune kernel forms (s)");
     st = bbtc_graph_upload(ctx, L, &dg);
     if (st) die(st, "uploading the graph");
@@ -569,6 +630,7 @@ int main(int argc, char** argv) {
     st = bbtc_autotune(ctx, dg, tasks.data(), ntasks, choice.data(), nullptr);
     if (st) die(st, "autotuning");
     F.end();
+    if (!P.forms_out.empty()) write_forms(P.forms_out, L, choice, ntasks);
   }
   for (int r = 0; r < P.repeat; ++r) {
     if (dg) {  // as ResetFlags (bb_tc_helper.cpp:31-50), every repeat starts without device tiles

@@ -141,3 +141,13 @@ def test_random_arguments_never_crash(c1, tmp_path):
                 args.append(rng.choice(vals))
         p = run(args + ["--ngpu", "0"])
         assert p.returncode in (0, 2), (args, p.returncode, p.stderr[-300:])
+
+
+def test_forms_flags_validation(c1, tmp_path):
+    _, path = c1
+    f = tmp_path / "x.forms"
+    f.write_text("bbtc-forms 1\n")
+    for extra, msg in ((["--forms-out", str(f)], "need a GPU"), (["--forms-in", str(f)], "need a GPU"),
+                       (["--forms-in", str(f), "--batch", path], "count one graph")):
+        p = run(["--graph", path, "--ngpu", "0"] + extra)
+        assert p.returncode == 2 and msg in p.stdout, (extra, p.stdout[:200])

@@ -218,6 +218,34 @@ def test_cli_gpu(tmp_path):
             assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
 
 
+def test_cli_forms_out_and_in(tmp_path):
+    # the forms measured by --kernel tuned, written once and reused: the same
+    # per-task counts without autotuning; a file for other cuts is refused
+    d = load_golden("c1_rmat_s16_ef8")[0]
+    path = str(tmp_path / "c1.bbtc")
+    graph_from_spec(d["graph"]).save(path)
+    run = next(x for x in d["runs"] if x["partitioner"] == "pal")
+    forms = tmp_path / "c1.forms"
+    base = [B.CLI_PATH, "--graph", path, "--partitioner", "pal", "--ncut", str(run["ncut"]), "--repeat", "1",
+            "--ngpu", "1", "--border", "-1"]
+    per = {}
+    for name, extra in (("out", ["--kernel", "tuned", "--forms-out", str(forms)]), ("in", ["--forms-in", str(forms)])):
+        out = tmp_path / f"pt_{name}.txt"
+        r = subprocess.run(base + extra + ["--per-task-out", str(out)], capture_output=True, text=True, timeout=120)
+        log("cli forms", name, r.returncode)
+        assert r.returncode == 0, r.stderr
+        assert f"Number of Triangles: {run['total']}" in r.stdout
+        assert ("Autotune kernel forms" in r.stdout) == (name == "out")
+        per[name] = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
+    assert per["out"] == per["in"] == {tuple(x[:3]): x[3] for x in run["tasks"]}
+    words = forms.read_text().split()
+    assert words[:2] == ["bbtc-forms", "1"] and int(words[3]) == len(run["tasks"])
+    r = subprocess.run([B.CLI_PATH, "--graph", path, "--partitioner", "uni", "--ncut", str(run["ncut"]),
+                        "--repeat", "1", "--ngpu", "1", "--forms-in", str(forms)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 1 and "written for other cuts" in r.stderr
+
+
 def test_cli_batch_and_serve(tmp_path):
     # golden graphs (totals written by the reference's code) as list files
     small = [d for d in GOLDEN if d["fingerprint"]["m"] < 3_000_000][:5]
