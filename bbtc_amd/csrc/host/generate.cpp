@@ -42,23 +42,30 @@ struct Stream {
 template <class T>
 inline T fetch_add(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
 
-// Exclusive prefix sum of cnt[0..n) into out[0..n] (out[n] = total).
+// Exclusive prefix sum of cnt[0..n) into out[0..n] (out[n] = total), over
+// T parts; each thread of the team (which may be smaller than T) takes parts
+// tid, tid + team, ...
 template <class C>
 void exclusive_scan(const C* cnt, uint64_t* out, size_t n) {
-  const int T = omp_get_max_threads();
+  const int T = std::max(1, omp_get_max_threads());
   std::vector<uint64_t> part(size_t(T) + 1, 0);
 #pragma omp parallel num_threads(T)
   {
-    const int t = omp_get_thread_num();
-    const size_t b = n * t / T, e = n * (t + 1) / T;
-    uint64_t s = 0;
-    for (size_t i = b; i < e; ++i) s += cnt[i];
-    part[t + 1] = s;
+    const int team = omp_get_num_threads();
+    for (int t = omp_get_thread_num(); t < T; t += team) {
+      const size_t b = n * t / T, e = n * (t + 1) / T;
+      uint64_t s = 0;
+      for (size_t i = b; i < e; ++i) s += cnt[i];
+      part[t + 1] = s;
+    }
 #pragma omp barrier
 #pragma omp single
     for (int i = 0; i < T; ++i) part[i + 1] += part[i];
-    uint64_t acc = part[t];
-    for (size_t i = b; i < e; ++i) { out[i] = acc; acc += cnt[i]; }
+    for (int t = omp_get_thread_num(); t < T; t += team) {
+      const size_t b = n * t / T, e = n * (t + 1) / T;
+      uint64_t acc = part[t];
+      for (size_t i = b; i < e; ++i) { out[i] = acc; acc += cnt[i]; }
+    }
   }
   out[n] = part[T];
 }

@@ -80,15 +80,15 @@ struct Coo {
 };
 
 // Entries of [b, e): split into T chunks on line boundaries as the
-// reference's reader does, each chunk parsed by one thread.
+// reference's reader does, each chunk parsed by one thread (a team smaller
+// than T takes chunks tid, tid + team, ...).
 Coo parse_entries(const char* b, const char* e) {
   const int T = std::max(1, omp_get_max_threads());
   std::vector<std::vector<uint32_t>> xs(T), ys(T);
   std::vector<uint32_t> mr(T, 0), mc(T, 0);
   const size_t size = size_t(e - b);
 #pragma omp parallel num_threads(T)
-  {
-    const int tid = omp_get_thread_num();
+  for (int tid = omp_get_thread_num(); tid < T; tid += omp_get_num_threads()) {
     Reader rs{b + (size_t(tid) * size) / size_t(T), e};
     Reader re{b + (size_t(tid + 1) * size) / size_t(T), e};
     re.move_to_eol();
@@ -188,8 +188,9 @@ bool histograms_fit(uint32_t n, size_t m, int T) {
 // Rows over n keys from m (key, value) pairs in a fixed order, each row
 // keeping that order (a stable counting scatter). The pairs come in T
 // consecutive parts; part(t, f) calls f(key, value) for the pairs of part t
-// in order. Thread t counts its part into a histogram of its own and places
-// it after the parts before it.
+// in order. Part t is counted into a histogram of its own and placed after
+// the parts before it; thread tid of the team (which may be smaller than T)
+// takes parts tid, tid + team, ...
 template <class Part>
 void stable_rows(uint32_t n, size_t m, int T, Part part, std::vector<uint32_t>& ptr,
                  std::vector<uint32_t>& idx) {
@@ -198,9 +199,11 @@ void stable_rows(uint32_t n, size_t m, int T, Part part, std::vector<uint32_t>& 
   std::vector<uint32_t> hist(size_t(T) * n, 0);
 #pragma omp parallel num_threads(T)
   {
-    const int t = omp_get_thread_num();
-    uint32_t* h = hist.data() + size_t(t) * n;
-    part(t, [&](uint32_t key, uint32_t) { ++h[key]; });
+    const int team = omp_get_num_threads();
+    for (int t = omp_get_thread_num(); t < T; t += team) {
+      uint32_t* h = hist.data() + size_t(t) * n;
+      part(t, [&](uint32_t key, uint32_t) { ++h[key]; });
+    }
 #pragma omp barrier
 #pragma omp for schedule(static)
     for (int64_t i = 0; i < int64_t(n); ++i) {
@@ -220,7 +223,10 @@ void stable_rows(uint32_t n, size_t m, int T, Part part, std::vector<uint32_t>& 
         at += k;
       }
     }
-    part(t, [&](uint32_t key, uint32_t val) { idx[h[key]++] = val; });
+    for (int t = omp_get_thread_num(); t < T; t += team) {
+      uint32_t* h = hist.data() + size_t(t) * n;
+      part(t, [&](uint32_t key, uint32_t val) { idx[h[key]++] = val; });
+    }
   }
 }
 

@@ -139,9 +139,12 @@ void transpose_rows(const Csr& g, std::vector<u32>& tp, std::vector<u32>& ti) {
   std::vector<u32> hist(size_t(T) * n, 0);
 #pragma omp parallel num_threads(T)
   {
-    const int t = omp_get_thread_num();
-    u32* h = hist.data() + size_t(t) * n;
-    for (uint64_t e = g.off[at[t]]; e < g.off[at[t + 1]]; ++e) ++h[g.col[e]];
+    // T parts; a team smaller than T takes parts tid, tid + team, ...
+    const int team = omp_get_num_threads();
+    for (int t = omp_get_thread_num(); t < T; t += team) {
+      u32* h = hist.data() + size_t(t) * n;
+      for (uint64_t e = g.off[at[t]]; e < g.off[at[t + 1]]; ++e) ++h[g.col[e]];
+    }
 #pragma omp barrier
 #pragma omp for schedule(static)
     for (int64_t c = 0; c < int64_t(n); ++c) {
@@ -161,8 +164,11 @@ void transpose_rows(const Csr& g, std::vector<u32>& tp, std::vector<u32>& ti) {
         slot += k;
       }
     }
-    for (u32 r = at[t]; r < at[t + 1]; ++r)
-      for (uint64_t e = g.off[r]; e < g.off[r + 1]; ++e) ti[h[g.col[e]]++] = r;
+    for (int t = omp_get_thread_num(); t < T; t += team) {
+      u32* h = hist.data() + size_t(t) * n;
+      for (u32 r = at[t]; r < at[t + 1]; ++r)
+        for (uint64_t e = g.off[r]; e < g.off[r + 1]; ++e) ti[h[g.col[e]]++] = r;
+    }
   }
 }
 
@@ -194,12 +200,11 @@ std::vector<u32> rac_refine(u32 n, const Off* off, const u32* col, const std::ve
 // threads x tiles would be large).
 u32 maxload(const Csr& g, const std::vector<u32>& c) {
   const size_t B = c.size() - 1;
-  const int T = B * B * size_t(omp_get_max_threads()) <= (size_t(1) << 24) ? omp_get_max_threads() : 1;
+  const int T = B * B * size_t(omp_get_max_threads()) <= (size_t(1) << 24) ? std::max(1, omp_get_max_threads()) : 1;
   const std::vector<u32> at = entry_split(g, T);
   std::vector<u32> load(size_t(T) * B * B, 0);
 #pragma omp parallel num_threads(T)
-  {
-    const int t = omp_get_thread_num();
+  for (int t = omp_get_thread_num(); t < T; t += omp_get_num_threads()) {
     u32* mine = load.data() + size_t(t) * B * B;
     // u: the number of cuts c[0..B) at or below row i (the row band is u - 1)
     size_t u = size_t(std::upper_bound(c.begin(), c.begin() + B, at[t]) - c.begin());
