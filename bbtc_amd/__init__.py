@@ -529,7 +529,8 @@ def layouts_build(graphs, ncuts, kind: str = "pal"):
         L.graph = graphs[x]
         L._h = C.c_void_p(out[x])
         L.p = int(nc[x])
-        L.cuts = None
+        ptr = C.cast(lib().bbtc_layout_cuts(L._h), C.POINTER(C.c_uint32))
+        L.cuts = np.array([ptr[c] for c in range(L.p + 1)], dtype=np.uint32)
         layouts.append(L)
     return layouts
 

@@ -5,7 +5,7 @@
 // only by tests/golden/make_golden.py in the container that has the
 // reference sources. It is not part of the product.
 //
-// Usage: ref_harness <graph.bbtc> <uni|n1d|pal> <ncut> [cuts-only]
+// Usage: ref_harness <graph.bbtc> <uni|n1d|pal> <ncut> [cuts-only|tasks-parallel]
 // Prints: {"n":..,"m":..,"p":..,"cuts":[..],"order_asc":[[i,j,k],..],
 //          "tasks":[[i,j,k,count],..] (lexicographic),"total":..}
 //
@@ -13,6 +13,10 @@
 // (src/bbtc.cpp:12-74): utils::BlockGraph -> ReadGraph (sparsify, partition,
 // cuts[p] = N+1, BlockCSR), tasks::TaskList + SortTasks, BBTc. Per-task
 // counts come from a BBTc constructed over a one-task list (bb_tc.cpp:3).
+// With tasks-parallel, the one-task BBTc objects run concurrently, one per
+// OpenMP thread (each owns its hash maps; the block CSR is only read), and
+// the whole-list count ("total_all") is skipped: for graphs whose tasks are
+// too many to count one after another.
 #include <cmath>
 
 #include "bb_tc.h"
@@ -23,13 +27,14 @@
 
 int main(int argc, const char** argv) {
   if (argc < 4) {
-    std::cerr << "usage: ref_harness <graph.bbtc> <uni|n1d|pal> <ncut> [cuts-only]\n";
+    std::cerr << "usage: ref_harness <graph.bbtc> <uni|n1d|pal|rac> <ncut> [cuts-only|tasks-parallel]\n";
     return 2;
   }
   std::string file = argv[1];
   std::string alg = argv[2];
   const int p = std::stoi(argv[3]);
   const bool cuts_only = argc > 4 && std::string(argv[4]) == "cuts-only";
+  const bool parallel = argc > 4 && std::string(argv[4]) == "tasks-parallel";
 
   BlockCSR<Ordinal, Vertex> st = utils::BlockGraph<Ordinal, Vertex, double>(file, alg, -1, p);
   const Ordinal P = st.GetNoOfCuts();
@@ -47,27 +52,38 @@ int main(int argc, const char** argv) {
     }
     std::cout << "]";
     auto lex = tasks::TaskList(P);
-    unsigned long long total = 0;
-    std::cout << ",\"tasks\":[";
-    for (size_t q = 0; q < lex.size(); ++q) {
+    std::vector<unsigned long long> per(lex.size(), 0);
+    auto one_task = [&](size_t q) {
       std::vector<std::vector<size_t>> one{lex[q]};
       BBTc bbtc(&st, one, P);
       bbtc.InitFlags();
       bbtc.Separator = 0;
       bbtc.ResetFlags();
-      const unsigned long long c = bbtc.NoOfTriangles();
+      per[q] = bbtc.NoOfTriangles();
       bbtc.CleanUp();
-      total += c;
+    };
+    if (parallel) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (size_t q = 0; q < lex.size(); ++q) one_task(q);
+    } else {
+      for (size_t q = 0; q < lex.size(); ++q) one_task(q);
+    }
+    unsigned long long total = 0;
+    std::cout << ",\"tasks\":[";
+    for (size_t q = 0; q < lex.size(); ++q) {
+      total += per[q];
       const size_t i = lex[q][0] / P, j = lex[q][0] % P, k = lex[q][1] % P;
-      std::cout << (q ? "," : "") << "[" << i << "," << j << "," << k << "," << c << "]";
+      std::cout << (q ? "," : "") << "[" << i << "," << j << "," << k << "," << per[q] << "]";
     }
     std::cout << "],\"total\":" << total;
-    BBTc all(&st, lex, P);
-    all.InitFlags();
-    all.Separator = 0;
-    all.ResetFlags();
-    std::cout << ",\"total_all\":" << all.NoOfTriangles();
-    all.CleanUp();
+    if (!parallel) {
+      BBTc all(&st, lex, P);
+      all.InitFlags();
+      all.Separator = 0;
+      all.ResetFlags();
+      std::cout << ",\"total_all\":" << all.NoOfTriangles();
+      all.CleanUp();
+    }
   }
   std::cout << "}" << std::endl;
   st.CleanUp();

@@ -19,14 +19,16 @@ def pytest_configure(config):
 
 def load_golden(name=None):
     """Per-graph fixtures (cuts and per-task counts); other fixture files,
-    such as the text-ingest hashes, have no "runs" and are skipped."""
+    such as the text-ingest hashes, have no "runs" and are skipped. Fixtures
+    of full-size graphs ("full_size": RMAT scale 26) are returned only when
+    asked for by name."""
     out = []
     for path in sorted(glob.glob(os.path.join(GOLDEN_DIR, "*.json"))):
         with open(path) as f:
             d = json.load(f)
         if "runs" not in d:
             continue
-        if name is None or d["name"] == name:
+        if name is None and not d.get("full_size") or d["name"] == name:
             out.append(d)
     return out
 

@@ -316,18 +316,31 @@ def s26():
 
 
 def test_s26_properties(dev, s26):
+    from tests.golden.fingerprint import fingerprint
     g = s26
+    # the reference's own cuts, task order and per-task counts of this graph
+    # (tests/golden/c3_rmat_s26_ef16.json, written by its CPU code)
+    (fx,) = load_golden("c3_rmat_s26_ef16")
+    (run,) = fx["runs"]
+    assert fingerprint(g.offsets(), g.cols()) == fx["fingerprint"], "generator drifted"
     cuts = g.partition("pal", 32)
+    assert cuts.tolist() == run["cuts"]
     L = B.Layout(g, cuts)
     t = L.tasks("desc")
+    asc = L.tasks("asc")
+    assert [tuple(x) for x in asc.tolist()] == [tuple(x) for x in run["order_asc"]]
     dg = dev.upload(L)
     per_a, tot_a = dg.count(t, "auto")
     per_r, tot_r = dg.count(t, "rdense")
     per_l, tot_l = dg.count(t, "lane")
-    assert int(per_a.sum()) == tot_a
+    assert int(per_a.sum()) == tot_a == run["total"]
     assert tot_a == tot_r and np.array_equal(per_a, per_r)
     assert tot_a == tot_l and np.array_equal(per_a, per_l)
-    # oracle on every 97th task (lexicographic), bit-exact
+    # every task against the reference's count
+    want = {tuple(x[:3]): x[3] for x in run["tasks"]}
+    got = {tuple(x): int(c) for x, c in zip(t.tolist(), per_a)}
+    assert got == want
+    # and every 97th task (lexicographic) against the oracle, bit-exact
     lex = np.lexsort((t["k"], t["j"], t["i"]))[::97]
     R = O.RefLayout(g.offsets(), g.cols(), cuts)
     tt = np.stack([t["i"][lex], t["j"][lex], t["k"][lex]], 1).astype(np.uint32)
@@ -346,6 +359,13 @@ def test_s26_properties(dev, s26):
 
 
 # ---- batches -----------------------------------------------------------------
+def oracle_lex(g, L):
+    """The oracle's per-task counts of L's tasks in lexicographic order, and
+    their total."""
+    R = O.RefLayout(g.offsets(), g.cols(), L.cuts)
+    return R.count(np.array(L.tasks("lex").tolist(), dtype=np.uint32))
+
+
 @pytest.mark.parametrize("order", ["heavy_first", "small_first"])
 def test_batch_matches_per_graph(dev, order):
     from bbtc_amd import workloads as W
@@ -353,13 +373,14 @@ def test_batch_matches_per_graph(dev, order):
     graphs = [W.make_graph(dev, s) for s in specs]
     ncut = [1, 2, 4, 8] * 12
     Ls = B.layouts_build(graphs, ncut, "pal")
-    cpu = [L.count_cpu(L.tasks("lex")) for L in Ls]
+    # per task: the oracle's count of each graph's tasks (lexicographic)
+    ref = [oracle_lex(g, L) for g, L in zip(graphs, Ls)]
     b = B.Batch(dev, Ls, "auto", order=order)
     gidx, bt = b.tasks()
-    assert len(gidx) == sum(len(c[0]) for c in cpu)
-    # per task of the batch: the CPU count of that graph's task
+    assert len(gidx) == sum(len(r[0]) for r in ref)
+    # per task of the batch: the oracle's count of that graph's task
     lex_index = [{tuple(x): q for q, x in enumerate(L.tasks("lex").tolist())} for L in Ls]
-    want = np.array([cpu[g][0][lex_index[g][tuple(x)]] for g, x in zip(gidx, bt.tolist())], dtype=np.uint64)
+    want = np.array([ref[g][0][lex_index[g][tuple(x)]] for g, x in zip(gidx, bt.tolist())], dtype=np.uint64)
     for mode in ["auto", "tuned", "captured"]:
         if mode == "tuned":
             b.autotune()
@@ -368,15 +389,58 @@ def test_batch_matches_per_graph(dev, order):
         for rep in range(3):  # replays, with host work between them
             b.launch()
             pg, pt, lat = b.results()
-            assert np.array_equal(pg, np.array([c[1] for c in cpu], dtype=np.uint64)), (mode, rep)
+            assert np.array_equal(pg, np.array([r[1] for r in ref], dtype=np.uint64)), (mode, rep)
             assert np.array_equal(pt, want), (mode, rep)
             assert (lat > 0).all() and np.isfinite(lat).all()
     b.destroy()
 
 
+@pytest.mark.timeout(900)
+def test_batch_4096_graphs(dev):
+    # the full batched workload: 4,096 uneven graphs (70% RMAT s10-18 ef8-32,
+    # 30% G(n,m)), cut counts from the cut-count autotuner, forms autotuned
+    # over the batch, one captured launch; every graph's per-task counts equal
+    # the plain per-graph launch, and every 64th graph's equal the oracle
+    from bbtc_amd import workloads as W
+    specs = W.batched_specs(4096)
+    graphs = [W.make_graph(dev, s) for s in specs]
+    ms = np.array([g.m for g in graphs])
+    log("c4 graphs", len(graphs), "edges", int(ms.sum()))
+    by_size = np.argsort(ms)
+    rule, _ = B.tune_ncut(dev, [graphs[i] for i in by_size[::64]], (1, 2, 4, 8))
+    ncut = [B.rule_ncut(rule, int(m)) for m in ms]
+    log("c4 ncut", dict(zip(*np.unique(ncut, return_counts=True))))
+    Ls = B.layouts_build(graphs, ncut, "pal")
+    b = B.Batch(dev, Ls, "auto")
+    b.autotune()
+    b.capture()
+    b.launch()
+    pg, pt, lat = b.results()
+    gidx, bt = b.tasks()
+    b.destroy()
+    log("c4 batch counted", int(pg.sum()))
+    per_graph = [dict() for _ in Ls]
+    for x, (gi, tk) in enumerate(zip(gidx.tolist(), bt.tolist())):
+        per_graph[gi][tuple(tk)] = int(pt[x])
+    for x, L in enumerate(Ls):
+        t = L.tasks("desc")
+        dg = dev.upload(L)
+        per, tot = dg.count(t, "auto")
+        dg.release()
+        assert tot == pg[x], x
+        assert per_graph[x] == {tuple(k): int(c) for k, c in zip(t.tolist(), per)}, x
+    log("c4 plain launches agree")
+    for x in range(0, len(Ls), 64):
+        rper, rtot = oracle_lex(graphs[x], Ls[x])
+        lex = [tuple(k) for k in Ls[x].tasks("lex").tolist()]
+        assert rtot == pg[x], x
+        assert {k: int(c) for k, c in zip(lex, rper)} == per_graph[x], x
+    assert (lat > 0).all()
+
+
 # ---- serving -----------------------------------------------------------------
 def _serving_requests():
-    """Small requests of uneven size and cut count; per-task CPU counts."""
+    """Small requests of uneven size and cut count; per-task oracle counts."""
     specs = [("rmat", 12, 8, 1, "pal", 4), ("gnm", 3000, 20000, 2, "uni", 2), ("rmat", 14, 16, 3, "pal", 8),
              ("rmat", 10, 4, 4, "n1d", 1), ("gnm", 20000, 160000, 5, "pal", 3), ("rmat", 13, 24, 6, "uni", 5),
              ("gnm", 64, 0, 7, "uni", 2), ("rmat", 16, 18, 1492, "uni", 1)]
@@ -388,9 +452,11 @@ def _serving_requests():
             g = B.Graph.gnm(a, b, seed)
         else:
             g = B.Graph.from_csr(np.zeros(a + 1, np.uint64), np.zeros(0, np.uint32))
-        L = B.Layout(g, g.partition(part, p))
+        cuts = g.partition(part, p)
+        L = B.Layout(g, cuts)
         t = L.tasks("desc")
-        reqs.append((L, t, L.count_cpu(t)[0]))
+        rper, _ = O.RefLayout(g.offsets(), g.cols(), cuts).count(np.array(t.tolist(), dtype=np.uint32))
+        reqs.append((L, t, rper))
     return reqs
 
 
@@ -445,7 +511,7 @@ def test_server_heap_admission(dev):
     g = B.Graph.rmat(12, 8, seed=11)
     L = B.Layout(g, g.partition("pal", 4))
     t = L.tasks("desc")
-    cpu = L.count_cpu(t)[0]
+    cpu, _ = O.RefLayout(g.offsets(), g.cols(), L.cuts).count(np.array(t.tolist(), dtype=np.uint32))
     srv = B.Server(dev, heap_bytes=4 << 20)
     # larger than the whole heap: refused at submission
     with pytest.raises(B.BbtcError) as e:

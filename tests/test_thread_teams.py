@@ -31,7 +31,8 @@ g = B.Graph.rmat(14, 8, seed=3)
 h = B.Graph.gnm(5000, 40000, 22)
 out["gen"] = [fingerprint(g.offsets(), g.cols()), fingerprint(h.offsets(), h.cols())]
 gold = json.load(open(os.path.join(%(root)r, "tests", "golden", "ingest.json")))["cases"]
-d = tempfile.mkdtemp()
+td = tempfile.TemporaryDirectory()
+d = td.name
 out["ingest"] = {}
 for rec in gold:
     src = os.path.join(d, textgen.filename(rec["case"]))
@@ -48,6 +49,7 @@ for fx in load_golden():
     for r in fx["runs"]:
         if r["partitioner"] == "rac":
             out["rac"].append([fx["name"], r["ncut"], gg.partition("rac", r["ncut"]).tolist()])
+td.cleanup()
 print(json.dumps(out))
 """
 
