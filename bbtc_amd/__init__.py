@@ -127,6 +127,7 @@ def _declare(lib):
         "bbtc_autotune": (st, [vp, vp, vp, u64, vp, vp]),
         "bbtc_plan_destroy": (None, [vp]),
         "bbtc_count": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
+        "bbtc_count_noalloc": (st, [vp, vp, vp, u64, i32, vp, C.POINTER(u64), vp]),
         "bbtc_count_hybrid": (st, [vp, vp, vp, vp, u64, u64, i32, i32, i32, vp, vp, C.POINTER(u64),
                                    C.POINTER(u64)]),
         "bbtc_layouts_build": (st, [vp, u64, vp, i32, vp]),
@@ -438,6 +439,17 @@ class DeviceGraph:
         _check(lib().bbtc_count(self.dev._h, self._h, _ptr(t), len(t), KERNEL[kernel], _ptr(per),
                                 C.byref(tot), None))
         return per, tot.value
+
+    def count_noalloc(self, tasks, kernel: str = "auto", per_task: bool = True):
+        """bbtc_count_noalloc: the same count with no device allocation (a
+        fixed per-device scratch array; at most 65,536 tasks; calls on one
+        device must not overlap). per_task False asks for the total only."""
+        t = tasks_array(tasks)
+        per = np.zeros(len(t), dtype=np.uint64)
+        tot = C.c_uint64()
+        _check(lib().bbtc_count_noalloc(self.dev._h, self._h, _ptr(t), len(t), KERNEL[kernel],
+                                        _ptr(per) if per_task else None, C.byref(tot), None))
+        return (per if per_task else None), tot.value
 
     def count_hybrid(self, tasks, separator, nthreads=0, nstreams=1, kernel="auto", choice=None):
         """Device from the front, host workers from the back (--border)."""

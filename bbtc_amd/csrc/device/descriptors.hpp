@@ -44,6 +44,12 @@ constexpr uint64_t kManyItemsPerWave = 64;
 constexpr uint32_t kChunkSteps = 256;
 constexpr uint32_t kSpillCap = 1u << 20;
 constexpr int kSpillShift = 36;  // SlotCtl::spill = (records << 36) | chunks
+// bbtc_count_noalloc: the most tasks its fixed scratch array holds, the spill
+// records it keeps (items past them are not split, only slower), and the
+// array's size.
+constexpr uint64_t kNoAllocMaxTasks = 1u << 16;
+constexpr uint32_t kNoAllocSpillCap = 1u << 18;
+constexpr uint64_t kScratchBytes = 12ull << 20;
 // BBTC_KERNEL_AUTO uses the bitmap form when at least this many rows' bitmaps
 // of band k fit the wave slice (band width <= kLdsWords * 32 / this).
 constexpr int kAutoBitmapRows = 1;

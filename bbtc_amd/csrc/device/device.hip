@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,48 @@
 
 namespace bbtc {
 void set_last_error(const std::string& s);
+
+// ---------------------------------------------------------------------------
+// Scratch of bbtc_count_noalloc: a fixed array in each device's global
+// memory, so that entry point allocates nothing. Its layout (256-byte aligned
+// parts): the slot control block, the segment descriptor, the task
+// descriptors, the item prefix, the per-task counters and the spill records
+// of split items. Host code never uses the symbol itself (its host-side name
+// is a stand-in, not a device address): bbtc_ctx_create resolves the
+// device's own address once with hipGetSymbolAddress and the kernels receive
+// that address.
+// ---------------------------------------------------------------------------
+constexpr uint64_t align256(uint64_t x) { return (x + 255) / 256 * 256; }
+constexpr uint64_t kScratchCtl = 0;
+constexpr uint64_t kScratchSeg = align256(kScratchCtl + sizeof(SlotCtl));
+constexpr uint64_t kScratchTasks = align256(kScratchSeg + sizeof(SegDesc));
+constexpr uint64_t kScratchPrefix = align256(kScratchTasks + kNoAllocMaxTasks * sizeof(TaskDesc));
+constexpr uint64_t kScratchCounts = align256(kScratchPrefix + (kNoAllocMaxTasks + 1) * 8);
+constexpr uint64_t kScratchSpill = align256(kScratchCounts + kNoAllocMaxTasks * 8);
+constexpr uint64_t kScratchSpillBase = align256(kScratchSpill + uint64_t(kNoAllocSpillCap) * sizeof(SpillRec));
+constexpr uint64_t kScratchEnd = align256(kScratchSpillBase + uint64_t(kNoAllocSpillCap) * 8);
+static_assert(kScratchEnd <= kScratchBytes, "scratch layout exceeds the array");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex scratch_mu;
+std::vector<void*> scratch_addr;  // per device; nullptr until resolved
+}  // namespace
+
+// The device address of `scratch` on the current device (resolved once per
+// device, with that device current).
+void* scratch_of_device(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_addr.size() <= size_t(device)) scratch_addr.resize(size_t(device) + 1, nullptr);
+  if (!scratch_addr[device]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(BBTC_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    scratch_addr[device] = p;
+  }
+  return scratch_addr[device];
+}
 
 bool valid_kernel(bbtc_kernel k) {
   return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
@@ -464,7 +507,9 @@ bbtc_status bbtc_ctx_create(int device, const bbtc_opts* opts, bbtc_ctx** out) {
                                                         bbtc::kBlock, 0));
     if (c->blocks_per_cu_lane < 1) c->blocks_per_cu_lane = 1;
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->scratch = static_cast<unsigned char*>(bbtc::scratch_of_device(device));
   } catch (...) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     throw;
   }
@@ -731,6 +776,83 @@ bbtc_status bbtc_count(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, ui
   if (st == BBTC_OK) st = bbtc_plan_results(p, per_task, total);
   bbtc_plan_destroy(p);
   return st;
+}
+
+bbtc_status bbtc_count_noalloc(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                               bbtc_kernel kernel, uint64_t* per_task, uint64_t* total, void* stream) {
+  DEV_REQUIRE(ctx && g && (tasks || n == 0) && (per_task || total), "null argument");
+  DEV_REQUIRE(g->ctx == ctx, "graph belongs to another context");
+  DEV_REQUIRE(bbtc::valid_kernel(kernel), "unknown kernel choice");
+  if (n > bbtc::kNoAllocMaxTasks) {
+    bbtc::set_last_error("bbtc_count_noalloc: " + std::to_string(n) + " tasks exceed the scratch limit of " +
+                         std::to_string(bbtc::kNoAllocMaxTasks));
+    return BBTC_ERR_INVALID;
+  }
+  for (uint64_t q = 0; q < n; ++q)
+    DEV_REQUIRE(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < g->P,
+                "task out of range or not i <= j <= k");
+  DEV_TRY
+  HIP_OK(hipSetDevice(ctx->device));
+  if (!ctx->scratch) bbtc::fail(BBTC_ERR_INTERNAL, "scratch address not resolved");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const bbtc::PlanDesc D = bbtc::describe_plan(g->htiles, g->deg, bbtc::refs_of(g->P, tasks, n), kernel, nullptr);
+  const uint64_t nitems = D.pref[n];
+  unsigned char* base = ctx->scratch;
+  auto* ctl = reinterpret_cast<bbtc::SlotCtl*>(base + bbtc::kScratchCtl);
+  auto* seg = reinterpret_cast<bbtc::SegDesc*>(base + bbtc::kScratchSeg);
+  auto* td = reinterpret_cast<bbtc::TaskDesc*>(base + bbtc::kScratchTasks);
+  auto* pref = reinterpret_cast<uint64_t*>(base + bbtc::kScratchPrefix);
+  auto* counts = reinterpret_cast<unsigned long long*>(base + bbtc::kScratchCounts);
+  auto* spill = reinterpret_cast<bbtc::SpillRec*>(base + bbtc::kScratchSpill);
+  auto* spill_base = reinterpret_cast<uint64_t*>(base + bbtc::kScratchSpillBase);
+  bbtc::SegDesc sd{};
+  sd.rowptr = g->rowptr;
+  sd.cols = g->cols;
+  sd.nzrows = g->nzrows;
+  sd.dense = g->dense;
+  sd.tiles = g->tiles;
+  sd.tasks = td;
+  sd.item_prefix = pref;
+  sd.per_task = counts;
+  sd.cycles = nullptr;
+  sd.item0 = 0;
+  sd.item1 = nitems;
+  sd.launch0 = 0;
+  sd.ntasks = uint32_t(n);
+  if (n) HIP_OK(hipMemcpyAsync(td, D.td.data(), n * sizeof(bbtc::TaskDesc), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(pref, D.pref.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(seg, &sd, sizeof(sd), hipMemcpyHostToDevice, s));
+  // the array keeps the previous call's counters: clear them (a fill kernel)
+  bbtc::enqueue_zero(counts, n, s);
+  if (nitems) {
+    bbtc::CountArgs a{};
+    a.segs = seg;
+    a.nseg = 1;
+    a.item_end = nitems;
+    a.ctl = ctl;  // reset by the launch sequence
+    a.spill = spill;
+    a.spill_base = spill_base;
+    a.spill_cap = uint32_t(std::min<uint64_t>(nitems, bbtc::kNoAllocSpillCap));
+    bbtc::enqueue_count(ctx, a, 0, nitems, D.lane_only, 0, s);
+  }
+  uint64_t t = 0;
+  if (per_task) {
+    if (n) HIP_OK(hipMemcpyAsync(per_task, counts, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (uint64_t q = 0; q < n; ++q) t += per_task[q];
+  } else {
+    // the total without a host array: counters copied back a block at a time
+    uint64_t buf[512];
+    for (uint64_t q0 = 0; q0 < n; q0 += 512) {
+      const uint64_t len = std::min<uint64_t>(512, n - q0);
+      HIP_OK(hipMemcpyAsync(buf, counts + q0, len * 8, hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      for (uint64_t q = 0; q < len; ++q) t += buf[q];
+    }
+    HIP_OK(hipStreamSynchronize(s));
+  }
+  if (total) *total = t;
+  DEV_CATCH
 }
 
 }  // extern "C"

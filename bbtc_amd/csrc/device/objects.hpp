@@ -29,6 +29,7 @@ struct bbtc_ctx {
   std::vector<hipStream_t> slots;         // slot i > 0, created on first use
   uint64_t budget = 0;
   uint64_t in_use = 0;
+  unsigned char* scratch = nullptr;  // this device's address of the scratch array (bbtc_count_noalloc)
 };
 
 struct bbtc_dgraph {

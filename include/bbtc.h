@@ -313,6 +313,21 @@ void bbtc_plan_destroy(bbtc_plan* plan);
  * graph; per_task may be NULL. */
 bbtc_status bbtc_count(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                        bbtc_kernel kernel, uint64_t* per_task, uint64_t* total, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bbtc_count with no device allocation and no free. The
+ * task descriptors, item prefix, per-task counters, launch control words
+ * and spill records live in a fixed 12 MiB array of each device's global
+ * memory, held until the process ends, so at most 65,536 tasks per call
+ * (a larger n returns BBTC_ERR_INVALID; 2^18 spill records, past which heavy
+ * items are not split, which costs time, not counts). Per-task counts
+ * (per_task, may be NULL) and their sum (total, may be NULL; not both) are
+ * the same as bbtc_count's. The call waits for its work before returning.
+ * Every call on one device shares that device's array: calls that overlap on
+ * one device (other threads, other contexts of the same device) race and
+ * must be serialised by the caller. */
+bbtc_status bbtc_count_noalloc(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
+                               bbtc_kernel kernel, uint64_t* per_task, uint64_t* total, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batches: many independent graphs, one persistent launch.                  */
