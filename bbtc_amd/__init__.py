@@ -52,7 +52,8 @@ class bbtc_opts(C.Structure):
 class bbtc_server_opts(C.Structure):
     _fields_ = [("heap_bytes", C.c_uint64), ("quantum_items", C.c_uint64), ("max_inflight", C.c_uint32),
                 ("max_segments", C.c_uint32), ("sequential", C.c_uint32), ("lane_form", C.c_uint32),
-                ("launch_us", C.c_uint32), ("oldest_pct", C.c_uint32)]
+                ("launch_us", C.c_uint32), ("oldest_pct", C.c_uint32), ("list_chunks", C.c_uint32),
+                ("defer_steps", C.c_uint32)]
 
 
 class bbtc_server_result(C.Structure):
@@ -105,6 +106,7 @@ def _declare(lib):
         "bbtc_task_count": (u64, [u32]),
         "bbtc_tasks_build": (st, [vp, i32, vp, u64, C.POINTER(u64)]),
         "bbtc_tasks_bytes": (st, [vp, vp, u64, u32, vp, C.POINTER(u64), C.POINTER(u64)]),
+        "bbtc_tasks_stats": (st, [vp, vp, u64, u32, vp]),
         "bbtc_count_cpu": (st, [vp, vp, u64, i32, vp, C.POINTER(u64)]),
         "bbtc_device_count": (st, [C.POINTER(i32)]),
         "bbtc_ctx_create": (st, [i32, C.POINTER(bbtc_opts), pvp]),
@@ -346,6 +348,16 @@ class Layout:
                                       C.byref(tot), C.byref(wed)))
         return per, tot.value, wed.value
 
+    def task_stats(self, tasks, sample_units: int = 0):
+        """Per task (n x 6): rows, wedges, sum d_ac(u), probes of bc lists,
+        probes of ac lists, bitmap-row lines met by ac probes
+        (bbtc_tasks_stats; sample_units > 0: estimates from at most that
+        many 64-row units per task)."""
+        t = tasks_array(tasks)
+        out = np.zeros((len(t), 6), dtype=np.uint64)
+        _check(lib().bbtc_tasks_stats(self._h, _ptr(t), len(t), sample_units, _ptr(out)))
+        return out
+
     def count_cpu(self, tasks, nthreads: int = 0):
         """Per-task and total counts on the host (the --ngpu 0 path). Workers
         claim from the back of `tasks`, so pass tasks("asc") to start with the
@@ -547,17 +559,19 @@ def layouts_build(graphs, ncuts, kind: str = "pal"):
     return layouts
 
 
-def tune_ncut(dev: "Device", sample, cands=(1, 2, 4, 8), kind: str = "pal", tune_forms: bool = True):
+def tune_ncut(dev: "Device", sample, cands=(1, 2, 4, 8), kind: str = "pal", tune_forms: bool = True,
+              by_time: bool = False):
     """Cut-count rule from measured device work on sample graphs (forms
-    autotuned per graph, or the AUTO rule's forms): returns (rule[64] indexed
-    by floor(log2(m+1)), work[n, ncand])."""
+    autotuned per graph, or the AUTO rule's forms; work as summed clocks, or
+    with by_time the launch time of each graph alone): returns (rule[64]
+    indexed by floor(log2(m+1)), work[n, ncand])."""
     n = len(sample)
     gh = (C.c_void_p * max(n, 1))(*[g._h for g in sample])
     cc = np.ascontiguousarray(cands, dtype=np.uint32)
     rule = np.zeros(64, dtype=np.uint32)
     work = np.zeros((max(n, 1), len(cc)), dtype=np.uint64)
-    _check(lib().bbtc_tune_ncut(dev._h, gh, n, _ptr(cc), len(cc), PART[kind], int(tune_forms), _ptr(rule),
-                                _ptr(work)))
+    flags = (1 if tune_forms else 0) | (2 if by_time else 0)
+    _check(lib().bbtc_tune_ncut(dev._h, gh, n, _ptr(cc), len(cc), PART[kind], flags, _ptr(rule), _ptr(work)))
     return rule, work[:n]
 
 
@@ -652,17 +666,20 @@ class Server:
     """Continuous-batching request loop on one device (bbtc_server_*).
 
     Options (0 / False: the library's defaults): heap_bytes, the device heap
-    (200 GiB); quantum_items, a fixed item budget per launch instead of the
-    default budget in estimated work sized to launch_us (10 ms); oldest_pct,
-    the oldest request's share of each launch (90); lane_form, let AUTO
-    requests whose tasks all admit the lane form run in launches of their
-    own; sequential, one request at a time (the plain path's order)."""
+    (200 GB); quantum_items, a fixed item budget per launch instead of the
+    default budget in estimated work sized to launch_us (40 ms); oldest_pct,
+    the oldest request's share of each launch (90); list_chunks, deal a
+    launch's items to the claim parts in 4,096-item chunks instead of one
+    contiguous range per part; defer_steps, the wave steps after which an item
+    is split; lane_form, let AUTO requests whose tasks all admit the lane form
+    run in launches of their own; sequential, one request at a time (the plain
+    path's order)."""
 
     def __init__(self, dev: "Device", heap_bytes: int = 0, quantum_items: int = 0, max_inflight: int = 0,
                  max_segments: int = 0, sequential: bool = False, lane_form: bool = False,
-                 launch_us: int = 0, oldest_pct: int = 0):
+                 launch_us: int = 0, oldest_pct: int = 0, list_chunks: bool = False, defer_steps: int = 0):
         o = bbtc_server_opts(heap_bytes, quantum_items, max_inflight, max_segments, int(sequential),
-                             int(lane_form), launch_us, oldest_pct)
+                             int(lane_form), launch_us, oldest_pct, int(list_chunks), defer_steps)
         h = C.c_void_p()
         _check(lib().bbtc_server_create(dev._h, C.byref(o), C.byref(h)))
         self._h = h

@@ -338,9 +338,12 @@ uint64_t edges_of(const bbtc_hgraph* g) {
 }
 
 // Cut count per graph: the list's column (use_column), else --ncut, else a
-// rule from measured device work on sample graphs spread over the sizes.
+// rule from measured device work on sample graphs spread over the sizes
+// (summed clocks for a batch, whose graphs share one launch; launch time for
+// requests served one by one: by_time).
 std::vector<uint32_t> cut_counts(bbtc_ctx* ctx, const std::vector<bbtc_hgraph*>& g,
-                                 const std::vector<ListEntry>& e, const Params& P, bool use_column) {
+                                 const std::vector<ListEntry>& e, const Params& P, bool use_column,
+                                 bool by_time) {
   const size_t n = g.size();
   std::vector<uint32_t> ncut(n, uint32_t(P.ncut));
   bool need_rule = false;
@@ -364,8 +367,8 @@ std::vector<uint32_t> cut_counts(bbtc_ctx* ctx, const std::vector<bbtc_hgraph*>&
   for (size_t x = 0; x < n; x += step) sample.push_back(g[by_m[x]]);
   const uint32_t cands[] = {1, 2, 4, 8};
   uint32_t rule[64];
-  const bbtc_status st = bbtc_tune_ncut(ctx, sample.data(), sample.size(), cands, 4, part_kind(P.part), 0, rule,
-                                        nullptr);
+  const bbtc_status st = bbtc_tune_ncut(ctx, sample.data(), sample.size(), cands, 4, part_kind(P.part),
+                                        by_time ? BBTC_TUNE_BY_TIME : 0, rule, nullptr);
   if (st) die(st, "choosing cut counts");
   for (size_t x = 0; x < n; ++x)
     if (!(use_column && e[x].has_value)) ncut[x] = bbtc_rule_ncut(rule, edges_of(g[x]));
@@ -391,7 +394,7 @@ int run_batch(const Params& P) {
   std::vector<bbtc_hgraph*> g;
   for (auto& x : e) g.push_back(load_listed(x.path));
   F.begin("Partitioning and Layout construction (s)");
-  const std::vector<uint32_t> ncut = cut_counts(ctx, g, e, P, true);
+  const std::vector<uint32_t> ncut = cut_counts(ctx, g, e, P, true, false);
   std::vector<bbtc_layout*> L(g.size(), nullptr);
   st = bbtc_layouts_build(g.data(), g.size(), ncut.data(), part_kind(P.part), L.data());
   if (st) die(st, "building the block layouts");
@@ -453,7 +456,7 @@ int run_serve(const Params& P) {
   std::vector<bbtc_hgraph*> g;
   for (auto& x : e) g.push_back(load_listed(x.path));
   F.begin("Partitioning and Layout construction (s)");
-  const std::vector<uint32_t> ncut = cut_counts(ctx, g, e, P, false);
+  const std::vector<uint32_t> ncut = cut_counts(ctx, g, e, P, false, true);
   std::vector<bbtc_layout*> L(g.size(), nullptr);
   st = bbtc_layouts_build(g.data(), g.size(), ncut.data(), part_kind(P.part), L.data());
   if (st) die(st, "building the block layouts");

@@ -107,11 +107,12 @@ void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s) {
 }
 
 void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t items, bool lane_only,
-                   uint64_t chunk, hipStream_t s) {
+                   uint64_t chunk, hipStream_t s, uint32_t defer_steps) {
   hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, i0, i0 + items, (unsigned long long)chunk);
   HIP_OK(hipGetLastError());
   a.phase = 0;
-  a.defer_steps = chunk == 0 && items >= kManyItemsPerWave * resident_blocks(ctx) * kWavesPerBlock
+  a.defer_steps = defer_steps ? defer_steps
+                  : chunk == 0 && items >= kManyItemsPerWave * resident_blocks(ctx) * kWavesPerBlock
                       ? kDeferStepsMany
                       : kDeferSteps;
   const uint64_t want = (items + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -129,9 +130,20 @@ void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t i
   HIP_OK(hipGetLastError());
 }
 
+std::vector<TaskWork> task_work(const Layout& L, const bbtc_task* tasks, uint64_t n) {
+  std::vector<uint64_t> st(n * kTaskStats);
+  task_stats(L, tasks, n, st.data(), kRuleSampleUnits);
+  std::vector<TaskWork> w(n);
+  for (uint64_t q = 0; q < n; ++q) {
+    const uint64_t* x = st.data() + kTaskStats * q;
+    w[q] = TaskWork{float(x[0]), float(x[1]), float(x[2]), float(x[3]), float(x[4]), float(x[5])};
+  }
+  return w;
+}
+
 PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<float>& deg,
                        const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice,
-                       bool allow_auto_lanes) {
+                       bool allow_auto_lanes, const std::vector<TaskWork>* work) {
   const uint64_t n = refs.size();
   PlanDesc P;
   P.td.assign(std::max<uint64_t>(n, 1), TaskDesc{});
@@ -177,8 +189,24 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
       if (has_dense) kind = kKindRdense;
     } else if (k == BBTC_KERNEL_LANE) {
       if (short_rows(x)) kind = kKindLane;
+    } else if (k == BBTC_KERNEL_AUTO && work) {
+      // the form of least estimated cost among those that apply
+      const TaskWork& s = (*work)[q];
+      if (s.W > 0) {
+        const FormCosts c = form_costs(s, wpr);
+        double best = c.merge;
+        if (fit >= 1 && c.bitmap < best) {
+          best = c.bitmap;
+          kind = kKindBitmap;
+        }
+        if (has_dense && fit >= 1 && c.dense < best) {
+          best = c.dense;
+          kind = kKindDense;
+        }
+        if (has_dense && c.rdense < best) kind = kKindRdense;
+      }
     } else if (k == BBTC_KERNEL_AUTO) {
-      // without measurements: the ac(u) list against the bc(v) bitmap row
+      // without statistics: the ac(u) list against the bc(v) bitmap row
       // where the bc tile has one, else the ac(u) bitmap where it fits
       if (has_dense) kind = kKindRdense;
       else if (fit >= uint32_t(kAutoBitmapRows)) kind = kKindBitmap;
@@ -268,16 +296,37 @@ void launch(bbtc_plan* p, hipStream_t s) {
 namespace bbtc {
 namespace {
 
-// Builds a plan over tile-addressed tasks of a resident graph.
+// Work statistics of the tasks of a single resident graph, from the table
+// gathered at upload (empty when it has none).
+std::vector<TaskWork> graph_work(const bbtc_dgraph* g, const std::vector<TaskRef>& refs) {
+  std::vector<TaskWork> w;
+  if (g->P == 0 || g->lex_work.empty()) return w;
+  const uint32_t P = g->P;
+  w.resize(refs.size());
+  for (size_t q = 0; q < refs.size(); ++q) {
+    const uint32_t i = refs[q].ab / P, j = refs[q].ab % P, k = refs[q].bc % P;
+    w[q] = g->lex_work[g->lex_ij[size_t(i) * P + j] + (k - j)];
+  }
+  return w;
+}
+
+// Builds a plan over tile-addressed tasks of a resident graph. work: the
+// tasks' statistics for the AUTO form rule (NULL: those of a single graph's
+// upload table, if any).
 bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& refs, bbtc_kernel kernel,
-                     const uint8_t* choice) {
+                     const uint8_t* choice, const std::vector<TaskWork>* work = nullptr) {
   const uint64_t n = refs.size();
   auto* p = new bbtc_plan;
   p->ctx = ctx;
   p->g = g;
   p->ntasks = n;
   try {
-    PlanDesc D = describe_plan(g->htiles, g->deg, refs, kernel, choice);
+    std::vector<TaskWork> own;
+    if (!work) {
+      own = graph_work(g, refs);
+      if (!own.empty()) work = &own;
+    }
+    PlanDesc D = describe_plan(g->htiles, g->deg, refs, kernel, choice, true, work);
     std::vector<TaskDesc>& td = D.td;
     std::vector<uint64_t>& pref = D.pref;
     p->hkind = std::move(D.hkind);
@@ -599,6 +648,18 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
     g->htiles = bbtc::tile_table(L);
     g->deg = L.deg;
     g->dwpr = L.dwpr;
+    if (L.P <= bbtc::kRuleMaxCuts) {
+      // every task's work statistics, for the AUTO form rule of later plans
+      const uint32_t P = L.P;
+      g->lex_ij.assign(size_t(P) * P, 0);
+      std::vector<bbtc_task> all;
+      for (uint32_t i = 0; i < P; ++i)
+        for (uint32_t j = i; j < P; ++j) {
+          g->lex_ij[size_t(i) * P + j] = all.size();
+          for (uint32_t k = j; k < P; ++k) all.push_back(bbtc_task{i, j, k});
+        }
+      g->lex_work = bbtc::task_work(L, all.data(), all.size());
+    }
     g->rowptr = bbtc::dev_alloc<uint32_t>(ctx, L.rowptr.size(), &g->bytes);
     g->cols = bbtc::dev_alloc<uint32_t>(ctx, L.cols.size(), &g->bytes);
     g->nzrows = bbtc::dev_alloc<uint32_t>(ctx, L.nzrows.size(), &g->bytes);
@@ -795,7 +856,10 @@ bbtc_status bbtc_count_noalloc(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* t
   HIP_OK(hipSetDevice(ctx->device));
   if (!ctx->scratch) bbtc::fail(BBTC_ERR_INTERNAL, "scratch address not resolved");
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  const bbtc::PlanDesc D = bbtc::describe_plan(g->htiles, g->deg, bbtc::refs_of(g->P, tasks, n), kernel, nullptr);
+  const std::vector<bbtc::TaskRef> refs = bbtc::refs_of(g->P, tasks, n);
+  const std::vector<bbtc::TaskWork> work = bbtc::graph_work(g, refs);
+  const bbtc::PlanDesc D =
+      bbtc::describe_plan(g->htiles, g->deg, refs, kernel, nullptr, true, work.empty() ? nullptr : &work);
   const uint64_t nitems = D.pref[n];
   unsigned char* base = ctx->scratch;
   auto* ctl = reinterpret_cast<bbtc::SlotCtl*>(base + bbtc::kScratchCtl);
@@ -866,6 +930,7 @@ struct bbtc_batch {
   bbtc_plan* plan = nullptr;
   uint64_t ngraphs = 0;
   std::vector<bbtc::TaskRef> refs;     // tasks in launch order
+  std::vector<bbtc::TaskWork> work;    // their work statistics (AUTO form rule)
   std::vector<uint32_t> task_ijk;      // i, j, k of each task (3 per task)
   std::vector<unsigned long long> hleft0;
   uint64_t bytes = 0;
@@ -911,7 +976,7 @@ void batch_replan(bbtc_batch* b, bbtc_kernel kernel, const uint8_t* choice) {
     release_plan(b->plan);
     b->plan = nullptr;
   }
-  b->plan = make_plan(b->ctx, b->arena, b->refs, kernel, choice);
+  b->plan = make_plan(b->ctx, b->arena, b->refs, kernel, choice, &b->work);
   b->plan->chunk = kListChunkItems;  // graphs early in the list finish early
   b->hleft0.assign(std::max<uint64_t>(b->ngraphs, 1), 0);
   for (uint64_t q = 0; q < b->refs.size(); ++q)
@@ -990,6 +1055,7 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
       uint64_t q;
     };
     std::vector<Order> order;
+    std::vector<bbtc::TaskWork> work;  // per task, in the order of b->refs before sorting
     for (uint64_t gi = 0; gi < ngraphs; ++gi) {
       const bbtc::Layout& L = layouts[gi]->L;
       const uint32_t P = L.P;
@@ -1016,6 +1082,7 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
       HIP_OK(hipMemcpyAsync(A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(A->dense + dn0[gi], L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, s));
       // this graph's tasks, weighted as tasks.hpp:21-23 for the cross-graph order
+      const size_t first = b->refs.size();
       for (uint32_t i = 0; i < P; ++i)
         for (uint32_t j = i; j < P; ++j)
           for (uint32_t k = j; k < P; ++k) {
@@ -1026,6 +1093,11 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
             const float w = nnz_ab * std::max(L.deg[L.tile(j, k)], L.deg[L.tile(i, k)]);
             order.push_back(Order{w, L.m, b->refs.size() - 1});
           }
+      std::vector<bbtc_task> mine;
+      for (size_t q = first; q < b->refs.size(); ++q)
+        mine.push_back(bbtc_task{b->task_ijk[3 * q], b->task_ijk[3 * q + 1], b->task_ijk[3 * q + 2]});
+      const std::vector<bbtc::TaskWork> w = bbtc::task_work(L, mine.data(), mine.size());
+      work.insert(work.end(), w.begin(), w.end());
     }
     HIP_OK(hipMemcpyAsync(A->tiles, A->htiles.data(), A->htiles.size() * sizeof(bbtc::TileDesc),
                           hipMemcpyHostToDevice, ctx->stream));
@@ -1039,8 +1111,10 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
     });
     std::vector<bbtc::TaskRef> refs(order.size());
     std::vector<uint32_t> ijk(3 * order.size());
+    b->work.resize(order.size());
     for (size_t x = 0; x < order.size(); ++x) {
       refs[x] = b->refs[order[x].q];
+      b->work[x] = work[order[x].q];
       for (int c = 0; c < 3; ++c) ijk[3 * x + c] = b->task_ijk[3 * order[x].q + c];
     }
     b->refs.swap(refs);

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "form_rule.hpp"
 #include "objects.hpp"
 
 namespace bbtc {
@@ -55,9 +56,15 @@ struct PlanDesc {
   bool lane_only = true;          // every task with work items uses the lane form
 };
 // auto_lanes: BBTC_KERNEL_AUTO may take the lane form for every task.
+// work (may be NULL): each task's work statistics, aligned with refs; with
+// them BBTC_KERNEL_AUTO takes the form of least estimated cost
+// (form_rule.hpp), without them the rule that needs no statistics.
 PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<float>& deg,
                        const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice,
-                       bool auto_lanes = true);
+                       bool auto_lanes = true, const std::vector<TaskWork>* work = nullptr);
+
+// Work statistics (estimated, form_rule.hpp) of tasks of a host layout.
+std::vector<TaskWork> task_work(const Layout& L, const bbtc_task* tasks, uint64_t n);
 
 // Resident blocks of count_probe (all forms, or lane form only) on the
 // context's device.
@@ -72,8 +79,11 @@ inline uint64_t resident_blocks(const bbtc_ctx* ctx, bool lane_only = false) {
 // runs in its own kernel instantiation (no LDS, more resident waves) and
 // never spills, so there is no second phase.
 // chunk: items per chunk of the first-phase parts (0: one chunk per part).
+// defer_steps: wave steps after which a first-phase unit spills (0: the rule
+// of one-graph launches, kDeferStepsMany for large launches with one chunk
+// per part, else kDeferSteps).
 void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t items, bool lane_only,
-                   uint64_t chunk, hipStream_t s);
+                   uint64_t chunk, hipStream_t s, uint32_t defer_steps = 0);
 
 // Enqueues p[0..n) = 0 on s.
 void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s);

@@ -9,6 +9,7 @@
 
 #include "../host/handles.hpp"
 #include "descriptors.hpp"
+#include "form_rule.hpp"
 
 #define HIP_OK(expr)                                                                   \
   do {                                                                                 \
@@ -44,6 +45,11 @@ struct bbtc_dgraph {
   std::vector<bbtc::TileDesc> htiles;
   std::vector<float> deg;
   std::vector<uint32_t> dwpr;  // bitmap words per row of each band
+  // Work statistics of every task (i, j, k), i <= j <= k, in lexicographic
+  // order (empty when there are more than kRuleMaxCuts cuts); task (i, j, k)
+  // is lex_work[lex_ij[i * P + j] + k - j].
+  std::vector<bbtc::TaskWork> lex_work;
+  std::vector<uint64_t> lex_ij;
 };
 
 struct bbtc_batch;

@@ -17,7 +17,9 @@
 // most about two launches); with `quantum_items` set, the budget is that
 // many items. The oldest resident request with work left keeps `oldest_pct`
 // percent of the budget, and the rest is filled with slices of the resident
-// requests in order of remaining work (shortest first). A request that
+// requests in order of remaining work (shortest first). A launch's items are
+// split into one contiguous range per claim part, as for one graph (each XCD
+// on its own range; `list_chunks` deals 4,096-item chunks instead). A request that
 // arrives while others run joins the next launch and shares the machine
 // with them; a large request is slowed by later arrivals by at most about
 // 100 / oldest_pct. With `lane_form`, requests whose tasks all take the lane
@@ -52,12 +54,12 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kAlign = 256;
-constexpr uint64_t kDefaultHeap = 200ull << 30;
+constexpr uint64_t kDefaultHeap = 200000000000ull;  // 200 GB
 constexpr uint64_t kDefaultQuantum = 1ull << 18;
 // adaptive launch budget: limits, and the default target launch time
 constexpr uint64_t kMinQuantum = 1ull << 14;
 constexpr uint64_t kMaxQuantum = 1ull << 24;
-constexpr double kDefaultLaunchMs = 10.0;
+constexpr double kDefaultLaunchMs = 40.0;
 constexpr uint32_t kDefaultOldestPct = 90;
 constexpr uint32_t kDefaultInflight = 2;
 constexpr uint32_t kDefaultSegments = 1024;
@@ -177,6 +179,8 @@ struct bbtc_server {
   bool lane_turn = false;  // kind of the next launch when both kinds have work
   bool lane_form = false;  // AUTO requests may take the lane form
   uint32_t oldest_pct = 90;  // share of each launch's budget kept by the oldest request
+  bool list_chunks = false;  // launches dealt to the claim parts in 4,096-item chunks
+  uint32_t defer_steps = 0;  // split threshold of launches (0: the one-graph rule)
 
   char* heap_base = nullptr;
   bbtc::Heap heap;
@@ -544,7 +548,8 @@ void sched_loop(bbtc_server* S) {
           S->event_pool.pop_back();
           l.items = total;
           HIP_OK(hipEventRecord(l.start, s));
-          enqueue_count(S->ctx, a, 0, total, lane_only, kListChunkItems, s);
+          enqueue_count(S->ctx, a, 0, total, lane_only, S->list_chunks ? kListChunkItems : 0, s,
+                        S->defer_steps);
           HIP_OK(hipEventRecord(l.done, s));
           inflight.push_back(std::move(l));
           std::lock_guard<std::mutex> lk(S->mu);
@@ -646,6 +651,8 @@ bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc
     S->lane_form = opts && opts->lane_form;
     if (opts && opts->oldest_pct > 100) bbtc::fail(BBTC_ERR_INVALID, "oldest_pct above 100");
     S->oldest_pct = opts && opts->oldest_pct ? opts->oldest_pct : bbtc::kDefaultOldestPct;
+    S->list_chunks = opts && opts->list_chunks;
+    S->defer_steps = opts ? opts->defer_steps : 0;
     if (S->sequential) S->max_inflight = 1;
     hipError_t he = hipMalloc(&S->heap_base, heap);
     if (he != hipSuccess) {
@@ -726,8 +733,9 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
   auto r = std::make_shared<bbtc::Request>();
   r->L = &L->L;
   r->tiles = bbtc::tile_table(L->L);
+  const std::vector<bbtc::TaskWork> work = bbtc::task_work(L->L, tasks, ntasks);
   r->plan = bbtc::describe_plan(r->tiles, L->L.deg, bbtc::refs_of(P, tasks, ntasks), kernel, nullptr,
-                                S->lane_form);
+                                S->lane_form, &work);
   r->ntasks = ntasks;
   r->nitems = r->plan.pref[ntasks];
   r->wpref.assign(ntasks + 1, 0.0);

@@ -2,6 +2,7 @@
 // the per-graph choice of the number of cuts from measured device work.
 #include <omp.h>
 
+#include <chrono>
 #include <cmath>
 #include <limits>
 #include <vector>
@@ -88,23 +89,35 @@ bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint
     if (st != BBTC_OK) return st;
     for (uint64_t x = 0; x < n && st == BBTC_OK; ++x) {
       bbtc_dgraph* g = nullptr;
+      const auto t_up = std::chrono::steady_clock::now();
       st = bbtc_graph_upload(ctx, L[x], &g);
       if (st != BBTC_OK) break;
+      const double up_ns = std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t_up).count();
       const uint64_t nt = bbtc_task_count(cands[c]);
       std::vector<bbtc_task> t(nt);
       uint64_t got = 0;
       st = bbtc_tasks_build(L[x], BBTC_ORDER_DESC, t.data(), nt, &got);
       std::vector<uint8_t> choice(nt);
-      if (st == BBTC_OK && tune_forms) st = bbtc_autotune(ctx, g, t.data(), got, choice.data(), nullptr);
+      const bool forms = (tune_forms & BBTC_TUNE_FORMS) != 0;
+      if (st == BBTC_OK && forms) st = bbtc_autotune(ctx, g, t.data(), got, choice.data(), nullptr);
       bbtc_plan* p = nullptr;
       if (st == BBTC_OK)
-        st = bbtc_plan_create_tuned(ctx, g, t.data(), got, BBTC_KERNEL_AUTO, tune_forms ? choice.data() : nullptr, &p);
-      std::vector<uint64_t> cyc(got);
-      if (st == BBTC_OK) st = bbtc_plan_profile(p, cyc.data());
-      if (st == BBTC_OK) {
-        uint64_t s = 0;
-        for (uint64_t v : cyc) s += v;
-        work[x * ncand + c] = s;
+        st = bbtc_plan_create_tuned(ctx, g, t.data(), got, BBTC_KERNEL_AUTO, forms ? choice.data() : nullptr, &p);
+      if (tune_forms & BBTC_TUNE_BY_TIME) {
+        // the upload, and the fastest of three launches after one that warms
+        // the caches, in ns
+        float ms[4] = {0.f, 0.f, 0.f, 0.f};
+        if (st == BBTC_OK) st = bbtc_plan_time(p, 4, ms);
+        if (st == BBTC_OK)
+          work[x * ncand + c] = uint64_t(up_ns + double(std::min(ms[1], std::min(ms[2], ms[3]))) * 1e6);
+      } else {
+        std::vector<uint64_t> cyc(got);
+        if (st == BBTC_OK) st = bbtc_plan_profile(p, cyc.data());
+        if (st == BBTC_OK) {
+          uint64_t s = 0;
+          for (uint64_t v : cyc) s += v;
+          work[x * ncand + c] = s;
+        }
       }
       bbtc_plan_destroy(p);
       bbtc_graph_release(g);
@@ -113,7 +126,8 @@ bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint
     if (st != BBTC_OK) return st;
   }
   // rule[b]: the candidate with the least total work over samples in bucket b;
-  // buckets without samples take the nearest sampled bucket's choice
+  // buckets without samples take the nearest sampled bucket's choice (below
+  // first); then no larger bucket takes fewer cuts than a smaller one
   std::vector<int> best(64, -1);
   for (uint32_t b = 0; b < 64; ++b) {
     uint64_t bestw = none;
@@ -141,6 +155,7 @@ bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint
       else if (b + d < 64 && best[b + d] >= 0) c = best[b + d];
     }
     rule[b] = cands[c < 0 ? 0 : c];
+    if (b > 0 && rule[b] < rule[b - 1]) rule[b] = rule[b - 1];
   }
   if (work_out)
     for (uint64_t x = 0; x < n * ncand; ++x) work_out[x] = work[x];

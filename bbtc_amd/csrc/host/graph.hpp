@@ -90,6 +90,13 @@ void sort_tasks(std::vector<bbtc_task>& t, const Layout& L, bbtc_order order);
 // Algorithmic bytes (see bbtc_tasks_bytes).
 void task_bytes(const Layout& L, const bbtc_task* t, uint64_t n, uint32_t o, uint64_t* per_task,
                 uint64_t* total, uint64_t* wedges);
+// Work statistics per task (see bbtc_tasks_stats): out[6 q .. 6 q + 6).
+// sample_units 0: exact; else at most that many units of kStatUnitRows
+// consecutive non-empty ab rows per task, spread evenly over its rows, and
+// the sums scaled to all its rows.
+constexpr int kTaskStats = 6;
+constexpr uint32_t kStatUnitRows = 64;
+void task_stats(const Layout& L, const bbtc_task* t, uint64_t n, uint64_t* out, uint32_t sample_units = 0);
 // True when the reference would skip the task: a tile with average degree 0.
 bool task_skipped(const Layout& L, const bbtc_task& t);
 

@@ -196,6 +196,17 @@ bbtc_status bbtc_tasks_bytes(const bbtc_layout* L, const bbtc_task* tasks, uint6
   BBTC_CATCH
 }
 
+bbtc_status bbtc_tasks_stats(const bbtc_layout* L, const bbtc_task* tasks, uint64_t n, uint32_t sample_units,
+                             uint64_t* stats) {
+  REQUIRE(L && (tasks || n == 0) && (stats || n == 0), "null argument");
+  for (uint64_t q = 0; q < n; ++q)
+    REQUIRE(tasks[q].i <= tasks[q].j && tasks[q].j <= tasks[q].k && tasks[q].k < L->L.P,
+            "task out of range or not i <= j <= k");
+  BBTC_TRY
+  bbtc::task_stats(L->L, tasks, n, stats, sample_units);
+  BBTC_CATCH
+}
+
 bbtc_status bbtc_count_cpu(const bbtc_layout* L, const bbtc_task* tasks, uint64_t n, int nthreads,
                            uint64_t* per_task, uint64_t* total) {
   REQUIRE(L && (tasks || n == 0), "null argument");

@@ -275,4 +275,69 @@ void task_bytes(const Layout& L, const bbtc_task* tasks, uint64_t n, uint32_t o,
   if (wedges) *wedges = wtot;
 }
 
+void task_stats(const Layout& L, const bbtc_task* tasks, uint64_t n, uint64_t* out, uint32_t sample_units) {
+  const uint32_t* rp = L.rowptr.data();
+  const uint32_t* cl = L.cols.data();
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t q = 0; q < int64_t(n); ++q) {
+    const bbtc_task x = tasks[q];
+    double R = 0, W = 0, A = 0, Pbc = 0, Pac = 0, Lac = 0;
+    if (!task_skipped(L, x)) {
+      // 128-byte lines of a bitmap row of band k
+      const uint64_t lines = (uint64_t(L.dwpr[x.k]) + 31) / 32;
+      const size_t ab = L.tile(x.i, x.j), bc = L.tile(x.j, x.k), ac = L.tile(x.i, x.k);
+      const uint32_t* ab_rp = rp + L.rp_base[ab];
+      const uint32_t* ac_rp = rp + L.rp_base[ac];
+      const uint32_t* bc_rp = rp + L.rp_base[bc];
+      const uint32_t* ab_c = cl + L.col_base[ab];
+      const uint32_t* nz = L.nzrows.data() + L.nz_base[ab];
+      // rows in units of kStatUnitRows consecutive non-empty rows; with
+      // sample_units > 0, at most that many units spread evenly over the list
+      const uint32_t nzr = L.nz_rows[ab];
+      const uint64_t units = (uint64_t(nzr) + kStatUnitRows - 1) / kStatUnitRows;
+      const uint64_t take = sample_units && units > sample_units ? sample_units : units;
+      uint64_t seen = 0;
+      double r_acc = 0, w_acc = 0, a_acc = 0, pbc_acc = 0, pac_acc = 0, lac_acc = 0;
+      for (uint64_t m = 0; m < take; ++m) {
+        const uint64_t unit = take == units ? m : (2 * m + 1) * units / (2 * take);
+        const uint32_t z0 = uint32_t(unit * kStatUnitRows);
+        const uint32_t z1 = std::min<uint32_t>(nzr, z0 + kStatUnitRows);
+        seen += z1 - z0;
+        for (uint32_t z = z0; z < z1; ++z) {
+          const uint32_t r = nz[z];
+          const uint64_t dac = ac_rp[r + 1] - ac_rp[r];
+          if (!dac) continue;
+          const uint32_t a0 = ab_rp[r], a1 = ab_rp[r + 1];
+          uint64_t pbc = 0;
+          for (uint32_t y = a0; y < a1; ++y) {
+            const uint32_t v = ab_c[y];
+            pbc += bc_rp[v + 1] - bc_rp[v];
+          }
+          r_acc += 1;
+          w_acc += a1 - a0;
+          a_acc += double(dac);
+          pbc_acc += double(pbc);
+          pac_acc += double(uint64_t(a1 - a0) * dac);
+          lac_acc += double(uint64_t(a1 - a0) * std::min(dac, lines));
+        }
+      }
+      // scaled from the rows seen to all rows of the list
+      const double f = seen ? double(nzr) / double(seen) : 0.0;
+      R = r_acc * f;
+      W = w_acc * f;
+      A = a_acc * f;
+      Pbc = pbc_acc * f;
+      Pac = pac_acc * f;
+      Lac = lac_acc * f;
+    }
+    uint64_t* o = out + kTaskStats * q;
+    o[0] = uint64_t(std::llround(R));
+    o[1] = uint64_t(std::llround(W));
+    o[2] = uint64_t(std::llround(A));
+    o[3] = uint64_t(std::llround(Pbc));
+    o[4] = uint64_t(std::llround(Pac));
+    o[5] = uint64_t(std::llround(Lac));
+  }
+}
+
 }  // namespace bbtc

@@ -181,7 +181,7 @@ def prepare_requests(dev: Device, specs, log=print, cands=(1, 2, 4, 8, 16), nsam
     t0 = time.time()
     order = np.argsort(ms)
     pick = np.unique(np.linspace(0, len(graphs) - 1, min(nsample, len(graphs))).round().astype(int))
-    rule, _ = tune_ncut(dev, [graphs[order[x]] for x in pick], cands, tune_forms=False)
+    rule, _ = tune_ncut(dev, [graphs[order[x]] for x in pick], cands, tune_forms=False, by_time=True)
     ncut = [rule_ncut(rule, m) for m in ms]
     log(f"serving: cut rule from {len(pick)} sample graphs in {time.time() - t0:.1f}s; "
         f"ncut histogram {dict(zip(*np.unique(ncut, return_counts=True)))}")
@@ -221,7 +221,7 @@ def run_plain(dev: Device, layouts, tasks, arrivals=None):
     return tot, lat, time.perf_counter() - t0, per
 
 
-def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200 << 30, srv=None, **opts):
+def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200_000_000_000, srv=None, **opts):
     """Every request through one continuous-batching server (`srv`, or a new
     one destroyed afterwards). Returns (totals, latencies s, makespan s,
     per-task counts, server statistics of this run)."""
@@ -247,12 +247,13 @@ def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200
     return tot, lat, max(r.done_s for r in rs), [r.per_task for r in rs], stats
 
 
-def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, quantum_items: int = 0,
-                lane_form: bool = False):
+def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, heap_bytes: int = 200_000_000_000,
+                **server_opts):
     """Serving workload: the plain per-graph launch and the server, first
     saturated (every request at time 0: capacity), then at Poisson arrivals of
     `rate` requests/s (default: `load` times the plain capacity). Per-task
-    counts must agree bit for bit between the two."""
+    counts must agree bit for bit between the two. server_opts: Server
+    options (launch_us, oldest_pct, part_chunks, ...)."""
     graphs, layouts, tasks, ncut = prepare_requests(dev, specs, log)
     del graphs
     n = len(layouts)
@@ -264,7 +265,7 @@ def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, qua
     p_cap = n / p_make
     log(f"plain saturated: {p_cap:.2f} graphs/s ({p_make:.2f}s for {n})")
     # one server (one heap reservation) for both server runs
-    srv = Server(dev, heap_bytes=200 << 30, quantum_items=quantum_items, lane_form=lane_form)
+    srv = Server(dev, heap_bytes=heap_bytes, **server_opts)
     try:
         return _serving_runs(dev, layouts, tasks, srv, n, p_tot, p_per, p_cap, rate, load, log)
     finally:

@@ -182,6 +182,20 @@ bbtc_status bbtc_tasks_bytes(const bbtc_layout* L, const bbtc_task* tasks, uint6
                              uint32_t offset_bytes, uint64_t* per_task_bytes,
                              uint64_t* total_bytes, uint64_t* total_wedges);
 
+/* Work of each task, as the device kernels meet it (no counterpart in the
+ * reference; the measure behind the kernel-form cost rule). stats[6q..6q+6)
+ * receives, over the rows u of band i with non-empty ab(u) and ac(u):
+ *   [0] rows, [1] wedges sum d_ab(u), [2] sum d_ac(u),
+ *   [3] probes of bc lists sum_{v in ab(u)} d_bc(v),
+ *   [4] probes of ac lists sum d_ab(u) d_ac(u),
+ *   [5] sum d_ab(u) min(d_ac(u), lines), lines = 128-byte lines of a bitmap
+ *       row of band k (the lines an ac list's probes of a bc bitmap row meet).
+ * Tasks the reference skips (bb_tc.cpp:86-91) get zeros. sample_units 0
+ * gives exact sums; otherwise each task's non-empty ab rows are read in
+ * units of 64 consecutive rows, at most sample_units units spread evenly
+ * over the rows, and the sums are scaled to all rows (estimates). */
+bbtc_status bbtc_tasks_stats(const bbtc_layout* L, const bbtc_task* tasks, uint64_t n, uint32_t sample_units,
+                             uint64_t* stats);
 /* ------------------------------------------------------------------------ */
 /* CPU count: the --ngpu 0 path and the host tail of --border.               */
 /* ------------------------------------------------------------------------ */
@@ -338,11 +352,18 @@ bbtc_status bbtc_layouts_build(const bbtc_hgraph* const* graphs, uint64_t n, con
                                bbtc_part kind, bbtc_layout** out);
 
 /* Cut-count autotuner: counts every sample graph with every candidate cut
- * count and measures the device work (summed per-item clocks), with the
- * forms autotuned per graph (tune_forms != 0) or chosen by BBTC_KERNEL_AUTO;
+ * count and measures the device work, with the forms autotuned per graph
+ * (flag BBTC_TUNE_FORMS in tune_forms) or chosen by BBTC_KERNEL_AUTO. The
+ * work is the summed per-item clocks (the share of the machine a graph takes
+ * inside a batch), or with flag BBTC_TUNE_BY_TIME the wall time of its
+ * upload plus the event time of a launch of that graph alone, in ns (what a
+ * graph counted by itself waits for).
  * rule[64] receives, per size class floor(log2(m+1)), the candidate with the
- * least work (classes without samples copy the nearest sampled class).
- * work_out (may be NULL) receives n x ncand clock sums. */
+ * least work; classes without samples copy the nearest sampled class (the
+ * smaller one first), and no class takes fewer cuts than a smaller class.
+ * work_out (may be NULL) receives the n x ncand measurements. */
+#define BBTC_TUNE_FORMS 1
+#define BBTC_TUNE_BY_TIME 2
 bbtc_status bbtc_tune_ncut(bbtc_ctx* ctx, const bbtc_hgraph* const* sample, uint64_t n,
                            const uint32_t* cands, uint32_t ncand, bbtc_part kind, int tune_forms,
                            uint32_t* rule, uint64_t* work_out);
@@ -405,7 +426,7 @@ bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* 
 typedef struct bbtc_server bbtc_server;
 
 typedef struct bbtc_server_opts {
-  uint64_t heap_bytes;     /* device heap for resident requests (0: 200 GiB) */
+  uint64_t heap_bytes;     /* device heap for resident requests (0: 200 GB, 2e11 bytes) */
   uint64_t quantum_items;  /* work items per launch (0: adaptive, see launch_us) */
   uint32_t max_inflight;   /* launches queued on the device at once (0: 2) */
   uint32_t max_segments;   /* requests per launch (0: 1024) */
@@ -417,9 +438,15 @@ typedef struct bbtc_server_opts {
   uint32_t launch_us;      /* with quantum_items 0: launches are composed by estimated work
                               (the reference's task weight spread over a task's items), and
                               the work budget follows the measured rate of finished launches
-                              so that a launch takes about this long (0: 10000 us) */
+                              so that a launch takes about this long (0: 40000 us) */
   uint32_t oldest_pct;     /* percent of each launch's budget the oldest request with work
                               keeps (0: 90); the rest goes shortest remaining work first */
+  uint32_t list_chunks;    /* 0: a launch's items are split into one contiguous range per
+                              claim part (each XCD on its own range, as for one graph);
+                              1: 4,096-item chunks dealt to the parts in turn */
+  uint32_t defer_steps;    /* wave steps after which a work item is split (0: 1,024 steps for
+                              launches of at least 64 items per resident wave without
+                              list_chunks, else 512) */
 } bbtc_server_opts;
 
 typedef struct bbtc_server_result {

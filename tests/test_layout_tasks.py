@@ -123,3 +123,55 @@ def test_bad_cuts_rejected():
         B.Layout(g, np.array([0, 50, 20, g.n + 1], dtype=np.uint32))
     with pytest.raises(B.BbtcError):
         B.Layout(g, np.array([0, 50, 100], dtype=np.uint32))  # does not cover all rows
+
+
+def _stats_numpy(g, cuts, tasks):
+    """Per-task rows, wedges, sum d_ac, bc probes, ac probes and the bitmap
+    lines those meet, straight from the CSR and the cuts (a task's rows: u in
+    band i with neighbours in band j and in band k)."""
+    off, col = g.offsets().astype(np.int64), g.cols().astype(np.int64)
+    band = np.searchsorted(cuts, np.arange(g.n), side="right") - 1
+    rows_of = [col[off[u]:off[u + 1]] for u in range(g.n)]
+    out = []
+    for i, j, k in tasks.tolist():
+        R = W = A = Pbc = Pac = Lac = 0
+        width = min(cuts[k + 1], g.n + 1) - cuts[k]
+        lines = ((((width + 31) // 32) + 3) // 4 * 4 + 31) // 32  # 16-byte bitmap rows, 128-byte lines
+        for u in range(cuts[i], min(cuts[i + 1], g.n)):
+            nb = rows_of[u]
+            ab = nb[band[nb] == j]
+            ac = nb[band[nb] == k]
+            if len(ab) == 0 or len(ac) == 0:
+                continue
+            R += 1
+            W += len(ab)
+            A += len(ac)
+            Pac += len(ab) * len(ac)
+            Lac += len(ab) * min(len(ac), lines)
+            for v in ab:
+                nv = rows_of[v]
+                Pbc += int(np.sum(band[nv] == k))
+        out.append((R, W, A, Pbc, Pac, Lac))
+    return np.array(out, dtype=np.uint64).reshape(-1, 6)
+
+
+@pytest.mark.parametrize("p,kind", [(1, "uni"), (4, "pal"), (6, "n1d")])
+def test_task_stats(p, kind):
+    g = B.Graph.rmat(9, 12, seed=31)
+    cuts = g.partition(kind, p)
+    L = B.Layout(g, cuts)
+    t = L.tasks("lex")
+    st = L.task_stats(t)
+    want = _stats_numpy(g, cuts, t)
+    # tasks the reference skips (a tile of average degree 0) carry no work
+    for q, x in enumerate(t.tolist()):
+        tiles = [L.tile(x[0], x[1]), L.tile(x[1], x[2]), L.tile(x[0], x[2])]
+        if any(ti.nnz == 0 for ti in tiles):
+            want[q] = 0
+    assert np.array_equal(st, want)
+    # the algorithmic bytes are these terms: 4o R + 4 W + 4 A + 2o W + 4 Pbc
+    for o in (4, 8):
+        per, _, wedges = L.task_bytes(t, o)
+        R, W, A, Pbc = (st[:, c].astype(np.int64) for c in range(4))
+        assert np.array_equal(per.astype(np.int64), 4 * o * R + 4 * W + 4 * A + 2 * o * W + 4 * Pbc)
+        assert wedges == int(W.sum())

@@ -101,3 +101,19 @@ def test_rac_parallel_and_serial_transpose(n):
     for p in (2, 5, 16):
         got = g.partition("rac", p)
         assert got.tolist() == O.cuts("rac", g.offsets(), g.cols(), p).tolist(), p
+
+
+@pytest.mark.slow
+def test_s26_pal32_cuts_match_reference():
+    # the headline graph (RMAT scale 26, edge factor 16, seed 3; about 2.5
+    # minutes and 25 GB on 8 cores): the reference's own pal cuts with 32 cuts
+    # and the ascending task order its tasks::SortTasks wrote
+    (fx,) = load_golden("c3_rmat_s26_ef16")
+    (run,) = fx["runs"]
+    g = graph_from_spec(fx["graph"])
+    assert fingerprint(g.offsets(), g.cols()) == fx["fingerprint"], "generator drifted"
+    cuts = g.partition("pal", 32)
+    assert cuts.tolist() == run["cuts"]
+    L = B.Layout(g, cuts)
+    assert [tuple(x) for x in L.tasks("asc").tolist()] == [tuple(x) for x in run["order_asc"]]
+    assert len(run["tasks"]) == 5984 and sum(x[3] for x in run["tasks"]) == run["total"]
