@@ -125,6 +125,7 @@ def _declare(lib):
         "bbtc_plan_bitmap_tasks": (u64, [vp]),
         "bbtc_plan_time_tasks": (st, [vp, vp]),
         "bbtc_plan_task_kinds": (st, [vp, vp]),
+        "bbtc_batch_task_kinds": (st, [vp, vp]),
         "bbtc_plan_profile": (st, [vp, vp]),
         "bbtc_autotune": (st, [vp, vp, vp, u64, vp, vp]),
         "bbtc_plan_destroy": (None, [vp]),
@@ -614,6 +615,12 @@ class Batch:
         c = np.zeros(max(self.ntasks, 1), dtype=np.uint8)
         _check(lib().bbtc_batch_autotune(self._h, _ptr(c)))
         return c[: self.ntasks]
+
+    def kinds(self) -> np.ndarray:
+        """Form per task (batch order): 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane."""
+        k = np.zeros(max(self.ntasks, 1), dtype=np.uint8)
+        _check(lib().bbtc_batch_task_kinds(self._h, _ptr(k)))
+        return k[: self.ntasks]
 
     def set_forms(self, kernel: str = "auto", choice=None):
         ch = None if choice is None else np.ascontiguousarray(choice, dtype=np.uint8)

@@ -1071,6 +1071,7 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
             d.col = col0[gi] + L.col_base[t];
             d.nz = nz0[gi] + L.nz_base[t];
             d.nzr = L.nz_rows[t];
+            d.maxd = L.max_row[t];
             if (L.dense_base[t] != ~0ull) d.dense = dn0[gi] + L.dense_base[t];
           }
           A->htiles[tile0[gi] + t] = d;
@@ -1144,6 +1145,12 @@ bbtc_status bbtc_batch_tasks(const bbtc_batch* b, uint32_t* graph, bbtc_task* ta
     if (graph) graph[q] = b->refs[q].graph;
     if (tasks) tasks[q] = bbtc_task{b->task_ijk[3 * q], b->task_ijk[3 * q + 1], b->task_ijk[3 * q + 2]};
   }
+  return BBTC_OK;
+}
+
+bbtc_status bbtc_batch_task_kinds(const bbtc_batch* b, uint8_t* kinds) {
+  DEV_REQUIRE(b && (kinds || b->refs.empty()), "null argument");
+  for (uint64_t q = 0; q < b->refs.size(); ++q) kinds[q] = uint8_t(b->plan->hkind[q]);
   return BBTC_OK;
 }
 

@@ -16,15 +16,22 @@ Prints one JSON line:
                 vs the measured streaming-read bandwidth (achievable); HBM
                 traffic from the committed rocprofv3 counters; the
                 compulsory bytes (each task's three tiles read once)
-  end_to_end_s  generate + partition + layout, upload, median count
-  cpu_baseline  the CPU oracle (restated reference CPU path) on a sample of
-                the same tasks, scaled by algorithmic bytes
-  c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), tuned forms:
-                edges/s and roofline fraction of the same count
+                (forms: --kernel, default auto, the measured-cost rule
+                without per-graph autotuning)
+  end_to_end_s  generate + partition + layout, upload (with the task work
+                statistics), autotuning (--kernel tuned only), median count
+  cpu_baseline  the CPU oracle (restated reference CPU path) on a
+                systematic 1/s sample of every task's rows, scaled by s
+                (--cpu-full: every task once, measured)
+  c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), AUTO forms:
+                edges/s and roofline fraction of the same count, and the
+                oracle over all of its tasks
   batched       4,096 uneven graphs (70% RMAT s10-18, 30% G(n,m)) resident,
-                one captured persistent launch vs the plain per-graph launch:
-                graphs/s, per-graph p50/p99, bit-exact per-graph counts, and
-                the batch's algorithmic bytes / launch time vs 8 TB/s
+                one captured persistent launch vs the plain per-graph launch,
+                both with the AUTO forms (and the batch with autotuned forms
+                beside it): graphs/s, per-graph p50/p99, bit-exact per-graph
+                counts, and the batch's algorithmic bytes / launch time vs
+                8 TB/s
   serving       requests (50% RMAT s12-24 ef16, 50% G(n,m) n 2^12-2^22 d 32)
                 through the continuous-batching server vs the plain
                 sequential launch: capacity, and p50/p99 latency at Poisson
@@ -47,15 +54,16 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "count_probe_rmat-s26-ef16_pal32_traffic.json")
 
 
-def measured_traffic(workload: str):
+def measured_traffic(workload: str, forms: str):
     """HBM bytes per count from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    runs of this workload (scaled by the copy-kernel calibration), or None."""
+    runs of this workload and kernel-form choice (scaled by the copy-kernel
+    calibration), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None, None
-    if t.get("workload") != workload:
+    if t.get("workload") != workload or t.get("forms") != forms:
         return None, None
     return int(t["traffic_bytes"]), os.path.relpath(TRAFFIC_FILE, ROOT)
 
@@ -98,10 +106,13 @@ def host_info(nthreads):
     return {"nproc": usable, "omp_threads": nthreads, "cpu_model": model}
 
 
-def cpu_baseline(g, cuts, tasks, per_task_gpu, bytes_per_task, total_bytes, m, budget_s):
-    """Time the oracle on a sample of tasks (every s-th in lexicographic order)
-    whose algorithmic bytes are about budget_s of CPU work; scale to the whole
-    task set by algorithmic bytes. Also checks the sample's counts bit-exact."""
+def cpu_baseline(g, cuts, tasks, per_task_gpu, m, budget_s, full=False):
+    """Time the oracle (the reference CPU path restated) on the box's host
+    cores. full: every task once, measured. Otherwise a systematic sample of
+    every task's rows, rows f with f % s == 0 of band i (1/s of each task's
+    rows, s sized from a 1/64 calibration run so the sample takes about
+    budget_s), its time scaled by s. Either way, the whole-task counts of
+    every 97th task (lexicographic) are checked bit-exact against the GPU."""
     import oracle as O
 
     t0 = time.time()
@@ -109,37 +120,45 @@ def cpu_baseline(g, cuts, tasks, per_task_gpu, bytes_per_task, total_bytes, m, b
     log(f"cpu baseline: reference layout built in {time.time() - t0:.1f}s")
     lex = np.lexsort((tasks["k"], tasks["j"], tasks["i"]))
     nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # calibrate on a small stride, then size the sample for the budget
-    stride = max(1, len(tasks) // 16)
-    chosen = lex[::stride]
-    while True:
-        tt = np.stack([tasks["i"][chosen], tasks["j"][chosen], tasks["k"][chosen]], 1).astype(np.uint32)
+    tt = np.stack([tasks["i"][lex], tasks["j"][lex], tasks["k"][lex]], 1).astype(np.uint32)
+    chk = np.arange(0, len(lex), 97)
+    per, _ = ref.count(tt[chk], nthreads)
+    if not np.array_equal(per, per_task_gpu[lex[chk]]):
+        raise RuntimeError("GPU per-task counts differ from the CPU oracle")
+    if full:
         t0 = time.time()
-        per, _ = ref.count(tt, nthreads)
+        per, tot = ref.count(tt, nthreads)
         dt = time.time() - t0
-        if not np.array_equal(per, per_task_gpu[chosen]):
-            raise RuntimeError("GPU per-task counts differ from the CPU oracle on the sample")
-        sb = int(bytes_per_task[chosen].sum())
-        if dt >= budget_s / 4 or stride == 1:
-            break
-        stride = max(1, stride // 4)
-        chosen = lex[::stride]
-    est_full = dt * (total_bytes / max(sb, 1))
+        if not np.array_equal(per, per_task_gpu[lex]):
+            raise RuntimeError("GPU per-task counts differ from the CPU oracle")
+        est, how = dt, (f"oracle (C restatement of the reference CPU path) on all {len(tasks)} tasks once, "
+                        f"measured: {dt:.1f}s; every task's count bit-exact vs GPU")
+    else:
+        t0 = time.time()
+        ref.count_rows(tt, 64, 0, nthreads)
+        cal = time.time() - t0
+        stride = int(min(64, max(1, np.ceil(cal * 64 / budget_s))))
+        t0 = time.time()
+        ref.count_rows(tt, stride, 0, nthreads)
+        dt = time.time() - t0
+        est = dt * stride
+        how = (f"oracle (C restatement of the reference CPU path) on every one of the {len(tasks)} tasks, rows "
+               f"f % {stride} == 0 of each (1/{stride} of the rows, a systematic sample): {dt:.1f}s, scaled by "
+               f"{stride} to {est:.1f}s per count; whole-task counts of every 97th task bit-exact vs GPU")
     return {
-        "value": m / est_full,
+        "value": m / est,
         "unit": "edges/s",
         "cores": nthreads,
         "kind": "port",
         **host_info(nthreads),
-        "sample": (f"oracle (C restatement of the reference CPU path) on {len(chosen)} of {len(tasks)} "
-                   f"tasks (every {stride}th in lexicographic order), {dt:.1f}s for "
-                   f"{sb / 1e9:.1f} GB algorithmic; scaled by algorithmic bytes to the full task set "
-                   f"({est_full:.1f}s); per-task counts of the sample bit-exact vs GPU"),
+        "seconds_per_count": round(est, 2),
+        "sample": how,
     }
 
 
-def count_c2(B, dev, steps):
-    """G(n=2^22, m=2^26) seed 2, pal 16, tuned forms: one resident count."""
+def count_c2(B, dev, steps, cpu=True):
+    """G(n=2^22, m=2^26) seed 2, pal 16, AUTO forms: one resident count, and
+    the oracle on every task beside it."""
     t0 = time.time()
     g = B.Graph.gnm_device(dev, 1 << 22, 1 << 26, 2)
     cuts = g.partition("pal", 16)
@@ -147,22 +166,26 @@ def count_c2(B, dev, steps):
     tasks = L.tasks("desc")
     _, total_bytes, _ = L.task_bytes(tasks, 4)
     dg = dev.upload(L)
-    choice, _ = dg.autotune(tasks)
-    plan = dg.plan(tasks, "auto", choice)
+    plan = dg.plan(tasks, "auto")
     prep = time.time() - t0
     plan.time(1)
     ms = plan.time(steps)
     per, total = plan.results()
+    kinds = plan.kinds()
     plan.destroy()
     dg.release()
     mean_s = float(np.mean(ms)) * 1e-3
     log(f"c2: G(n=2^22, m=2^26) pal 16: {total} triangles, events ms {np.round(ms, 3).tolist()} "
         f"(prepared in {prep:.1f}s)")
-    return {"workload": "gnm-n22-m26 pal ncut16", "metric": "edges/s", "value": g.m / mean_s,
-            "ms_per_count": mean_s * 1e3, "triangles": int(total), "tasks": int(len(tasks)),
-            "algorithmic_bytes": int(total_bytes), "offset_bytes": 4,
-            "roofline_frac": total_bytes / mean_s / 1e9 / HBM_PEAK_GBS,
-            "forms_merge_bitmap_dense_rdense_lane": np.bincount(choice, minlength=6)[1:].tolist()}
+    out = {"workload": "gnm-n22-m26 pal ncut16", "metric": "edges/s", "value": g.m / mean_s,
+           "ms_per_count": mean_s * 1e3, "triangles": int(total), "tasks": int(len(tasks)),
+           "algorithmic_bytes": int(total_bytes), "offset_bytes": 4,
+           "roofline_frac": total_bytes / mean_s / 1e9 / HBM_PEAK_GBS,
+           "forms_merge_bitmap_dense_rdense_lane": np.bincount(kinds, minlength=6)[1:].tolist()}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(g, cuts, tasks, per, g.m, 0, full=True)
+        log(f"c2 cpu baseline: {out['cpu_baseline']['seconds_per_count']} s per count")
+    return out
 
 
 def main():
@@ -175,8 +198,10 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--ncut", type=int, default=32)
     ap.add_argument("--partitioner", default="pal")
-    ap.add_argument("--kernel", default="tuned", help="tuned | auto | merge | bitmap | dense | rdense | lane")
+    ap.add_argument("--kernel", default="auto", help="auto | tuned | merge | bitmap | dense | rdense | lane")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline over every task once (about 2.5 minutes on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the G(n=2^22, m=2^26) count")
     ap.add_argument("--c2-only", action="store_true",
@@ -197,7 +222,7 @@ def main():
     offset_bytes = 8 if args.scale >= 26 else 4
     dev = B.Device(0)
     if args.c2_only:
-        print(json.dumps(count_c2(B, dev, steps=args.steps)), flush=True)
+        print(json.dumps(count_c2(B, dev, steps=args.steps, cpu=not args.no_cpu)), flush=True)
         return
     if args.batched_only:
         from bbtc_amd import workloads as W
@@ -246,6 +271,7 @@ def main():
     dev.synchronize()
     wall = time.perf_counter() - t0
     per, total = plan.results()
+    forms = np.bincount(plan.kinds(), minlength=6)[1:].tolist()
     ms_per_step = wall * 1e3 / args.steps
     ev_mean_s = float(np.mean(ms)) * 1e-3
     ev_median_ms = float(np.median(ms))
@@ -257,10 +283,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(g, cuts, tasks, per, bytes_per_task, total_bytes, m, args.cpu_seconds)
+        cpu = cpu_baseline(g, cuts, tasks, per, m, args.cpu_seconds, full=args.cpu_full)
 
     workload = f"rmat-s{args.scale}-ef{args.edge_factor} {args.partitioner} ncut{args.ncut}"
-    traffic, traffic_src = measured_traffic(workload) if args.kernel == "tuned" else (None, None)
+    traffic, traffic_src = measured_traffic(workload, args.kernel)
     out = {
         "metric": "edges/s (RMAT s26 block-based triangle count)",
         "value": m / (ms_per_step * 1e-3),
@@ -279,11 +305,12 @@ def main():
             "n": n, "m": m, "ncut": args.ncut, "tasks": int(len(tasks)),
             "partitioner": args.partitioner, "kernel": args.kernel,
             "triangles": int(total), "autotune_s": round(tune_s, 2),
+            "forms_merge_bitmap_dense_rdense_lane": forms,
         },
         "end_to_end_s": {
             "generate_partition_layout": round(prep_s, 3), "upload": round(upload_s, 3),
-            "count_median": round(ev_median_ms * 1e-3, 4),
-            "total": round(prep_s + upload_s + ev_median_ms * 1e-3, 3),
+            "autotune": round(tune_s, 3), "count_median": round(ev_median_ms * 1e-3, 4),
+            "total": round(prep_s + upload_s + tune_s + ev_median_ms * 1e-3, 3),
         },
         "triangles_per_s": total / (ms_per_step * 1e-3),
         "roofline": {
@@ -293,6 +320,11 @@ def main():
             "kernel": "count_probe", "event_ms_mean": ev_mean_s * 1e3, "event_ms_median": ev_median_ms,
             "achievable": stream_gbs, "frac_achievable": achieved / stream_gbs,
             "compulsory_bytes": min_bytes,
+            # the HBM bytes rocprofv3 counted for this workload (traffic), per
+            # second of this run's count, against the spec peak
+            "hbm_frac": traffic / ev_mean_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "traffic_over_compulsory": traffic / min_bytes if traffic else None,
+            "traffic_over_algorithmic": traffic / total_bytes if traffic else None,
         },
         "cpu_baseline": cpu,
     }
@@ -306,7 +338,11 @@ def main():
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
 
     if not args.no_c2:
-        out["c2"] = rounded(count_c2(B, dev, steps=3))
+        c2 = count_c2(B, dev, steps=3, cpu=not args.no_cpu)
+        cpu2 = c2.pop("cpu_baseline", None)
+        out["c2"] = rounded(c2)
+        if cpu2:
+            out["c2"]["cpu_baseline"] = cpu2
 
     if args.batched_graphs:
         res = W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log)

@@ -383,6 +383,9 @@ uint64_t bbtc_batch_ntasks(const bbtc_batch* b);
 uint64_t bbtc_batch_bytes(const bbtc_batch* b);
 /* The batch's task order: graph index and (i,j,k) of each task (either may be NULL). */
 bbtc_status bbtc_batch_tasks(const bbtc_batch* b, uint32_t* graph, bbtc_task* tasks);
+/* Form the batch's plan chose per task (batch task order): 1 merge, 2 bitmap,
+ * 3 dense, 4 rdense, 5 lane. */
+bbtc_status bbtc_batch_task_kinds(const bbtc_batch* b, uint8_t* kinds);
 /* Measures every form on the batch (as bbtc_autotune) and replans with the
  * cheapest per task; choice_out (may be NULL) receives the forms. */
 bbtc_status bbtc_batch_autotune(bbtc_batch* b, uint8_t* choice_out);

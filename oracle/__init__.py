@@ -50,6 +50,8 @@ def lib():
         L.orc_tasks.restype = u64
         L.orc_count.argtypes = [vp, vp, u64, C.c_int, vp]
         L.orc_count.restype = u64
+        L.orc_count_rows.argtypes = [vp, vp, u64, C.c_int, vp, u32, u32]
+        L.orc_count_rows.restype = u64
         L.orc_count_definition.argtypes = [vp, vp, u32, vp, u32, vp]
         L.orc_count_definition.restype = u64
         L.orc_brute_force.argtypes = [vp, vp, u32]
@@ -133,6 +135,14 @@ class RefLayout:
         t = np.ascontiguousarray(np.asarray(tasks, dtype=np.uint32).reshape(-1, 3))
         per = np.zeros(len(t), dtype=np.uint64)
         tot = lib().orc_count(self._h, _p(t), len(t), nthreads, _p(per))
+        return per, tot
+
+    def count_rows(self, tasks, stride, phase=0, nthreads=0):
+        """Counts of only the rows f % stride == phase of each task (a
+        systematic sample of every task's rows, for timing)."""
+        t = np.ascontiguousarray(np.asarray(tasks, dtype=np.uint32).reshape(-1, 3))
+        per = np.zeros(len(t), dtype=np.uint64)
+        tot = lib().orc_count_rows(self._h, _p(t), len(t), nthreads, _p(per), stride, phase)
         return per, tot
 
     def __del__(self):

@@ -642,7 +642,11 @@ static long list_intersect(const orc_bcsr* B, uint64_t ab_src, uint64_t bc_src, 
   return c;
 }
 
-static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, uint64_t* hcap) {
+/* Rows f of band i with f % stride == phase only (stride 1: the whole task,
+ * as the reference counts it; larger strides time a systematic sample of
+ * every task's rows). */
+static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, uint64_t* hcap,
+                       uint32_t stride, uint32_t phase) {
   const uint32_t P = B->P;
   const uint32_t ab = t[0] * P + t[1], bc = t[1] * P + t[2], ac = t[0] * P + t[2];
   if (B->deg[bc] == 0 || B->deg[ac] == 0 || B->deg[ab] == 0) return 0;
@@ -661,7 +665,7 @@ static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, ui
     }
     uint32_t* hm = *hmap;
     for (uint32_t x = 0; x < width; ++x) hm[x] = 0x7FFFFFFFu;
-    for (uint32_t f = 0; f < h; ++f) {
+    for (uint32_t f = phase; f < h; f += stride) {
       if (rows[ac_src + f + 1] == rows[ac_src + f]) continue;
       if (rows[ab_src + f + 1] - rows[ab_src + f] < 4) {
         c += list_intersect(B, ab_src, bc_src, ac_src, f);
@@ -674,7 +678,7 @@ static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, ui
       }
     }
   } else {
-    for (uint32_t f = 0; f < h; ++f) {
+    for (uint32_t f = phase; f < h; f += stride) {
       if (rows[ac_src + f + 1] == rows[ac_src + f]) continue;
       c += list_intersect(B, ab_src, bc_src, ac_src, f);
     }
@@ -682,9 +686,24 @@ static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, ui
   return c;
 }
 
-/* Workers claim tasks from the back of the list (RunCpuKernels). */
+/* Workers claim tasks from the back of the list (RunCpuKernels); with
+ * stride > 1 only the rows f % stride == phase of each task. */
+static uint64_t count_rows(const orc_bcsr* B, const uint32_t* ijk, uint64_t ntasks, int nthreads,
+                           uint64_t* per_task, uint32_t stride, uint32_t phase);
+
 ORC_API uint64_t orc_count(const orc_bcsr* B, const uint32_t* ijk, uint64_t ntasks, int nthreads,
                            uint64_t* per_task) {
+  return count_rows(B, ijk, ntasks, nthreads, per_task, 1, 0);
+}
+
+ORC_API uint64_t orc_count_rows(const orc_bcsr* B, const uint32_t* ijk, uint64_t ntasks, int nthreads,
+                                uint64_t* per_task, uint32_t stride, uint32_t phase) {
+  if (stride == 0) stride = 1;
+  return count_rows(B, ijk, ntasks, nthreads, per_task, stride, phase % stride);
+}
+
+static uint64_t count_rows(const orc_bcsr* B, const uint32_t* ijk, uint64_t ntasks, int nthreads,
+                           uint64_t* per_task, uint32_t stride, uint32_t phase) {
   int64_t next = (int64_t)ntasks - 1;
   uint64_t total = 0;
   const int T = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -694,7 +713,7 @@ ORC_API uint64_t orc_count(const orc_bcsr* B, const uint32_t* ijk, uint64_t ntas
     uint64_t hcap = 0;
     for (int64_t q = __atomic_fetch_sub(&next, 1, __ATOMIC_RELAXED); q >= 0;
          q = __atomic_fetch_sub(&next, 1, __ATOMIC_RELAXED)) {
-      const long c = task_count(B, ijk + 3 * q, &hmap, &hcap);
+      const long c = task_count(B, ijk + 3 * q, &hmap, &hcap, stride, phase);
       if (per_task) per_task[q] = (uint64_t)c;
       total += (uint64_t)c;
     }

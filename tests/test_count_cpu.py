@@ -189,3 +189,26 @@ def test_long_bc_row_merges():
     for part, p in (("uni", 1), ("uni", 3)):
         _, _, _, per, tot = count_all(g, part, p)
         assert tot == 1 == int(per.sum())
+
+
+@pytest.mark.parametrize("stride", [2, 5, 64])
+def test_oracle_row_samples_add_up(stride):
+    # the oracle's row samples (rows f % stride == phase of every task, the
+    # CPU baseline's timing sample) add up to the whole tasks' counts, also
+    # for a task of average degree above 10,000 (the stamp-table form): 1,025
+    # apex rows joined to all 10,001 vertices of a path
+    apex, base = 1025, 10001
+    n = apex + base
+    row_len = np.concatenate([np.full(apex, base), np.ones(base - 1), [0]]).astype(np.uint64)
+    off = np.concatenate([[0], np.cumsum(row_len)]).astype(np.uint64)
+    cols = np.concatenate([np.tile(np.arange(apex, n, dtype=np.uint32), apex),
+                           np.arange(apex + 1, n, dtype=np.uint32)])
+    dense = (B.Graph.from_csr(off, cols), None, np.array([0, apex, n + 1], np.uint32))
+    for g, part, p in ((B.Graph.rmat(14, 16, seed=3), "pal", 6), (B.Graph.gnm(600, 60000, 15), "uni", 3), dense):
+        cuts = g.partition(part, p) if part else p
+        R = O.RefLayout(g.offsets(), g.cols(), cuts)
+        t = R.tasks("lex")
+        full, tot = R.count(t)
+        parts = [R.count_rows(t, stride, ph) for ph in range(stride)]
+        assert sum(x[1] for x in parts) == tot
+        assert np.array_equal(sum(x[0] for x in parts), full)

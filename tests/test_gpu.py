@@ -378,6 +378,13 @@ def test_batch_matches_per_graph(dev, order):
     b = B.Batch(dev, Ls, "auto", order=order)
     gidx, bt = b.tasks()
     assert len(gidx) == sum(len(r[0]) for r in ref)
+    # the lane form only where no row of the task's three tiles holds more
+    # than 32 entries (the batch's tile table carries each tile's longest row)
+    kinds = b.kinds()
+    for g, (i, j, k), kd in zip(gidx.tolist(), bt.tolist(), kinds.tolist()):
+        longest = max(Ls[g].tile(i, j).max_row, Ls[g].tile(j, k).max_row, Ls[g].tile(i, k).max_row)
+        assert kd != 5 or longest <= 32, (g, i, j, k)
+    assert (kinds != 5).any()
     # per task of the batch: the oracle's count of that graph's task
     lex_index = [{tuple(x): q for q, x in enumerate(L.tasks("lex").tolist())} for L in Ls]
     want = np.array([ref[g][0][lex_index[g][tuple(x)]] for g, x in zip(gidx, bt.tolist())], dtype=np.uint64)
