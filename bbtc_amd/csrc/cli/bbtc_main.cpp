@@ -231,15 +231,51 @@ bool ends_with(const std::string& s, const std::string& t) {
   return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
 }
 
-// Forms file: "bbtc-forms 1", then "<ncut> <ntasks>", the ncut+1 cuts, and
-// one form per task in task order (1 merge, 2 bitmap, 3 dense, 4 rdense,
-// 5 lane, 0 the default rule), all as text. The cuts tie it to a layout.
-void write_forms(const std::string& path, const bbtc_layout* L, const std::vector<uint8_t>& choice,
-                 uint64_t ntasks) {
+// Fingerprint of a graph for the forms file: n, m and a hash of its offsets
+// and columns (chunks hashed in parallel, combined in chunk order).
+struct GraphPrint {
+  uint64_t n = 0, m = 0, hash = 0;
+};
+
+GraphPrint graph_print(const bbtc_hgraph* g) {
+  GraphPrint f;
+  uint32_t n = 0, nc = 0;
+  bbtc_hgraph_info(g, &n, &nc, &f.m);
+  f.n = n;
+  const uint64_t* off = bbtc_hgraph_offsets(g);
+  const uint32_t* col = bbtc_hgraph_cols(g);
+  auto mix = [](uint64_t h, uint64_t x) {
+    h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    return h * 0xBF58476D1CE4E5B9ull;
+  };
+  constexpr uint64_t kChunk = 1ull << 20;
+  const uint64_t nchunks = (f.m + kChunk - 1) / kChunk;
+  std::vector<uint64_t> part(nchunks, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < int64_t(nchunks); ++c) {
+    uint64_t h = uint64_t(c);
+    const uint64_t e = std::min<uint64_t>(f.m, uint64_t(c + 1) * kChunk);
+    for (uint64_t x = uint64_t(c) * kChunk; x < e; ++x) h = mix(h, col[x]);
+    part[c] = h;
+  }
+  uint64_t h = mix(f.n, f.m);
+  for (uint64_t x = 0; x <= f.n; ++x) h = mix(h, off[x]);
+  for (uint64_t v : part) h = mix(h, v);
+  f.hash = h;
+  return f;
+}
+
+// Forms file: "bbtc-forms 2", then "<ncut> <ntasks> <n> <m> <hash>" (the
+// graph's fingerprint), the ncut+1 cuts, and one form per task in task order
+// (1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane, 0 the default rule), all as
+// text. The fingerprint and the cuts tie it to one graph and layout.
+void write_forms(const std::string& path, const bbtc_hgraph* g, const bbtc_layout* L,
+                 const std::vector<uint8_t>& choice, uint64_t ntasks) {
   const uint32_t p = bbtc_layout_ncut(L);
   const uint32_t* cuts = bbtc_layout_cuts(L);
+  const GraphPrint f = graph_print(g);
   std::ofstream o(path);
-  o << "bbtc-forms 1\n" << p << " " << ntasks << "\n";
+  o << "bbtc-forms 2\n" << p << " " << ntasks << " " << f.n << " " << f.m << " " << f.hash << "\n";
   for (uint32_t b = 0; b <= p; ++b) o << cuts[b] << (b == p ? "\n" : " ");
   for (uint64_t q = 0; q < ntasks; ++q) o << int(choice[q]) << (q + 1 == ntasks ? "\n" : " ");
   if (!o) {
@@ -248,7 +284,8 @@ void write_forms(const std::string& path, const bbtc_layout* L, const std::vecto
   }
 }
 
-std::vector<uint8_t> read_forms(const std::string& path, const bbtc_layout* L, uint64_t ntasks) {
+std::vector<uint8_t> read_forms(const std::string& path, const bbtc_hgraph* g, const bbtc_layout* L,
+                                uint64_t ntasks) {
   auto fail = [&](const std::string& why) {
     std::cerr << "bbtc: forms file " << path << ": " << why << std::endl;
     std::exit(1);
@@ -256,10 +293,14 @@ std::vector<uint8_t> read_forms(const std::string& path, const bbtc_layout* L, u
   std::ifstream f(path);
   std::string magic;
   int version = 0;
-  uint64_t p = 0, nt = 0;
-  if (!(f >> magic >> version) || magic != "bbtc-forms" || version != 1) fail("not a bbtc forms file");
-  if (!(f >> p >> nt)) fail("truncated header");
+  uint64_t p = 0, nt = 0, n = 0, m = 0, hash = 0;
+  if (!(f >> magic >> version) || magic != "bbtc-forms" || (version != 1 && version != 2))
+    fail("not a bbtc forms file");
+  if (version == 1) fail("written without a graph fingerprint; write it again with --forms-out");
+  if (!(f >> p >> nt >> n >> m >> hash)) fail("truncated header");
   if (p != bbtc_layout_ncut(L) || nt != ntasks) fail("written for another number of cuts or tasks");
+  const GraphPrint gp = graph_print(g);
+  if (n != gp.n || m != gp.m || hash != gp.hash) fail("written for another graph");
   const uint32_t* cuts = bbtc_layout_cuts(L);
   for (uint64_t b = 0; b <= p; ++b) {
     uint64_t c = 0;
@@ -624,7 +665,7 @@ int main(int argc, char** argv) {
   uint64_t total = 0;
   bbtc_dgraph* dg = nullptr;
   if (ctx && !P.forms_in.empty()) {
-    choice = read_forms(P.forms_in, L, ntasks);
+    choice = read_forms(P.forms_in, g, L, ntasks);
   } else if (ctx && P.kernel == "tuned") {
     F.begin("Autotune kernel forms (s)");
     st = bbtc_graph_upload(ctx, L, &dg);
@@ -633,7 +674,7 @@ int main(int argc, char** argv) {
     st = bbtc_autotune(ctx, dg, tasks.data(), ntasks, choice.data(), nullptr);
     if (st) die(st, "autotuning");
     F.end();
-    if (!P.forms_out.empty()) write_forms(P.forms_out, L, choice, ntasks);
+    if (!P.forms_out.empty()) write_forms(P.forms_out, g, L, choice, ntasks);
   }
   for (int r = 0; r < P.repeat; ++r) {
     if (dg) {  // as ResetFlags (bb_tc_helper.cpp:31-50), every repeat starts without device tiles

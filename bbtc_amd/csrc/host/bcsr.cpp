@@ -5,10 +5,13 @@
 // slots of rows starting at sparse_tiles[t], each the global offset in cols
 // of that local row's first entry; cols hold tile-local column ids.
 //
-// The file keeps cuts[P] = N, so the slots of all tiles are exactly N*P
-// words; the end of the last tile's last row is E. (The reference's reader
-// allocates N*P words and reads that last end one word past them; its
-// writer side does not exist.) Reading rebuilds the CSR and this library's
+// The tiles' row slots are N*P words (cuts[P] = N), and the end of the last
+// tile's last row is one slot further. The reference's reader allocates and
+// reads (header N) * P words, and its counters read that end slot, so the
+// file announces N + 1 in the header: N*P slots, the end slot (E), and P - 1
+// more words of E, which the reader takes and nothing uses (its writer side
+// does not exist). Files whose header holds N itself, with the end slot
+// implied as E, are read too. Reading rebuilds the CSR and this library's
 // tile-major layout from the stored cuts.
 #include <cstdio>
 #include <cstring>
@@ -41,13 +44,13 @@ void get(std::FILE* f, void* p, size_t bytes, const std::string& path) {
 void save_bcsr(const Layout& L, const std::string& path) {
   const uint32_t P = L.P, N = L.n;
   if (L.m >= (1ull << 32)) fail(BBTC_ERR_INVALID, ".bcsr holds at most 2^32-1 entries");
-  if (uint64_t(N) * P >= (1ull << 32)) fail(BBTC_ERR_INVALID, ".bcsr holds at most 2^32-1 row slots");
+  if ((uint64_t(N) + 1) * P >= (1ull << 32)) fail(BBTC_ERR_INVALID, ".bcsr holds at most 2^32-1 row slots");
   std::vector<uint32_t> cuts(L.cuts.begin(), L.cuts.end());
   cuts[P] = N;
   std::vector<uint32_t> tiles(size_t(P) * P + 1, 0);
   for (uint32_t i = 0; i < P; ++i)
     for (uint32_t k = 0; k < P; ++k) tiles[size_t(i) * P + k + 1] = tiles[size_t(i) * P + k] + (cuts[i + 1] - cuts[i]);
-  std::vector<uint32_t> rows(size_t(N) * P, 0);
+  std::vector<uint32_t> rows((size_t(N) + 1) * P, uint32_t(L.m));  // the slots past N*P: E
   std::vector<uint32_t> cols(L.m);
   uint64_t at = 0;  // entries written before the current tile
   for (uint32_t i = 0; i < P; ++i)
@@ -66,7 +69,8 @@ void save_bcsr(const Layout& L, const std::string& path) {
   fc.f = std::fopen(path.c_str(), "wb");
   if (!fc.f) fail(BBTC_ERR_IO, "cannot create " + path);
   const uint32_t E = uint32_t(L.m);
-  put(fc.f, &N, 4, path);
+  const uint32_t Nfile = N + 1;
+  put(fc.f, &Nfile, 4, path);
   put(fc.f, &E, 4, path);
   put(fc.f, &P, 4, path);
   put(fc.f, cuts.data(), cuts.size() * 4, path);
@@ -86,6 +90,8 @@ void load_bcsr(const std::string& path, Csr& g, std::vector<uint32_t>& cuts_out)
   get(fc.f, &P, 4, path);
   if (P < 1 || P > 65535) fail(BBTC_ERR_FORMAT, "bad number of cuts in " + path);
   if (uint64_t(N) * P >= (1ull << 32)) fail(BBTC_ERR_FORMAT, "too many row slots in " + path);
+  // N here is the header's; the graph has cuts[P] vertices (N, or N - 1 when
+  // the file carries the end slot)
   // the arrays the header announces must be in the file before they are
   // allocated, so a damaged header fails at once instead of after a large
   // allocation
@@ -99,7 +105,14 @@ void load_bcsr(const std::string& path, Csr& g, std::vector<uint32_t>& cuts_out)
   get(fc.f, tiles.data(), tiles.size() * 4, path);
   get(fc.f, rows.data(), rows.size() * 4, path);
   get(fc.f, cols.data(), cols.size() * 4, path);
-  if (cuts[0] != 0 || cuts[P] != N) fail(BBTC_ERR_FORMAT, "cuts must run from 0 to N in " + path);
+  if (cuts[0] != 0 || (cuts[P] != N && uint64_t(cuts[P]) + 1 != N))
+    fail(BBTC_ERR_FORMAT, "cuts must run from 0 to N or N - 1 in " + path);
+  const bool end_slot = cuts[P] != N;  // N*P slots of rows, then the end slot and padding
+  N = cuts[P];
+  const uint64_t used = uint64_t(N) * P;
+  if (end_slot)
+    for (uint64_t s = used; s < rows.size(); ++s)
+      if (rows[s] != E) fail(BBTC_ERR_FORMAT, "end slot is not E in " + path);
   for (uint32_t b = 0; b < P; ++b)
     if (cuts[b] > cuts[b + 1]) fail(BBTC_ERR_FORMAT, "decreasing cuts in " + path);
   for (uint32_t i = 0; i < P; ++i)
@@ -109,8 +122,8 @@ void load_bcsr(const std::string& path, Csr& g, std::vector<uint32_t>& cuts_out)
         fail(BBTC_ERR_FORMAT, "tile slots do not match the cuts in " + path);
     }
   // slot s ends where slot s + 1 starts; the last slot ends at E
-  auto slot_end = [&](uint64_t s) { return s + 1 < rows.size() ? rows[s + 1] : E; };
-  for (uint64_t s = 0; s < rows.size(); ++s)
+  auto slot_end = [&](uint64_t s) { return s + 1 < used ? rows[s + 1] : E; };
+  for (uint64_t s = 0; s < used; ++s)
     if (rows[s] > slot_end(s) || slot_end(s) > E) fail(BBTC_ERR_FORMAT, "row offsets out of order in " + path);
   std::vector<uint32_t> band_of(N);
   for (uint32_t b = 0; b < P; ++b)

@@ -139,10 +139,13 @@ bbtc_status bbtc_layout_tile(const bbtc_layout* L, uint32_t i, uint32_t k, bbtc_
 /* Block CSR files (.bcsr): replaces BlockCSR::DeSerialize
  * (block_csr.cpp:215-255) and supplies the writer the reference lacks. Same
  * fields and widths (uint32 N, E, P, cuts[P+1], sparse_tiles[P*P+1],
- * rows[N*P], cols[E], tiles row-major over the full grid), written with
- * cuts[P] = N so the N*P row slots are exactly the tiles' rows; the end of
- * the last row is E. Loading rebuilds the graph and the layout with the
- * stored cuts (cuts[P] set to N+1 as the driver does), skipping
+ * rows[N*P], cols[E], tiles row-major over the full grid). A graph of n
+ * vertices is written with cuts[P] = n, so n*P row slots are the tiles'
+ * rows, and with N = n + 1 in the header: the slot after them holds the end
+ * of the last row (E), which the reference's counters read, and the P - 1
+ * words after it repeat E. Files whose header holds n itself (the end slot
+ * implied) are read too. Loading rebuilds the graph and the layout with the
+ * stored cuts (cuts[P] set to n+1 as the driver does), skipping
  * partitioning and the CSR-to-tile pass. */
 bbtc_status bbtc_layout_save_bcsr(const bbtc_layout* L, const char* path);
 bbtc_status bbtc_layout_load_bcsr(const char* path, bbtc_hgraph** graph_out, bbtc_layout** layout_out);

@@ -1,8 +1,10 @@
 """Block CSR files (.bcsr): the reference's serialised BlockCSR fields
-(block_csr.cpp:215-255) written with cuts[P] = N. A layout written and read
-back keeps its graph, its cuts and every per-task count (pinned by the golden
+(block_csr.cpp:215-255) written with cuts[P] = n and n + 1 in the header
+(the end slot of the last row included). A layout written and read back
+keeps its graph, its cuts and every per-task count (pinned by the golden
 vectors the reference's own code wrote); the file's arrays follow the
-reference's definitions, checked here against a direct construction."""
+reference's definitions, checked here against a direct construction; the
+reference's own binary reads the files (tests/test_reference_binary.py)."""
 import struct
 import subprocess
 
@@ -42,13 +44,14 @@ def test_file_layout(tmp_path):
     raw = np.frombuffer(path.read_bytes(), dtype=np.uint32)
     P = 3
     N_, E_, P_ = raw[:3]
-    assert (N_, E_, P_) == (n, m, P)
+    assert (N_, E_, P_) == (n + 1, m, P)
     c = raw[3:3 + P + 1]
     assert np.array_equal(c[:P], cuts[:P]) and c[P] == n
     tiles = raw[3 + P + 1: 3 + P + 1 + P * P + 1]
     rows = raw[3 + P + 1 + P * P + 1: 3 + P + 1 + P * P + 1 + n * P]
-    cols = raw[3 + P + 1 + P * P + 1 + n * P:]
-    assert len(cols) == m
+    end = raw[3 + P + 1 + P * P + 1 + n * P: 3 + P + 1 + P * P + 1 + (n + 1) * P]
+    cols = raw[3 + P + 1 + P * P + 1 + (n + 1) * P:]
+    assert len(cols) == m and np.all(end == m)
     h = np.diff(c.astype(np.int64))
     assert np.array_equal(tiles, np.concatenate([[0], np.cumsum(np.repeat(h, P))]))
     # entries of tile (y, x) in row-major tile order, each row's entries in column order
@@ -64,6 +67,23 @@ def test_file_layout(tmp_path):
                 at += len(s)
     assert np.array_equal(rows, np.array(want_rows)) and np.array_equal(cols, np.array(want_cols))
     assert band.max() == P - 1
+
+
+def test_file_without_end_slot_reads(tmp_path):
+    # a file with n in the header and the end slot implied reads the same
+    g = B.Graph.rmat(9, 8, seed=4)
+    L = B.Layout(g, g.partition("pal", 3))
+    path = tmp_path / "x.bcsr"
+    L.save_bcsr(str(path))
+    raw = np.frombuffer(path.read_bytes(), dtype=np.uint32).copy()
+    n, P = g.n, 3
+    head = 3 + P + 1 + P * P + 1
+    short = np.concatenate([[n], raw[1:head + n * P], raw[head + (n + 1) * P:]]).astype(np.uint32)
+    p2 = tmp_path / "y.bcsr"
+    p2.write_bytes(short.tobytes())
+    a, b = B.Layout.load_bcsr(str(path)), B.Layout.load_bcsr(str(p2))
+    assert np.array_equal(a.cuts, b.cuts) and np.array_equal(a.graph.cols(), b.graph.cols())
+    assert np.array_equal(a.graph.offsets(), g.offsets())
 
 
 def test_malformed_files(tmp_path):

@@ -239,7 +239,14 @@ def test_cli_forms_out_and_in(tmp_path):
         per[name] = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
     assert per["out"] == per["in"] == {tuple(x[:3]): x[3] for x in run["tasks"]}
     words = forms.read_text().split()
-    assert words[:2] == ["bbtc-forms", "1"] and int(words[3]) == len(run["tasks"])
+    g = graph_from_spec(d["graph"])
+    assert words[:2] == ["bbtc-forms", "2"] and int(words[3]) == len(run["tasks"])
+    assert (int(words[4]), int(words[5])) == (g.n, g.m)
+    # the same cuts and task count but another graph's fingerprint: refused
+    other = tmp_path / "other.forms"
+    other.write_text(" ".join(words[:6] + [str(int(words[6]) ^ 1)] + words[7:]) + "\n")
+    r = subprocess.run(base + ["--forms-in", str(other)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "written for another graph" in r.stderr
     r = subprocess.run([B.CLI_PATH, "--graph", path, "--partitioner", "uni", "--ncut", str(run["ncut"]),
                         "--repeat", "1", "--ngpu", "1", "--forms-in", str(forms)], capture_output=True, text=True,
                        timeout=120)

@@ -2,7 +2,8 @@
 built from its sources (oracle/_ref/bbtc_cpu, `make -C oracle ref`, only in
 the container that holds the reference tree): the same .bbtc file and
 partitioner give the same triangle count as the product's CLI on its CPU path
-and its device-free C-ABI count. Skipped where the binary is absent."""
+and its device-free C-ABI count, and the reference reads the .bcsr files the
+product writes. Skipped where the binary is absent."""
 import os
 import re
 import subprocess
@@ -38,3 +39,23 @@ def test_reference_binary_agrees(tmp_path, spec, part, ncut):
     for key in ("Number of Vertices", "Number of Edges", "Number of Cuts", "Number of Tasks"):
         line = re.search(rf"{key}: (\d+)", ref.stdout).group(0)
         assert line in ours.stdout, key
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference CPU binary not built here")
+@pytest.mark.parametrize("spec,part,ncut", [(("rmat", 14, 8, 5), "pal", 8), (("gnm", 30000, 240000, 7), "uni", 4),
+                                            (("rmat", 12, 24, 8), "rac", 6)])
+def test_reference_reads_written_bcsr(tmp_path, spec, part, ncut):
+    # BlockCSR::DeSerialize (block_csr.cpp:215-255) on a file this library
+    # wrote: the reference counts the same triangles with the same cuts and
+    # tasks, and reports the header's N (the vertices plus the end slot)
+    kind, a, b, seed = spec
+    g = B.Graph.rmat(a, b, seed=seed) if kind == "rmat" else B.Graph.gnm(a, b, seed)
+    L = B.Layout(g, g.partition(part, ncut))
+    path = str(tmp_path / "g.bcsr")
+    L.save_bcsr(path)
+    ref = subprocess.run([REF, "--graph", path, "--repeat", "1"], capture_output=True, text=True, timeout=300)
+    assert ref.returncode == 0, ref.stderr
+    want = L.count_cpu(L.tasks("desc"))[1]
+    assert f"Number of Triangles: {want}" in ref.stdout
+    assert f"Number of Cuts: {ncut}" in ref.stdout and f"Number of Edges: {g.m}" in ref.stdout
+    assert f"Number of Vertices: {g.n + 1}" in ref.stdout
