@@ -139,6 +139,15 @@ constexpr unsigned long long part_item(unsigned long long base, unsigned q, unsi
   return base + ((x >> shift) * kQueues + q) * C + (x & ((1ull << shift) - 1));
 }
 
+// One tile whose bitmap rows are built on the device: its row offsets (h + 1
+// tile-local words at rp), columns (at col), bitmap rows (h rows of dwpr words
+// at dense), and row0, the number of bitmap rows of the jobs before it.
+struct DenseJob {
+  uint64_t rp, col, dense, row0;
+  uint32_t h, dwpr;
+};
+static_assert(sizeof(DenseJob) == 40, "DenseJob layout");
+
 // Wedges [w0, w1) of launch-local item `item` (in its flattened wedge order),
 // counted by the second phase in chunks of wc wedges.
 struct SpillRec {

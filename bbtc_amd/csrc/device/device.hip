@@ -106,6 +106,38 @@ void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s) {
   HIP_OK(hipGetLastError());
 }
 
+std::vector<DenseJob> dense_jobs(const Layout& L, uint64_t rp0, uint64_t col0, uint64_t dense0,
+                                 uint64_t row0, uint64_t* rows_out) {
+  std::vector<DenseJob> jobs;
+  uint64_t rows = row0;
+  for (uint32_t i = 0; i < L.P; ++i)
+    for (uint32_t k = i; k < L.P; ++k) {
+      const size_t t = L.tile(i, k);
+      if (L.dense_base[t] == ~0ull) continue;
+      const uint32_t h = L.height(i);
+      if (L.dense_base[t] + uint64_t(h) * L.dwpr[k] > L.dense_words)
+        fail(BBTC_ERR_INTERNAL, "bitmap rows outside the layout's bitmap area");
+      jobs.push_back(DenseJob{rp0 + L.rp_base[t], col0 + L.col_base[t], dense0 + L.dense_base[t], rows, h, L.dwpr[k]});
+      rows += h;
+    }
+  if (rows_out) *rows_out = rows - row0;
+  return jobs;
+}
+
+void enqueue_dense_build(const uint32_t* rowptr, const uint32_t* cols, uint32_t* dense, uint64_t dense_words,
+                         const DenseJob* jobs_dev, uint64_t njobs, uint64_t rows, hipStream_t s) {
+  const uint64_t zb = std::min<uint64_t>((dense_words + kBlock - 1) / kBlock, 65536);
+  hipLaunchKernelGGL(fill_zero_u32, dim3(unsigned(std::max<uint64_t>(zb, 1))), dim3(kBlock), 0, s, dense, dense_words);
+  HIP_OK(hipGetLastError());
+  if (!njobs || !rows) return;
+  if (njobs >= 0xFFFFFFFFull) fail(BBTC_ERR_INVALID, "too many bitmap tiles");
+  const uint64_t waves = (rows + kWave - 1) / kWave;
+  const uint64_t blocks = std::min<uint64_t>((waves + kWavesPerBlock - 1) / kWavesPerBlock, 16384);
+  hipLaunchKernelGGL(build_bitmaps, dim3(unsigned(std::max<uint64_t>(blocks, 1))), dim3(kBlock), 0, s, rowptr, cols,
+                     dense, jobs_dev, uint32_t(njobs), rows);
+  HIP_OK(hipGetLastError());
+}
+
 void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t items, bool lane_only,
                    uint64_t chunk, hipStream_t s, uint32_t defer_steps) {
   hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, i0, i0 + items, (unsigned long long)chunk);
@@ -479,6 +511,17 @@ void plan_launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot) {
 
 uint64_t plan_items(const bbtc_plan* p, uint64_t t0, uint64_t t1) { return p->hprefix[t1] - p->hprefix[t0]; }
 
+bool graph_task_work(const bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n, double* out) {
+  if (g->P == 0 || g->lex_work.empty()) return false;
+  const uint32_t P = g->P;
+  for (uint64_t q = 0; q < n; ++q) {
+    const bbtc_task& x = tasks[q];
+    const TaskWork& s = g->lex_work[g->lex_ij[size_t(x.i) * P + x.j] + (x.k - x.j)];
+    out[q] = 16.0 * s.R + 12.0 * s.W + 4.0 * s.A + 4.0 * s.Pbc;
+  }
+  return true;
+}
+
 uint64_t plan_batch_items(const bbtc_plan* p) {
   return uint64_t(p->ctx->num_cu) * uint64_t(p->ctx->blocks_per_cu) * kWavesPerBlock * 4;
 }
@@ -663,14 +706,30 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
     g->rowptr = bbtc::dev_alloc<uint32_t>(ctx, L.rowptr.size(), &g->bytes);
     g->cols = bbtc::dev_alloc<uint32_t>(ctx, L.cols.size(), &g->bytes);
     g->nzrows = bbtc::dev_alloc<uint32_t>(ctx, L.nzrows.size(), &g->bytes);
-    g->dense = bbtc::dev_alloc<uint32_t>(ctx, L.dense.size(), &g->bytes);
+    g->dense = bbtc::dev_alloc<uint32_t>(ctx, L.dense_words, &g->bytes);
     g->tiles = bbtc::dev_alloc<bbtc::TileDesc>(ctx, T, &g->bytes);
     HIP_OK(hipMemcpyAsync(g->rowptr, L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(g->cols, L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(g->nzrows, L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipMemcpyAsync(g->dense, L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(g->tiles, g->htiles.data(), T * sizeof(bbtc::TileDesc), hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipStreamSynchronize(ctx->stream));
+    // bitmap rows of the dense tiles, built from the uploaded tiles
+    uint64_t rows = 0, acct = 0;
+    const std::vector<bbtc::DenseJob> jobs = bbtc::dense_jobs(L, 0, 0, 0, 0, &rows);
+    bbtc::DenseJob* jd = jobs.empty() ? nullptr : bbtc::dev_alloc<bbtc::DenseJob>(ctx, jobs.size(), &acct);
+    try {
+      if (jd)
+        HIP_OK(hipMemcpyAsync(jd, jobs.data(), jobs.size() * sizeof(bbtc::DenseJob), hipMemcpyHostToDevice,
+                              ctx->stream));
+      bbtc::enqueue_dense_build(g->rowptr, g->cols, g->dense, L.dense_words, jd, jobs.size(), rows, ctx->stream);
+      HIP_OK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
+      bbtc::dev_free(ctx, jd);
+      ctx->in_use -= std::min(ctx->in_use, acct);
+      throw;
+    }
+    bbtc::dev_free(ctx, jd);
+    ctx->in_use -= std::min(ctx->in_use, acct);
   } catch (...) {
     bbtc::release_graph(g);
     throw;
@@ -1034,7 +1093,7 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
       n_rp += L.rowptr.size();
       n_col += L.cols.size();
       n_nz += L.nzrows.size();
-      n_dense += L.dense.size();
+      n_dense += L.dense_words;
       n_tiles += uint64_t(L.P) * L.P;
     }
     if (n_tiles >= 0xFFFFFFFFull) bbtc::fail(BBTC_ERR_INVALID, "too many tiles in one batch");
@@ -1056,6 +1115,8 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
     };
     std::vector<Order> order;
     std::vector<bbtc::TaskWork> work;  // per task, in the order of b->refs before sorting
+    std::vector<bbtc::DenseJob> jobs;  // bitmap rows of every graph's dense tiles
+    uint64_t job_rows = 0;
     for (uint64_t gi = 0; gi < ngraphs; ++gi) {
       const bbtc::Layout& L = layouts[gi]->L;
       const uint32_t P = L.P;
@@ -1081,7 +1142,10 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
       HIP_OK(hipMemcpyAsync(A->rowptr + rp0[gi], L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(A->cols + col0[gi], L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(A->dense + dn0[gi], L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, s));
+      uint64_t rows = 0;
+      const std::vector<bbtc::DenseJob> gj = bbtc::dense_jobs(L, rp0[gi], col0[gi], dn0[gi], job_rows, &rows);
+      jobs.insert(jobs.end(), gj.begin(), gj.end());
+      job_rows += rows;
       // this graph's tasks, weighted as tasks.hpp:21-23 for the cross-graph order
       const size_t first = b->refs.size();
       for (uint32_t i = 0; i < P; ++i)
@@ -1102,6 +1166,25 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
     }
     HIP_OK(hipMemcpyAsync(A->tiles, A->htiles.data(), A->htiles.size() * sizeof(bbtc::TileDesc),
                           hipMemcpyHostToDevice, ctx->stream));
+    {
+      uint64_t acct = 0;
+      bbtc::DenseJob* jd = jobs.empty() ? nullptr : bbtc::dev_alloc<bbtc::DenseJob>(ctx, jobs.size(), &acct);
+      try {
+        if (jd)
+          HIP_OK(hipMemcpyAsync(jd, jobs.data(), jobs.size() * sizeof(bbtc::DenseJob), hipMemcpyHostToDevice,
+                                ctx->stream));
+        bbtc::enqueue_dense_build(A->rowptr, A->cols, A->dense, std::max<uint64_t>(n_dense, 1), jd, jobs.size(),
+                                  job_rows, ctx->stream);
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+      } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        bbtc::dev_free(ctx, jd);
+        ctx->in_use -= std::min(ctx->in_use, acct);
+        throw;
+      }
+      bbtc::dev_free(ctx, jd);
+      ctx->in_use -= std::min(ctx->in_use, acct);
+    }
     HIP_OK(hipStreamSynchronize(ctx->stream));
     // heavy tasks first (NaN weights of empty bands last), over all graphs or
     // within each graph with the smallest graphs first

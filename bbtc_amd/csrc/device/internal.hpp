@@ -88,4 +88,13 @@ void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t i
 // Enqueues p[0..n) = 0 on s.
 void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s);
 
+// Bitmap-row jobs of a layout's dense tiles, with its arrays placed at rp0,
+// col0 and dense0 of the device arrays; rows_out: the bitmap rows in all.
+std::vector<DenseJob> dense_jobs(const Layout& L, uint64_t rp0, uint64_t col0, uint64_t dense0,
+                                 uint64_t row0, uint64_t* rows_out);
+// Enqueues on s: dense[0..dense_words) = 0, then the bitmap rows of the jobs
+// (jobs_dev holds njobs jobs with rows rows in all) from the uploaded tiles.
+void enqueue_dense_build(const uint32_t* rowptr, const uint32_t* cols, uint32_t* dense, uint64_t dense_words,
+                         const DenseJob* jobs_dev, uint64_t njobs, uint64_t rows, hipStream_t s);
+
 }  // namespace bbtc

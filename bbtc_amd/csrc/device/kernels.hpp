@@ -629,6 +629,60 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
   count_units<false>(a, nullptr);
 }
 
+// Zeroes n words (grid-stride; a fill kernel rather than a memset).
+__global__ void fill_zero_u32(uint32_t* __restrict__ p, uint64_t n) {
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+// Bitmap rows of the tiles listed in jobs (all-zero beforehand): one wave per
+// 64 consecutive bitmap rows of the concatenated jobs, a lane per row; the
+// rows' entries are flattened over the lanes (a wave prefix sum), so each
+// lane reads one column id and sets its bit (entries of one row may share a
+// word, hence the atomic OR).
+__global__ __launch_bounds__(kBlock) void build_bitmaps(const uint32_t* __restrict__ rowptr,
+                                                        const uint32_t* __restrict__ cols,
+                                                        uint32_t* __restrict__ dense,
+                                                        const DenseJob* __restrict__ jobs, uint32_t njobs,
+                                                        uint64_t total_rows) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t r0 = wave * kWave; r0 < total_rows; r0 += nwaves * kWave) {
+    const uint64_t row = r0 + uint64_t(lane);
+    uint32_t d = 0;
+    uint64_t cb = 0, db = 0;  // first column entry, first bitmap word of the row
+    if (row < total_rows) {
+      uint32_t lo = 0, hi = njobs;  // last job with row0 <= row
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].row0 <= row) lo = mid;
+        else hi = mid;
+      }
+      const DenseJob J = gload(jobs + lo);
+      const uint64_t r = row - J.row0;
+      const uint32_t b = rowptr[J.rp + r], e = rowptr[J.rp + r + 1];
+      d = e - b;
+      cb = J.col + b;
+      db = J.dense + r * J.dwpr;
+    }
+    const uint32_t incl = wave_incl_scan(d, lane);
+    const uint32_t excl = incl - d;
+    const uint32_t D = __shfl(incl, kWave - 1, kWave);
+    for (uint32_t x = 0; x < D; x += kWave) {
+      const uint32_t e = x + uint32_t(lane);
+      const int q = owner_of(incl, e);
+      const uint64_t qcb = __shfl(cb, q, kWave);
+      const uint64_t qdb = __shfl(db, q, kWave);
+      const uint32_t qex = __shfl(excl, q, kWave);
+      if (e < D) {
+        const uint32_t c = cols[qcb + (e - qex)];
+        atomicOr(&dense[qdb + (c >> 5)], 1u << (c & 31));
+      }
+    }
+  }
+}
+
 // Start of a batch launch: stamp the device wall clock and reset each
 // graph's outstanding item count (graphs without items finish at once).
 __global__ void batch_begin(const unsigned long long* __restrict__ left0, uint64_t ngraphs,

@@ -119,7 +119,9 @@ struct Request {
   uint64_t* per_task_out = nullptr;
   double arrival = 0;
   // placement of the arrays inside the request's heap block
-  uint64_t o_rp = 0, o_col = 0, o_nz = 0, o_dense = 0, o_tiles = 0, o_tasks = 0, o_pref = 0, o_cnt = 0;
+  uint64_t o_rp = 0, o_col = 0, o_nz = 0, o_dense = 0, o_tiles = 0, o_tasks = 0, o_pref = 0, o_cnt = 0, o_jobs = 0;
+  std::vector<DenseJob> jobs;  // bitmap rows built on the device after the upload
+  uint64_t job_rows = 0;
   uint64_t bytes = 0;
   uint64_t heap_off = ~0ull;
   hipEvent_t uploaded = nullptr;
@@ -237,11 +239,12 @@ uint64_t place(Request& r) {
   put(r.o_rp, L.rowptr.size() * 4);
   put(r.o_col, L.cols.size() * 4);
   put(r.o_nz, L.nzrows.size() * 4);
-  put(r.o_dense, L.dense.size() * 4);
+  put(r.o_dense, L.dense_words * 4);
   put(r.o_tiles, r.tiles.size() * sizeof(TileDesc));
   put(r.o_tasks, r.plan.td.size() * sizeof(TaskDesc));
   put(r.o_pref, r.plan.pref.size() * 8);
   put(r.o_cnt, std::max<uint64_t>(r.ntasks, 1) * 8);
+  put(r.o_jobs, r.jobs.size() * sizeof(DenseJob));
   return o;
 }
 
@@ -319,7 +322,7 @@ void admit_loop(bbtc_server* S) {
       HIP_OK(hipMemcpyAsync(b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(b + r->o_col, L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(b + r->o_nz, L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(b + r->o_dense, L.dense.data(), L.dense.size() * 4, hipMemcpyHostToDevice, s));
+
       HIP_OK(hipMemcpyAsync(b + r->o_tiles, r->tiles.data(), r->tiles.size() * sizeof(TileDesc),
                             hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(b + r->o_tasks, r->plan.td.data(), r->plan.td.size() * sizeof(TaskDesc),
@@ -327,6 +330,12 @@ void admit_loop(bbtc_server* S) {
       HIP_OK(hipMemcpyAsync(b + r->o_pref, r->plan.pref.data(), r->plan.pref.size() * 8, hipMemcpyHostToDevice, s));
       auto* cnt = reinterpret_cast<unsigned long long*>(b + r->o_cnt);
       enqueue_zero(cnt, r->ntasks, s);
+      // the bitmap rows of the dense tiles, built from the uploaded tiles
+      auto* jd = reinterpret_cast<DenseJob*>(b + r->o_jobs);
+      if (!r->jobs.empty())
+        HIP_OK(hipMemcpyAsync(jd, r->jobs.data(), r->jobs.size() * sizeof(DenseJob), hipMemcpyHostToDevice, s));
+      enqueue_dense_build(reinterpret_cast<const uint32_t*>(b + r->o_rp), reinterpret_cast<const uint32_t*>(b + r->o_col),
+                          reinterpret_cast<uint32_t*>(b + r->o_dense), L.dense_words, jd, r->jobs.size(), r->job_rows, s);
       SegDesc& g = r->seg;
       g.rowptr = reinterpret_cast<const uint32_t*>(b + r->o_rp);
       g.cols = reinterpret_cast<const uint32_t*>(b + r->o_col);
@@ -751,6 +760,7 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
   }
   r->per_task_out = per_task_out;
   r->arrival = arrival_s;
+  r->jobs = bbtc::dense_jobs(L->L, 0, 0, 0, 0, &r->job_rows);
   r->bytes = bbtc::place(*r);
   {
     std::lock_guard<std::mutex> lk(S->mu);

@@ -24,6 +24,9 @@ void plan_slot_wait(bbtc_plan* p, int slot);
 // Waits for all slots, copies all per-task counters to host.
 void plan_copy_counts(bbtc_plan* p, uint64_t* out);
 int max_slots();
+// Estimated work of each task (algorithmic bytes with 4-byte offsets) from the
+// work statistics the graph gathered at upload; false when it has none.
+bool graph_task_work(const bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n, double* out);
 
 }  // namespace dev
 }  // namespace bbtc

@@ -47,7 +47,9 @@ std::vector<uint32_t> partition(const Csr& g, bbtc_part kind, uint32_t p);
 //
 // Tiles whose column band is narrow also keep a bitmap form: row r of tile
 // t is dense[dense_base[t] + r * dwpr[k] .. + dwpr[k]), one bit per column of
-// band k, dwpr[k] = ceil(width/32) rounded up to 4 words (16-byte rows).
+// band k, dwpr[k] = ceil(width/32) rounded up to 4 words (16-byte rows). The
+// host layout holds only the placement (dense_words in all); the rows are
+// built on the device from the uploaded tiles.
 constexpr uint32_t kDenseMaxBits = 16384;            // widest band kept as bitmaps
 constexpr uint64_t kDenseTileWords = 64ull << 20;    // largest bitmap tile (256 MiB)
 constexpr uint64_t kDenseTotalWords = 4096ull << 20; // all bitmap tiles (16 GiB)
@@ -70,15 +72,15 @@ struct Layout {
   std::vector<float> deg;          // P*P, (float)nnz / (float)h
   std::vector<uint64_t> dense_base; // P*P, ~0 when the tile has no bitmap form
   std::vector<uint32_t> dwpr;       // P, bitmap words per row of band k
+  uint64_t dense_words = 4;         // words of all bitmap rows (device only)
   Buf<uint32_t> rowptr;
   Buf<uint32_t> cols;
   Buf<uint32_t> nzrows;
-  Buf<uint32_t> dense;
 
   uint32_t height(uint32_t b) const { return cuts[b + 1] - cuts[b]; }
   size_t tile(uint32_t i, uint32_t k) const { return size_t(i) * P + k; }
   uint64_t device_bytes() const {
-    return rowptr.size() * 4 + cols.size() * 4 + nzrows.size() * 4 + dense.size() * 4 +
+    return rowptr.size() * 4 + cols.size() * 4 + nzrows.size() * 4 + dense_words * 4 +
            cuts.size() * 4 + 4 * rp_base.size() * 8 + nnz.size() * 8;
   }
 };

@@ -22,7 +22,9 @@ inline uint64_t align_up(uint64_t x) { return (x + kAlignEntries - 1) & ~(kAlign
 }  // namespace
 
 // Bitmap form of the tiles whose column band is at most kDenseMaxBits wide,
-// smallest tiles first until the total cap.
+// smallest tiles first until the total cap: only the placement; the rows are
+// filled on the device from the uploaded tiles (bbtc_graph_upload), so they
+// never cross PCIe.
 static void build_dense(Layout& L) {
   const uint32_t P = L.P;
   L.dense_base.assign(size_t(P) * P, ~0ull);
@@ -38,31 +40,12 @@ static void build_dense(Layout& L) {
     }
   std::sort(cand.begin(), cand.end());
   uint64_t total = 0;
-  std::vector<size_t> chosen;
   for (auto& c : cand) {
     if (total + c.first > kDenseTotalWords) break;
     L.dense_base[c.second] = total;
     total += c.first;
-    chosen.push_back(c.second);
   }
-  L.dense.resize(std::max<uint64_t>(total, 4));
-  uint32_t* d = L.dense.data();
-#pragma omp parallel for schedule(static, 1 << 16)
-  for (int64_t x = 0; x < int64_t(L.dense.size()); ++x) d[x] = 0;
-  const uint32_t* rp = L.rowptr.data();
-  const uint32_t* cl = L.cols.data();
-  for (size_t t : chosen) {
-    const uint32_t i = uint32_t(t / P), k = uint32_t(t % P);
-    const uint32_t h = L.height(i), w = L.dwpr[k];
-    const uint32_t* row = rp + L.rp_base[t];
-    const uint32_t* c = cl + L.col_base[t];
-    uint32_t* base = d + L.dense_base[t];
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int64_t r = 0; r < int64_t(h); ++r) {
-      uint32_t* dst = base + uint64_t(r) * w;
-      for (uint32_t x = row[r]; x < row[r + 1]; ++x) dst[c[x] >> 5] |= 1u << (c[x] & 31);
-    }
-  }
+  L.dense_words = std::max<uint64_t>(total, 4);
 }
 
 Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
