@@ -173,8 +173,15 @@ struct SegDesc {
   uint64_t launch0;             // launch-local index of item0
   uint32_t ntasks;
   uint32_t pad;
+  // Row slots of the lane form (NULL: none): for row v of tile t, 16 bytes at
+  // slots[4 * (rp(t) + v)]: the row's entry count, then its entries when it
+  // has at most kSlotEntries, else its first entry's offset in the tile.
+  const uint32_t* slots;
 };
-static_assert(sizeof(SegDesc) == 104, "SegDesc layout");
+static_assert(sizeof(SegDesc) == 112, "SegDesc layout");
+constexpr uint32_t kSlotEntries = 3;
+// Row slots are built for a graph's lane-form plans only up to this size.
+constexpr uint64_t kSlotMaxBytes = 8ull << 30;
 
 struct CountArgs {
   const SegDesc* __restrict__ segs;  // launch0 increasing

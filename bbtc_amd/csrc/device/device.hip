@@ -269,6 +269,7 @@ void release_graph(bbtc_dgraph* g) {
   dev_free(g->ctx, g->nzrows);
   dev_free(g->ctx, g->dense);
   dev_free(g->ctx, g->tiles);
+  dev_free(g->ctx, g->slots);
   if (g->ctx) g->ctx->in_use -= std::min(g->ctx->in_use, g->bytes);
   delete g;
 }
@@ -328,6 +329,49 @@ void launch(bbtc_plan* p, hipStream_t s) {
 namespace bbtc {
 namespace {
 
+// Row slots of every non-empty upper tile of a single resident graph (the
+// lane form reads a short bc(v) list from them in one 16-byte load), built
+// from the resident tiles; none when they would take more than kSlotMaxBytes.
+void build_graph_slots(bbtc_ctx* ctx, bbtc_dgraph* g) {
+  const uint64_t bytes = g->rowptr_words * 16;
+  if (bytes > kSlotMaxBytes || (ctx->budget && ctx->in_use + bytes > ctx->budget)) return;
+  std::vector<DenseJob> jobs;
+  uint64_t rows = 0;
+  for (uint32_t i = 0; i < g->P; ++i)
+    for (uint32_t k = i; k < g->P; ++k) {
+      const TileDesc& d = g->htiles[size_t(i) * g->P + k];
+      if (d.nzr == 0) continue;
+      if (d.rp + d.h >= g->rowptr_words) fail(BBTC_ERR_INTERNAL, "tile rows outside the row offsets");
+      jobs.push_back(DenseJob{d.rp, d.col, 0, rows, d.h, 0});
+      rows += d.h;
+    }
+  if (jobs.empty()) return;
+  if (jobs.size() >= 0xFFFFFFFFull) return;
+  uint64_t acct = 0;
+  uint32_t* slots = reinterpret_cast<uint32_t*>(dev_alloc<u32x4>(ctx, g->rowptr_words, &acct));
+  DenseJob* jd = nullptr;
+  try {
+    jd = dev_alloc<DenseJob>(ctx, jobs.size(), &acct);
+    HIP_OK(hipMemcpyAsync(jd, jobs.data(), jobs.size() * sizeof(DenseJob), hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t blocks = std::min<uint64_t>((rows + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(build_slots, dim3(unsigned(std::max<uint64_t>(blocks, 1))), dim3(kBlock), 0, ctx->stream,
+                       g->rowptr, g->cols, reinterpret_cast<u32x4*>(slots), jd, uint32_t(jobs.size()), rows);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, jd);
+    dev_free(ctx, slots);
+    ctx->in_use -= std::min(ctx->in_use, acct);
+    throw;
+  }
+  dev_free(ctx, jd);
+  const uint64_t job_bytes = std::max<uint64_t>(jobs.size(), 1) * sizeof(DenseJob);
+  ctx->in_use -= std::min(ctx->in_use, job_bytes);
+  g->slots = slots;
+  g->bytes += acct - job_bytes;
+}
+
 // Work statistics of the tasks of a single resident graph, from the table
 // gathered at upload (empty when it has none).
 std::vector<TaskWork> graph_work(const bbtc_dgraph* g, const std::vector<TaskRef>& refs) {
@@ -359,6 +403,7 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
       if (!own.empty()) work = &own;
     }
     PlanDesc D = describe_plan(g->htiles, g->deg, refs, kernel, choice, true, work);
+    if (D.lane_only && g->P > 0 && !g->slots) build_graph_slots(ctx, g);
     std::vector<TaskDesc>& td = D.td;
     std::vector<uint64_t>& pref = D.pref;
     p->hkind = std::move(D.hkind);
@@ -387,6 +432,7 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
     sd[0].item_prefix = p->item_prefix;
     sd[0].per_task = p->per_task;
     sd[0].cycles = nullptr;
+    sd[0].slots = g->slots;
     sd[0].item0 = 0;
     sd[0].item1 = p->nitems;
     sd[0].launch0 = 0;
@@ -704,6 +750,7 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
       g->lex_work = bbtc::task_work(L, all.data(), all.size());
     }
     g->rowptr = bbtc::dev_alloc<uint32_t>(ctx, L.rowptr.size(), &g->bytes);
+    g->rowptr_words = L.rowptr.size();
     g->cols = bbtc::dev_alloc<uint32_t>(ctx, L.cols.size(), &g->bytes);
     g->nzrows = bbtc::dev_alloc<uint32_t>(ctx, L.nzrows.size(), &g->bytes);
     g->dense = bbtc::dev_alloc<uint32_t>(ctx, L.dense_words, &g->bytes);
@@ -938,6 +985,7 @@ bbtc_status bbtc_count_noalloc(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* t
   sd.item_prefix = pref;
   sd.per_task = counts;
   sd.cycles = nullptr;
+  sd.slots = g->slots;  // built by an earlier plan, if any (never here)
   sd.item0 = 0;
   sd.item1 = nitems;
   sd.launch0 = 0;

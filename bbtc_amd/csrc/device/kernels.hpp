@@ -31,7 +31,9 @@
 //   lane:   one lane per wedge intersects bc(v) and ac(u) in HBM, four
 //           entries of each at a time (rows of at most 32 entries; no LDS,
 //           no second phase; a launch of lane-form tasks only runs the
-//           count_probe_lanes instantiation at 8 waves per SIMD).
+//           count_probe_lanes instantiation at 8 waves per SIMD, which reads
+//           a bc(v) of up to 3 entries from its 16-byte row slot: the count
+//           and the entries in one load).
 // Matches are counted with ballot + popcount (or a wave sum) into a
 // wave-uniform 64-bit count and added to the task's slot with one atomic per
 // task change.
@@ -382,11 +384,32 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
         const uint32_t q_dac = __shfl(dac, q, kWave);
         if (w < W) {
           const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
-          const uint32_t b0 = bc_rp[v], b1 = bc_rp[v + 1];
+          uint32_t b0, b1, y[4];
+          bool in_slot = false;
+          if (!kAll && S.slots) {
+            // one 16-byte read: the count and the entries of a short bc(v)
+            const u32x4 sl = ((gptr<u32x4>)S.slots)[BC.rp + v];
+            if (sl.x <= kSlotEntries) {
+              in_slot = true;
+              b0 = 0;
+              b1 = sl.x;
+              y[0] = sl.x > 0 ? sl.y : 0xFFFFFFFEu;
+              y[1] = sl.x > 1 ? sl.z : 0xFFFFFFFEu;
+              y[2] = sl.x > 2 ? sl.w : 0xFFFFFFFEu;
+              y[3] = 0xFFFFFFFEu;
+            } else {
+              b0 = sl.y;
+              b1 = sl.y + sl.x;
+            }
+          } else {
+            b0 = bc_rp[v];
+            b1 = bc_rp[v + 1];
+          }
           if (b0 < b1) {
-            uint32_t x[4], y[4];
+            uint32_t x[4];
             load4(ac_c, q_ac0, q_ac0 + q_dac, 0xFFFFFFFFu, x);
-            load4(bc_c, b0, b1, 0xFFFFFFFEu, y);
+            if (!in_slot) load4(bc_c, b0, b1, 0xFFFFFFFEu, y);
+            // a list held in the slot is one block: it never loads from bc_c
             c += intersect_from(ac_c, q_ac0, q_ac0 + q_dac, bc_c, b0, b1, x, y);
           }
         }
@@ -680,6 +703,35 @@ __global__ __launch_bounds__(kBlock) void build_bitmaps(const uint32_t* __restri
         atomicOr(&dense[qdb + (c >> 5)], 1u << (c & 31));
       }
     }
+  }
+}
+
+// Row slots (see SegDesc::slots) of the tiles listed in jobs (the dense and
+// dwpr fields unused): one thread per row of the concatenated jobs.
+__global__ __launch_bounds__(kBlock) void build_slots(const uint32_t* __restrict__ rowptr,
+                                                      const uint32_t* __restrict__ cols, u32x4* __restrict__ slots,
+                                                      const DenseJob* __restrict__ jobs, uint32_t njobs,
+                                                      uint64_t total_rows) {
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t row = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; row < total_rows; row += stride) {
+    uint32_t lo = 0, hi = njobs;  // last job with row0 <= row
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (jobs[mid].row0 <= row) lo = mid;
+      else hi = mid;
+    }
+    const DenseJob J = gload(jobs + lo);
+    const uint64_t r = row - J.row0;
+    const uint32_t b = rowptr[J.rp + r], e = rowptr[J.rp + r + 1];
+    const uint32_t cnt = e - b;
+    u32x4 sl = {cnt, b, 0u, 0u};
+    if (cnt <= kSlotEntries) {
+      const uint32_t* c = cols + J.col + b;
+      sl.y = cnt > 0 ? c[0] : 0u;
+      sl.z = cnt > 1 ? c[1] : 0u;
+      sl.w = cnt > 2 ? c[2] : 0u;
+    }
+    slots[J.rp + r] = sl;
   }
 }
 

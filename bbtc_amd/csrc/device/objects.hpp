@@ -50,6 +50,8 @@ struct bbtc_dgraph {
   // is lex_work[lex_ij[i * P + j] + k - j].
   std::vector<bbtc::TaskWork> lex_work;
   std::vector<uint64_t> lex_ij;
+  uint64_t rowptr_words = 0;
+  uint32_t* slots = nullptr;  // row slots of the lane form (4 words per rowptr word), built on first need
 };
 
 struct bbtc_batch;

@@ -1,8 +1,8 @@
 """Register budget of the counting kernels: count_probe must keep 4 waves per
 SIMD (at most 128 VGPRs: 1,024 resident blocks of 256 threads) and its
 lane-form instantiation 8, since both are latency-bound and their speed
-falls with occupancy. Read from the compiler's resource remarks for gfx950,
-with the library's own flags; no GPU needed."""
+falls with occupancy; neither may spill. Read from the compiler's resource
+remarks for gfx950, with the library's own flags; no GPU needed."""
 import os
 import re
 import shutil
