@@ -435,7 +435,9 @@ class DeviceGraph:
 
     def autotune(self, tasks):
         """Per-task form choice from measured clocks: (choice[n], cycles[n,5]),
-        forms in the order merge, bitmap, dense, rdense, lane."""
+        forms in the order merge, bitmap, dense, rdense, lane. choice & 7 is
+        the form (1-5), choice >> 4 the times the rows per work item were
+        halved (0-2)."""
         t = tasks_array(tasks)
         choice = np.zeros(len(t), dtype=np.uint8)
         cyc = np.zeros((len(t), 5), dtype=np.uint64)

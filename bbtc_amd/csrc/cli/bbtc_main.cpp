@@ -266,9 +266,10 @@ GraphPrint graph_print(const bbtc_hgraph* g) {
 }
 
 // Forms file: "bbtc-forms 2", then "<ncut> <ntasks> <n> <m> <hash>" (the
-// graph's fingerprint), the ncut+1 cuts, and one form per task in task order
-// (1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane, 0 the default rule), all as
-// text. The fingerprint and the cuts tie it to one graph and layout.
+// graph's fingerprint), the ncut+1 cuts, and one choice per task in task
+// order (form 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane, 0 the default
+// rule, plus 16 times the rows' halvings), all as text. The fingerprint and
+// the cuts tie it to one graph and layout.
 void write_forms(const std::string& path, const bbtc_hgraph* g, const bbtc_layout* L,
                  const std::vector<uint8_t>& choice, uint64_t ntasks) {
   const uint32_t p = bbtc_layout_ncut(L);
@@ -309,7 +310,8 @@ std::vector<uint8_t> read_forms(const std::string& path, const bbtc_hgraph* g, c
   std::vector<uint8_t> choice(std::max<uint64_t>(ntasks, 1), 0);
   for (uint64_t q = 0; q < ntasks; ++q) {
     int c = -1;
-    if (!(f >> c) || c < 0 || c > 5) fail("bad or missing form for task " + std::to_string(q));
+    if (!(f >> c) || c < 0 || (c & 7) > 5 || (c >> 4) > 2 || (c & 8))
+      fail("bad or missing form for task " + std::to_string(q));
     choice[q] = uint8_t(c);
   }
   return choice;

@@ -58,6 +58,12 @@ constexpr uint32_t kKindBitmap = 1;
 constexpr uint32_t kKindDense = 2;
 constexpr uint32_t kKindRdense = 3;
 constexpr uint32_t kKindLane = 4;
+// A per-task choice byte (bbtc_plan_create_tuned): the form in the low bits
+// (0 the kernel's rule, 1-5 a form), and in bits 4-5 the rows per work item
+// as a right shift of the form's default (64, or as many bitmaps as fit).
+constexpr uint32_t kChoiceFormMask = 7;
+constexpr uint32_t kChoiceRowsShift = 4;
+constexpr uint32_t kRowsShifts = 3;  // the autotuner tries 64, 32 and 16 rows
 // The lane form (one lane per row u, lists merged in HBM) applies to tasks
 // whose three tiles hold at most this many entries in any row, which bounds
 // a lane's work to kLaneMaxRow * 2 * kLaneMaxRow merge steps.

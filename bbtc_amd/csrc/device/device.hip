@@ -205,12 +205,14 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
     const uint32_t fit = wpr ? uint32_t(kLdsWords) / wpr : uint32_t(kWave);
     const bool has_dense = tiles[x.bc].dense != ~0ull;
     bbtc_kernel k = kernel;
-    if (choice && choice[q] == 1) k = BBTC_KERNEL_MERGE;
-    else if (choice && choice[q] == 2) k = BBTC_KERNEL_BITMAP;
-    else if (choice && choice[q] == 3) k = BBTC_KERNEL_DENSE;
-    else if (choice && choice[q] == 4) k = BBTC_KERNEL_RDENSE;
-    else if (choice && choice[q] == 5) k = BBTC_KERNEL_LANE;
-    else if (auto_lanes && !(choice && choice[q])) k = BBTC_KERNEL_LANE;
+    const uint32_t form = choice ? choice[q] & kChoiceFormMask : 0u;
+    const uint32_t rows_shift = choice ? (choice[q] >> kChoiceRowsShift) & 3u : 0u;
+    if (form == 1) k = BBTC_KERNEL_MERGE;
+    else if (form == 2) k = BBTC_KERNEL_BITMAP;
+    else if (form == 3) k = BBTC_KERNEL_DENSE;
+    else if (form == 4) k = BBTC_KERNEL_RDENSE;
+    else if (form == 5) k = BBTC_KERNEL_LANE;
+    else if (auto_lanes && !form) k = BBTC_KERNEL_LANE;
     uint32_t kind = kKindSearch;
     if (k == BBTC_KERNEL_BITMAP) {
       if (fit >= 1) kind = kKindBitmap;
@@ -247,6 +249,7 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
     d.kind = kind;
     P.hkind.push_back(kind + 1);
     d.rows = use_bitmap ? std::min<uint32_t>(fit, kWave) : uint32_t(kWave);
+    d.rows = std::max<uint32_t>(1, d.rows >> rows_shift);  // the autotuner's rows per item
     d.wpr = kind == kKindSearch || kind == kKindLane ? 0 : wpr;
     P.td[q] = d;
     uint64_t items = 0;
@@ -521,6 +524,28 @@ void autotune_refs(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& re
     const float per_task = fastest_ms(BBTC_KERNEL_AUTO, choice.data());
     const float lanes = fastest_ms(BBTC_KERNEL_LANE, nullptr);
     if (lanes < per_task) std::fill(choice.begin(), choice.end(), uint8_t(5));
+  }
+  // Rows per item: the chosen forms counted with 64 (or as many as fit the
+  // LDS slice), half and a quarter of that many rows per work item, all tasks
+  // sharing the device in each pass; each task keeps its cheapest.
+  if (n > 0) {
+    std::vector<uint64_t> best_rows(n, none);
+    std::vector<uint8_t> shift(n, 0);
+    for (uint32_t sh = 0; sh < kRowsShifts; ++sh) {
+      std::vector<uint8_t> ch(n);
+      for (uint64_t q = 0; q < n; ++q) ch[q] = uint8_t(choice[q] | (sh << kChoiceRowsShift));
+      bbtc_plan* p = make_plan(ctx, g, refs, BBTC_KERNEL_AUTO, ch.data());
+      std::vector<uint64_t> cyc(n);
+      const bbtc_status st = bbtc_plan_profile(p, cyc.data());
+      release_plan(p);
+      if (st != BBTC_OK) fail(st, bbtc_last_error());
+      for (uint64_t q = 0; q < n; ++q)
+        if (cyc[q] < best_rows[q]) {
+          best_rows[q] = cyc[q];
+          shift[q] = uint8_t(sh);
+        }
+    }
+    for (uint64_t q = 0; q < n; ++q) choice[q] = uint8_t(choice[q] | (shift[q] << kChoiceRowsShift));
   }
   std::copy(choice.begin(), choice.end(), choice_out);
 }

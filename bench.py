@@ -260,7 +260,8 @@ def main():
         tune_s = time.time() - t0
         plan = dg.plan(tasks, "auto", choice)
         log(f"autotune: {tune_s:.1f}s, forms merge/bitmap/dense/rdense/lane = "
-            f"{np.bincount(choice, minlength=6)[1:].tolist()}")
+            f"{np.bincount(choice & 7, minlength=6)[1:].tolist()}, rows halved 0/1/2 times = "
+            f"{np.bincount(choice >> 4, minlength=3).tolist()}")
     else:
         plan = dg.plan(tasks, args.kernel)
     if args.warmup:

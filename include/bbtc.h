@@ -284,9 +284,10 @@ void bbtc_graph_release(bbtc_dgraph* g);
  * NoOfTrianglesInGPU (bb_tc.cpp:76-115, 142-181). */
 bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                              bbtc_kernel kernel, bbtc_plan** out);
-/* As bbtc_plan_create, with a per-task override choice[q]: 0 keeps `kernel`'s
- * rule, 1 forces MERGE, 2 BITMAP, 3 DENSE, 4 RDENSE, 5 LANE (each where it
- * applies).
+/* As bbtc_plan_create, with a per-task override choice[q]: in bits 0-2, 0
+ * keeps `kernel`'s rule, 1 forces MERGE, 2 BITMAP, 3 DENSE, 4 RDENSE, 5 LANE
+ * (each where it applies); bits 4-5 halve the rows of a work item that many
+ * times (0: 64 rows, or as many as the form's LDS bitmaps fit).
  * Used with bbtc_autotune's output; choice may be NULL. */
 bbtc_status bbtc_plan_create_tuned(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks,
                                    uint64_t n, bbtc_kernel kernel, const uint8_t* choice,
@@ -310,13 +311,16 @@ bbtc_status bbtc_plan_time_tasks(bbtc_plan* plan, float* ms_per_task);
 bbtc_status bbtc_plan_profile(bbtc_plan* plan, uint64_t* cycles_per_task);
 /* Autotuner: measures every form on the whole task list (each forced where it
  * applies, per-item clock counters) and writes per task the cheapest of the
- * forms 1 merge, 2 bitmap, 3 dense, 4 rdense into choice_out[n], ready for
- * bbtc_plan_create_tuned. The lane form (5) runs in a kernel of its own only
- * when every task of a launch uses it, so where it applies to every task with
- * work, that per-task choice is timed against all tasks in the lane form and
- * the faster one is written. cycles_out (may be NULL) receives n x 5 clock
- * counts (forms 1-5 in order), UINT64_MAX where a form does not apply.
- * Replaces the fixed threshold of bb_tc.cpp:98. */
+ * forms 1 merge, 2 bitmap, 3 dense, 4 rdense into bits 0-2 of choice_out[n],
+ * ready for bbtc_plan_create_tuned. The lane form (5) runs in a kernel of its
+ * own only when every task of a launch uses it, so where it applies to every
+ * task with work, that per-task choice is timed against all tasks in the lane
+ * form and the faster one is written. Then the chosen forms are measured with
+ * 64 (or as many as fit), 32 and 16 rows per work item, and each task's
+ * cheapest goes into bits 4-5 (the rows' halvings). cycles_out (may be NULL)
+ * receives n x 5 clock counts (forms 1-5 in order, at 64 rows), UINT64_MAX
+ * where a form does not apply. Replaces the fixed threshold of bb_tc.cpp:98
+ * and the fixed 256-thread row blocks of bb_tc.cpp:96-105. */
 bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                           uint8_t* choice_out, uint64_t* cycles_out);
 /* Form the plan chose per task: 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane. */

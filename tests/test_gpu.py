@@ -48,6 +48,12 @@ def all_forms(dev, L, tasks):
     pl.launch()
     out["tuned"] = pl.results()
     pl.destroy()
+    # every form with its rows per work item halved twice (choice bits 4-5)
+    for form in (1, 2, 3, 4, 5):
+        pl = dg.plan(tasks, "auto", np.full(len(tasks), form | (2 << 4), dtype=np.uint8))
+        pl.launch()
+        out[f"form{form}-rows/4"] = pl.results()
+        pl.destroy()
     dg.release()
     return out
 
