@@ -57,7 +57,7 @@ def percentile(x, q):
 
 
 def run_batched(dev: Device, specs, log=print, sample_every: int = 64, cands=(1, 2, 4, 8),
-                order: str = "small_first"):
+                order: str = "small_first", tune: bool = True):
     """Counts every graph of `specs` with the plain per-graph launch and with
     one batched persistent launch; returns a dict of measurements and checks
     the per-graph counts agree bit for bit. Latencies are completion times
@@ -137,10 +137,12 @@ def run_batched(dev: Device, specs, log=print, sample_every: int = 64, cands=(1,
     b = Batch(dev, layouts, "auto", order=order)
     setup_s = time.time() - t0
     batched_s, counts, lat = timed_batch(b)
-    t0 = time.time()
-    b.autotune()
-    batch_tune_s = time.time() - t0
-    tuned_s, _, tuned_lat = timed_batch(b)
+    batch_tune_s, tuned_s, tuned_lat = 0.0, batched_s, lat
+    if tune:
+        t0 = time.time()
+        b.autotune()
+        batch_tune_s = time.time() - t0
+        tuned_s, _, tuned_lat = timed_batch(b)
     res = dict(
         graphs=len(graphs), edges=int(sum(ms)), triangles=int(counts.sum()), order=order,
         plain_s=plain_s, plain_graphs_per_s=len(graphs) / plain_s,

@@ -68,6 +68,23 @@ def measured_traffic(workload: str, forms: str):
     return int(t["traffic_bytes"]), os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
+CPU_FULL_FILE = os.path.join(ROOT, "profiles", "cpu_baseline_rmat-s26-ef16_pal32.json")
+
+
+def measured_cpu_full(m: int):
+    """The committed measurement of the oracle over every task of this graph
+    (m edges; bench.py --cpu-full), or None."""
+    try:
+        with open(CPU_FULL_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("m") != m:
+        return None
+    return {"seconds_per_count": t["seconds_per_count"], "value": m / t["seconds_per_count"],
+            "cores": t["cores"], "source": os.path.relpath(CPU_FULL_FILE, ROOT)}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -109,10 +126,12 @@ def host_info(nthreads):
 def cpu_baseline(g, cuts, tasks, per_task_gpu, m, budget_s, full=False):
     """Time the oracle (the reference CPU path restated) on the box's host
     cores. full: every task once, measured. Otherwise a systematic sample of
-    every task's rows, rows f with f % s == 0 of band i (1/s of each task's
-    rows, s sized from a 1/64 calibration run so the sample takes about
-    budget_s), its time scaled by s. Either way, the whole-task counts of
-    every 97th task (lexicographic) are checked bit-exact against the GPU."""
+    every task's rows, the 64-row blocks b with b % s == 0 of band i (1/s of
+    each task's rows, s sized from a 1/64 calibration run so the sample takes
+    about budget_s), its time scaled by s; the committed measurement over
+    every task (--cpu-full on a box like this one) is reported beside it.
+    Either way, the whole-task counts of every 97th task (lexicographic) are
+    checked bit-exact against the GPU."""
     import oracle as O
 
     t0 = time.time()
@@ -142,10 +161,10 @@ def cpu_baseline(g, cuts, tasks, per_task_gpu, m, budget_s, full=False):
         ref.count_rows(tt, stride, 0, nthreads)
         dt = time.time() - t0
         est = dt * stride
-        how = (f"oracle (C restatement of the reference CPU path) on every one of the {len(tasks)} tasks, rows "
-               f"f % {stride} == 0 of each (1/{stride} of the rows, a systematic sample): {dt:.1f}s, scaled by "
-               f"{stride} to {est:.1f}s per count; whole-task counts of every 97th task bit-exact vs GPU")
-    return {
+        how = (f"oracle (C restatement of the reference CPU path) on every one of the {len(tasks)} tasks, the "
+               f"64-row blocks b % {stride} == 0 of each (1/{stride} of the rows, a systematic sample): {dt:.1f}s, "
+               f"scaled by {stride} to {est:.1f}s per count; whole-task counts of every 97th task bit-exact vs GPU")
+    out = {
         "value": m / est,
         "unit": "edges/s",
         "cores": nthreads,
@@ -154,6 +173,10 @@ def cpu_baseline(g, cuts, tasks, per_task_gpu, m, budget_s, full=False):
         "seconds_per_count": round(est, 2),
         "sample": how,
     }
+    committed = None if full else measured_cpu_full(m)
+    if committed:
+        out["all_tasks_measured"] = committed
+    return out
 
 
 def count_c2(B, dev, steps, cpu=True):
@@ -209,6 +232,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched workload (--batched-graphs graphs); prints its object as the line")
     ap.add_argument("--batched-graphs", type=int, default=4096, help="0: skip the batched workload")
+    ap.add_argument("--no-batch-tune", action="store_true",
+                    help="batched workload with the AUTO forms only (no autotuned batch beside it)")
     ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
     args = ap.parse_args()
 
@@ -226,7 +251,8 @@ def main():
         return
     if args.batched_only:
         from bbtc_amd import workloads as W
-        print(json.dumps(W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log)), flush=True)
+        print(json.dumps(W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log,
+                                       tune=not args.no_batch_tune)), flush=True)
         return
     read_gbs = dev.read_bandwidth(8 << 30, 5)
     stream_gbs = float(np.median(read_gbs))
@@ -346,7 +372,7 @@ def main():
             out["c2"]["cpu_baseline"] = cpu2
 
     if args.batched_graphs:
-        res = W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log)
+        res = W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log, tune=not args.no_batch_tune)
         out["batched"] = rounded(dict(res, metric="graphs/s", unit="graphs/s",
                                       value=res["batched_graphs_per_s"], vs_plain=res["speedup"]))
     if args.serving_requests:

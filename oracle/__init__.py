@@ -138,8 +138,9 @@ class RefLayout:
         return per, tot
 
     def count_rows(self, tasks, stride, phase=0, nthreads=0):
-        """Counts of only the rows f % stride == phase of each task (a
-        systematic sample of every task's rows, for timing)."""
+        """Counts of only the 64-row blocks b % stride == phase of each
+        task's band rows (a systematic sample of every task's rows, for
+        timing)."""
         t = np.ascontiguousarray(np.asarray(tasks, dtype=np.uint32).reshape(-1, 3))
         per = np.zeros(len(t), dtype=np.uint64)
         tot = lib().orc_count_rows(self._h, _p(t), len(t), nthreads, _p(per), stride, phase)

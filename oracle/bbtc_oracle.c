@@ -642,9 +642,14 @@ static long list_intersect(const orc_bcsr* B, uint64_t ab_src, uint64_t bc_src, 
   return c;
 }
 
-/* Rows f of band i with f % stride == phase only (stride 1: the whole task,
- * as the reference counts it; larger strides time a systematic sample of
- * every task's rows). */
+/* Rows f of band i in blocks of ORC_ROW_BLOCK consecutive rows, the blocks
+ * b with b % stride == phase only (stride 1: the whole task, as the
+ * reference counts it; larger strides time a systematic sample of every
+ * task's rows that keeps the runs of consecutive rows a whole count reads). */
+#define ORC_ROW_BLOCK 64u
+#define FOR_SAMPLED_ROWS(f, h, stride, phase)                                              \
+  for (uint32_t blk_ = (phase); (uint64_t)blk_ * ORC_ROW_BLOCK < (h); blk_ += (stride))     \
+    for (uint32_t f = blk_ * ORC_ROW_BLOCK, fe_ = (f + ORC_ROW_BLOCK < (h) ? f + ORC_ROW_BLOCK : (h)); f < fe_; ++f)
 static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, uint64_t* hcap,
                        uint32_t stride, uint32_t phase) {
   const uint32_t P = B->P;
@@ -665,7 +670,7 @@ static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, ui
     }
     uint32_t* hm = *hmap;
     for (uint32_t x = 0; x < width; ++x) hm[x] = 0x7FFFFFFFu;
-    for (uint32_t f = phase; f < h; f += stride) {
+    FOR_SAMPLED_ROWS(f, h, stride, phase) {
       if (rows[ac_src + f + 1] == rows[ac_src + f]) continue;
       if (rows[ab_src + f + 1] - rows[ab_src + f] < 4) {
         c += list_intersect(B, ab_src, bc_src, ac_src, f);
@@ -678,7 +683,7 @@ static long task_count(const orc_bcsr* B, const uint32_t* t, uint32_t** hmap, ui
       }
     }
   } else {
-    for (uint32_t f = phase; f < h; f += stride) {
+    FOR_SAMPLED_ROWS(f, h, stride, phase) {
       if (rows[ac_src + f + 1] == rows[ac_src + f]) continue;
       c += list_intersect(B, ab_src, bc_src, ac_src, f);
     }
