@@ -28,6 +28,7 @@ CLI_PATH = os.path.join(_HERE, "bbtc")
 PART = {"uni": 0, "n1d": 1, "pal": 2, "rac": 3}
 ORDER = {"lex": 0, "desc": 1, "asc": 2}
 KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4, "lane": 5}
+BUILD_HOST_ARRAYS = 1  # bbtc_graph_build: copy the layout's arrays back to the host
 
 
 class BbtcError(RuntimeError):
@@ -100,6 +101,7 @@ def _declare(lib):
         "bbtc_layout_cuts": (vp, [vp]),
         "bbtc_layout_tile": (st, [vp, u32, u32, C.POINTER(bbtc_tile_info)]),
         "bbtc_layout_device_bytes": (u64, [vp]),
+        "bbtc_layout_arrays": (st, [vp, pvp, C.POINTER(u64), pvp, C.POINTER(u64), pvp, C.POINTER(u64)]),
         "bbtc_layout_save_bcsr": (st, [vp, C.c_char_p]),
         "bbtc_layout_load_bcsr": (st, [C.c_char_p, pvp, pvp]),
         "bbtc_layout_free": (None, [vp]),
@@ -115,6 +117,8 @@ def _declare(lib):
         "bbtc_ctx_synchronize": (st, [vp]),
         "bbtc_ctx_read_bandwidth": (st, [vp, u64, i32, vp]),
         "bbtc_graph_upload": (st, [vp, vp, pvp]),
+        "bbtc_graph_build": (st, [vp, vp, vp, u32, u32, pvp, pvp]),
+        "bbtc_graph_tasks_stats": (st, [vp, vp, u64, u32, vp]),
         "bbtc_graph_bytes": (u64, [vp]),
         "bbtc_graph_release": (None, [vp]),
         "bbtc_plan_create": (st, [vp, vp, vp, u64, i32, pvp]),
@@ -334,6 +338,18 @@ class Layout:
     def device_bytes(self) -> int:
         return lib().bbtc_layout_device_bytes(self._h)
 
+    def arrays(self):
+        """Copies of (rowptr, cols, nzrows) (bbtc_layout_arrays); None when
+        the layout holds no host arrays."""
+        ps = [C.c_void_p() for _ in range(3)]
+        ns = [C.c_uint64() for _ in range(3)]
+        _check(lib().bbtc_layout_arrays(self._h, C.byref(ps[0]), C.byref(ns[0]), C.byref(ps[1]),
+                                        C.byref(ns[1]), C.byref(ps[2]), C.byref(ns[2])))
+        if not ps[0].value:
+            return None
+        return tuple(np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(n.value,)).copy()
+                     for p, n in zip(ps, ns))
+
     def tasks(self, order: str = "lex") -> np.ndarray:
         n = C.c_uint64()
         cap = lib().bbtc_task_count(self.p)
@@ -412,6 +428,25 @@ class Device:
     def upload(self, layout: Layout) -> "DeviceGraph":
         return DeviceGraph(self, layout)
 
+    def build(self, graph: Graph, cuts, host_arrays: bool = True) -> "DeviceGraph":
+        """The block layout of graph built on this device (bbtc_graph_build):
+        a resident DeviceGraph whose .layout is the host side, with the
+        arrays copied back when host_arrays is set."""
+        c = np.ascontiguousarray(cuts, dtype=np.uint32)
+        lh, gh = C.c_void_p(), C.c_void_p()
+        _check(lib().bbtc_graph_build(self._h, graph._h, _ptr(c), len(c) - 1,
+                                      BUILD_HOST_ARRAYS if host_arrays else 0, C.byref(lh), C.byref(gh)))
+        L = Layout.__new__(Layout)
+        L.graph = graph
+        L._h = lh
+        L.p = len(c) - 1
+        L.cuts = c.copy()
+        g = DeviceGraph.__new__(DeviceGraph)
+        g.dev = self
+        g.layout = L
+        g._h = gh
+        return g
+
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.bbtc_ctx_destroy(self._h)
@@ -446,6 +481,13 @@ class DeviceGraph:
 
     def plan(self, tasks, kernel: str = "auto", choice=None) -> "Plan":
         return Plan(self, tasks, kernel, choice)
+
+    def task_stats(self, tasks, sample_units: int = 0):
+        """Layout.task_stats computed on the device (bbtc_graph_tasks_stats)."""
+        t = tasks_array(tasks)
+        out = np.zeros((len(t), 6), dtype=np.uint64)
+        _check(lib().bbtc_graph_tasks_stats(self._h, _ptr(t), len(t), sample_units, _ptr(out)))
+        return out
 
     def count(self, tasks, kernel: str = "auto"):
         t = tasks_array(tasks)

@@ -49,6 +49,7 @@ struct Params {
   int ncut = 32;
   int repeat = 5;
   bool copy_first = false;
+  bool device_layout = false;  // build the block layout on the GPU
   std::string kernel = "auto";
   int threads = 0;
   std::string cuts_file;
@@ -81,6 +82,8 @@ void usage() {
             << "  --border <int>        Work queue border for GPUs                 (0)\n"
             << "                          Note: if set to -1, GPUs process all.\n"
             << "  --copy-first          If set then copy graph to GPUs first       (false)\n"
+            << "  --device-layout       Build the block layout on the GPU from the CSR; it\n"
+            << "                          stays resident for every repeat (no tile copies)\n"
             << "  --repeat <int>        Run bbTC algorithm '--repeat' times        (5)\n"
             << "  --kernel <str>        auto | merge | bitmap | dense | rdense | lane | tuned (auto)\n"
             << "  --threads <int>       Host worker threads                        (all)\n"
@@ -158,6 +161,7 @@ Params parse(int argc, char** argv) {
       p.repeat = to_int(val("--repeat"), "--repeat");
       check_int("--repeat", p.repeat);
     } else if (!strcasecmp(a, "--copy-first")) p.copy_first = true;
+    else if (!strcasecmp(a, "--device-layout")) p.device_layout = true;
     else if (!strcasecmp(a, "--ncut")) {
       p.ncut = to_int(val("--ncut"), "--ncut");
       check_int("--ncut", p.ncut, 1);
@@ -188,6 +192,8 @@ Params parse(int argc, char** argv) {
     } else bad("Wrong argument '" + std::string(a) + "' given.");
   }
   if (!p.batch_list.empty() && !p.serve_list.empty()) bad("--batch and --serve are exclusive");
+  if (p.device_layout && (p.ngpu == 0 || !p.batch_list.empty() || !p.serve_list.empty()))
+    bad("--device-layout builds one graph's layout on a GPU (--ngpu 1, no --batch or --serve)");
   if (!p.forms_out.empty() || !p.forms_in.empty()) {
     if (!p.batch_list.empty() || !p.serve_list.empty()) bad("--forms-out and --forms-in count one graph");
     if (p.ngpu == 0) bad("--forms-out and --forms-in need a GPU (--ngpu 1)");
@@ -585,11 +591,22 @@ int main(int argc, char** argv) {
   }
   bbtc_hgraph* g = nullptr;
   bbtc_layout* L = nullptr;
+  bbtc_dgraph* dg = nullptr;
+  bbtc_ctx* ctx = nullptr;
   bbtc_status st = BBTC_OK;
   Fields F;
   uint32_t n = 0, ncols = 0;
   uint64_t m = 0;
   uint32_t p = uint32_t(P.ncut);
+  if (bcsr && P.device_layout) {
+    std::cerr << "bbtc: --device-layout builds the layout from a CSR graph, not from a .bcsr file" << std::endl;
+    return 1;
+  }
+  // bbtc.cu:54-62: tasks [border, n) are shared with the host workers
+  auto separator_of = [&](uint64_t ntasks) -> uint64_t {
+    if (P.border == -1 || uint64_t(P.border) > ntasks) return ntasks;
+    return uint64_t(P.border);
+  };
   if (bcsr) {
     // a stored block layout: its own cuts, no partitioning (block_csr.cpp:215-255)
     F.begin("Partitioning and Layout construction (s)");
@@ -618,8 +635,22 @@ int main(int argc, char** argv) {
       st = bbtc_partition(g, part_kind(P.part), p, cuts.data());
       if (st) die(st, "partitioning");
     }
-    st = bbtc_layout_build(g, cuts.data(), p, &L);
-    if (st) die(st, "building the block layout");
+    if (P.device_layout) {
+      F.end();
+      F.begin("HIP Initial Memory Allocations and Stream Creation");
+      st = bbtc_ctx_create(0, nullptr, &ctx);
+      if (st) die(st, "creating the device context");
+      F.end();
+      F.begin("Layout construction on the device (s)");
+      // the host arrays only where host code reads them: the host tail of
+      // the task queue and the .bcsr writer
+      const bool host = separator_of(bbtc_task_count(p)) < bbtc_task_count(p) || !P.save_bcsr.empty();
+      st = bbtc_graph_build(ctx, g, cuts.data(), p, host ? BBTC_BUILD_HOST_ARRAYS : 0u, &L, &dg);
+      if (st) die(st, "building the block layout on the device");
+    } else {
+      st = bbtc_layout_build(g, cuts.data(), p, &L);
+      if (st) die(st, "building the block layout");
+    }
     F.end();
   }
   if (!P.save_bcsr.empty()) {
@@ -647,31 +678,30 @@ int main(int argc, char** argv) {
   std::cout << "Number of Workers: " << (P.threads > 0 ? P.threads : omp_get_max_threads()) << std::endl;
   std::cout << "=====================================" << std::endl;
 
-  // bbtc.cu:54-62
-  uint64_t separator = 0;
-  if (P.border == -1 || uint64_t(P.border) > ntasks) separator = ntasks;
-  else separator = uint64_t(P.border);
+  const uint64_t separator = separator_of(ntasks);
 
   const bbtc_kernel kern = kernel_kind(P.kernel);
   // --kernel tuned: per-task forms measured once on the first upload
   std::vector<uint8_t> choice;
-  bbtc_ctx* ctx = nullptr;
-  F.begin("HIP Initial Memory Allocations and Stream Creation");
-  if (P.ngpu > 0) {
-    st = bbtc_ctx_create(0, nullptr, &ctx);
-    if (st) die(st, "creating the device context");
+  if (!ctx) {
+    F.begin("HIP Initial Memory Allocations and Stream Creation");
+    if (P.ngpu > 0) {
+      st = bbtc_ctx_create(0, nullptr, &ctx);
+      if (st) die(st, "creating the device context");
+    }
+    F.end();
   }
-  F.end();
 
   std::vector<uint64_t> per(ntasks, 0);
   uint64_t total = 0;
-  bbtc_dgraph* dg = nullptr;
   if (ctx && !P.forms_in.empty()) {
     choice = read_forms(P.forms_in, g, L, ntasks);
   } else if (ctx && P.kernel == "tuned") {
     F.begin("Autotune kernel forms (s)");
-    st = bbtc_graph_upload(ctx, L, &dg);
-    if (st) die(st, "uploading the graph");
+    if (!dg) {
+      st = bbtc_graph_upload(ctx, L, &dg);
+      if (st) die(st, "uploading the graph");
+    }
     choice.resize(std::max<uint64_t>(ntasks, 1));
     st = bbtc_autotune(ctx, dg, tasks.data(), ntasks, choice.data(), nullptr);
     if (st) die(st, "autotuning");
@@ -679,11 +709,13 @@ int main(int argc, char** argv) {
     if (!P.forms_out.empty()) write_forms(P.forms_out, g, L, choice, ntasks);
   }
   for (int r = 0; r < P.repeat; ++r) {
-    if (dg) {  // as ResetFlags (bb_tc_helper.cpp:31-50), every repeat starts without device tiles
+    // as ResetFlags (bb_tc_helper.cpp:31-50), every repeat starts without
+    // device tiles; a layout built on the device stays there
+    if (dg && !P.device_layout) {
       bbtc_graph_release(dg);
       dg = nullptr;
     }
-    if (ctx && P.copy_first) {
+    if (ctx && P.copy_first && !dg) {
       F.begin("Copy tiles to device (s)");
       st = bbtc_graph_upload(ctx, L, &dg);
       if (st) die(st, "uploading the graph");

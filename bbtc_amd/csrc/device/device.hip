@@ -165,9 +165,13 @@ void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t i
 std::vector<TaskWork> task_work(const Layout& L, const bbtc_task* tasks, uint64_t n) {
   std::vector<uint64_t> st(n * kTaskStats);
   task_stats(L, tasks, n, st.data(), kRuleSampleUnits);
+  return work_from_stats(st.data(), n);
+}
+
+std::vector<TaskWork> work_from_stats(const uint64_t* st, uint64_t n) {
   std::vector<TaskWork> w(n);
   for (uint64_t q = 0; q < n; ++q) {
-    const uint64_t* x = st.data() + kTaskStats * q;
+    const uint64_t* x = st + kTaskStats * q;
     w[q] = TaskWork{float(x[0]), float(x[1]), float(x[2]), float(x[3]), float(x[4]), float(x[5])};
   }
   return w;
@@ -754,6 +758,7 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
   DEV_REQUIRE(ctx && Lh && out, "null argument");
   DEV_TRY
   const bbtc::Layout& L = Lh->L;
+  bbtc::require_host_arrays(L, "bbtc_graph_upload");
   auto* g = new bbtc_dgraph;
   g->ctx = ctx;
   g->P = L.P;
@@ -1158,6 +1163,7 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
     std::vector<uint64_t> rp0(ngraphs), col0(ngraphs), nz0(ngraphs), dn0(ngraphs), tile0(ngraphs);
     for (uint64_t gi = 0; gi < ngraphs; ++gi) {
       const bbtc::Layout& L = layouts[gi]->L;
+      bbtc::require_host_arrays(L, "bbtc_batch_create");
       rp0[gi] = n_rp;
       col0[gi] = n_col;
       nz0[gi] = n_nz;

@@ -65,6 +65,8 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
 
 // Work statistics (estimated, form_rule.hpp) of tasks of a host layout.
 std::vector<TaskWork> task_work(const Layout& L, const bbtc_task* tasks, uint64_t n);
+// The same from task_stats output (kTaskStats numbers per task).
+std::vector<TaskWork> work_from_stats(const uint64_t* st, uint64_t n);
 
 // Resident blocks of count_probe (all forms, or lane form only) on the
 // context's device.

@@ -739,6 +739,7 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
       return BBTC_ERR_INVALID;
     }
   SRV_TRY
+  bbtc::require_host_arrays(L->L, "bbtc_server_submit");
   auto r = std::make_shared<bbtc::Request>();
   r->L = &L->L;
   r->tiles = bbtc::tile_table(L->L);

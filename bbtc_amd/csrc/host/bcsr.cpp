@@ -42,6 +42,7 @@ void get(std::FILE* f, void* p, size_t bytes, const std::string& path) {
 }  // namespace
 
 void save_bcsr(const Layout& L, const std::string& path) {
+  require_host_arrays(L, "bbtc_layout_save_bcsr");
   const uint32_t P = L.P, N = L.n;
   if (L.m >= (1ull << 32)) fail(BBTC_ERR_INVALID, ".bcsr holds at most 2^32-1 entries");
   if ((uint64_t(N) + 1) * P >= (1ull << 32)) fail(BBTC_ERR_INVALID, ".bcsr holds at most 2^32-1 row slots");

@@ -169,6 +169,7 @@ uint64_t count_task_cpu(const Layout& L, const bbtc_task& x, CpuWorker& w) {
 // integers is the same whichever worker counts which rows.
 uint64_t count_cpu(const Layout& L, const bbtc_task* tasks, uint64_t n, int nthreads,
                    uint64_t* per_task) {
+  require_host_arrays(L, "the CPU count");
   // first[s]: the first unit of the s-th task from the back
   std::vector<uint64_t> first(n + 1, 0);
   for (uint64_t s = 0; s < n; ++s) {

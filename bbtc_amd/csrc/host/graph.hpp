@@ -73,6 +73,12 @@ struct Layout {
   std::vector<uint64_t> dense_base; // P*P, ~0 when the tile has no bitmap form
   std::vector<uint32_t> dwpr;       // P, bitmap words per row of band k
   uint64_t dense_words = 4;         // words of all bitmap rows (device only)
+  uint64_t rp_words = 0;            // entries of rowptr, cols and nzrows
+  uint64_t col_words = 1;
+  uint64_t nz_words = 1;
+  // false for a layout built on the device without a host copy of its arrays:
+  // only the per-tile fields above are valid then
+  bool host_arrays = true;
   Buf<uint32_t> rowptr;
   Buf<uint32_t> cols;
   Buf<uint32_t> nzrows;
@@ -80,11 +86,25 @@ struct Layout {
   uint32_t height(uint32_t b) const { return cuts[b + 1] - cuts[b]; }
   size_t tile(uint32_t i, uint32_t k) const { return size_t(i) * P + k; }
   uint64_t device_bytes() const {
-    return rowptr.size() * 4 + cols.size() * 4 + nzrows.size() * 4 + dense_words * 4 +
+    return rp_words * 4 + col_words * 4 + nz_words * 4 + dense_words * 4 +
            cuts.size() * 4 + 4 * rp_base.size() * 8 + nnz.size() * 8;
   }
 };
 Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts);
+// The parts of build_layout shared with the device build: the checked cuts,
+// per-tile vectors and row-offset bases of an n-row graph (no arrays)...
+Layout layout_frame(uint32_t n, uint64_t m, const std::vector<uint32_t>& cuts);
+// ...and, once nnz, nz_rows and max_row of every tile are known, the column
+// and row-list bases, the tile degrees and the bitmap placement.
+void layout_place(Layout& L);
+// memcpy by all OpenMP threads (the destination's pages are touched in
+// parallel, which a fresh host array needs to fill at memory speed).
+void parallel_copy(void* dst, const void* src, size_t bytes);
+// One past the largest column of g (0 when it has no entries); a layout
+// needs it at most cuts[P].
+uint64_t column_end(const Csr& g);
+// Throws BBTC_ERR_INVALID naming `what` when L holds no host arrays.
+void require_host_arrays(const Layout& L, const char* what);
 
 // ---- tasks ----------------------------------------------------------------
 std::vector<bbtc_task> task_list(uint32_t p);

@@ -164,6 +164,21 @@ bbtc_status bbtc_layout_tile(const bbtc_layout* L, uint32_t i, uint32_t k, bbtc_
 }
 
 uint64_t bbtc_layout_device_bytes(const bbtc_layout* L) { return L ? L->L.device_bytes() : 0; }
+
+bbtc_status bbtc_layout_arrays(const bbtc_layout* L, const uint32_t** rowptr, uint64_t* rowptr_n,
+                               const uint32_t** cols, uint64_t* cols_n, const uint32_t** nzrows,
+                               uint64_t* nzrows_n) {
+  REQUIRE(L && rowptr && rowptr_n && cols && cols_n && nzrows && nzrows_n, "null argument");
+  const bbtc::Layout& x = L->L;
+  const bool h = x.host_arrays;
+  *rowptr = h ? x.rowptr.data() : nullptr;
+  *rowptr_n = h ? x.rowptr.size() : 0;
+  *cols = h ? x.cols.data() : nullptr;
+  *cols_n = h ? x.cols.size() : 0;
+  *nzrows = h ? x.nzrows.data() : nullptr;
+  *nzrows_n = h ? x.nzrows.size() : 0;
+  return BBTC_OK;
+}
 void bbtc_layout_free(bbtc_layout* L) { delete L; }
 
 uint64_t bbtc_task_count(uint32_t p) { return uint64_t(p) * (p + 1) * (p + 2) / 6; }

@@ -67,6 +67,7 @@ constexpr uint32_t kStatUnits = 16;  // units of 64 rows read per task for the e
 HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bbtc_task* tasks,
                         uint64_t n, uint64_t sep, int nthreads, int nstreams, bbtc_kernel kernel,
                         const uint8_t* choice, uint64_t* per_task) {
+  if (sep < n) require_host_arrays(L, "the host tail of bbtc_count_hybrid");
   bbtc_plan* plan = nullptr;
   const bbtc_status st = bbtc_plan_create_tuned(ctx, g, tasks, n, kernel, choice, &plan);
   if (st != BBTC_OK) fail(st, bbtc_last_error());

@@ -48,18 +48,18 @@ static void build_dense(Layout& L) {
   L.dense_words = std::max<uint64_t>(total, 4);
 }
 
-Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
+Layout layout_frame(uint32_t n, uint64_t m, const std::vector<uint32_t>& cuts) {
   if (cuts.size() < 2) fail(BBTC_ERR_INVALID, "need at least one band");
   const uint32_t P = uint32_t(cuts.size() - 1);
   if (cuts[0] != 0) fail(BBTC_ERR_INVALID, "cuts[0] must be 0");
   for (uint32_t b = 0; b < P; ++b)
     if (cuts[b] > cuts[b + 1]) fail(BBTC_ERR_INVALID, "cuts must not decrease");
-  if (cuts[P] < g.n) fail(BBTC_ERR_INVALID, "cuts[P] must cover every vertex");
+  if (cuts[P] < n) fail(BBTC_ERR_INVALID, "cuts[P] must cover every vertex");
 
   Layout L;
   L.P = P;
-  L.n = g.n;
-  L.m = g.m;
+  L.n = n;
+  L.m = m;
   L.cuts = cuts;
   const size_t T = size_t(P) * P;
   L.rp_base.assign(T, ~0ull);
@@ -69,13 +69,69 @@ Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
   L.nz_rows.assign(T, 0);
   L.max_row.assign(T, 0);
   L.deg.assign(T, 0.f);
-
   uint64_t rp_total = 0;
   for (uint32_t i = 0; i < P; ++i)
     for (uint32_t k = i; k < P; ++k) {
       L.rp_base[L.tile(i, k)] = rp_total;
       rp_total += uint64_t(L.height(i)) + 1;
     }
+  L.rp_words = rp_total;
+  return L;
+}
+
+void layout_place(Layout& L) {
+  const uint32_t P = L.P;
+  uint64_t col_total = 0, nz_total = 0;
+  for (uint32_t i = 0; i < P; ++i)
+    for (uint32_t k = i; k < P; ++k) {
+      const size_t t = L.tile(i, k);
+      L.col_base[t] = col_total;
+      col_total = align_up(col_total + L.nnz[t]);
+      L.nz_base[t] = nz_total;
+      nz_total += L.nz_rows[t];
+    }
+  L.col_words = std::max<uint64_t>(col_total, 1);
+  L.nz_words = std::max<uint64_t>(nz_total, 1);
+  for (uint32_t i = 0; i < P; ++i)
+    for (uint32_t k = 0; k < P; ++k) {
+      const size_t t = L.tile(i, k);
+      // block_csr.cpp:272-278: 32-bit sum over the rows, then float division
+      const uint32_t nnz32 = uint32_t(L.nnz[t]);
+      L.deg[t] = static_cast<float>(nnz32) / static_cast<float>(L.height(i));
+    }
+  build_dense(L);
+}
+
+void require_host_arrays(const Layout& L, const char* what) {
+  if (!L.host_arrays)
+    fail(BBTC_ERR_INVALID, std::string(what) +
+                               " reads the layout's arrays, which a layout built on the device without "
+                               "BBTC_BUILD_HOST_ARRAYS does not hold on the host");
+}
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kPiece = 1 << 20;
+  const int64_t pieces = int64_t((bytes + kPiece - 1) / kPiece);
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < pieces; ++q) {
+    const size_t o = size_t(q) * kPiece;
+    std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(kPiece, bytes - o));
+  }
+}
+
+uint64_t column_end(const Csr& g) {
+  uint64_t mx = 0;
+#pragma omp parallel for reduction(max : mx) schedule(static, 1 << 16)
+  for (int64_t u = 0; u < int64_t(g.n); ++u)
+    if (g.off[u + 1] > g.off[u]) mx = std::max<uint64_t>(mx, uint64_t(g.col[g.off[u + 1] - 1]) + 1);
+  return mx;
+}
+
+Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
+  Layout L = layout_frame(g.n, g.m, cuts);
+  const uint32_t P = L.P;
+  if (column_end(g) > cuts[P]) fail(BBTC_ERR_INVALID, "a column lies at or beyond cuts[P]");
+  const uint64_t rp_total = L.rp_words;
   L.rowptr.resize(rp_total);
   uint32_t* rp = L.rowptr.data();
 #pragma omp parallel for schedule(static, 1 << 16)
@@ -132,16 +188,12 @@ Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
   }
   if (overflow) fail(BBTC_ERR_INVALID, "a tile holds 2^32 or more entries; use more cuts");
 
-  uint64_t col_total = 0, nz_total = 0;
-  for (auto& ik : upper) {
-    const size_t t = L.tile(ik.first, ik.second);
-    L.col_base[t] = col_total;
-    col_total = align_up(col_total + L.nnz[t]);
-    L.nz_base[t] = nz_total;
-    nz_total += L.nz_rows[t];
-  }
-  L.cols.resize(std::max<uint64_t>(col_total, 1));
-  L.nzrows.resize(std::max<uint64_t>(nz_total, 1));
+  layout_place(L);
+  L.cols.resize(L.col_words);
+  L.nzrows.resize(L.nz_words);
+  // a layout with no entries keeps one zero word in each array
+  L.cols[0] = 0;
+  L.nzrows[0] = 0;
   uint32_t* cl = L.cols.data();
   // zero the alignment padding so the array content is deterministic
 #pragma omp parallel for schedule(dynamic, 1)
@@ -182,14 +234,6 @@ Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
     for (uint32_t r = 0; r < h; ++r)
       if (row[r + 1] > row[r]) nz[w++] = r;
   }
-  for (uint32_t i = 0; i < P; ++i)
-    for (uint32_t k = 0; k < P; ++k) {
-      const size_t t = L.tile(i, k);
-      // block_csr.cpp:272-278: 32-bit sum over the rows, then float division
-      const uint32_t nnz32 = uint32_t(L.nnz[t]);
-      L.deg[t] = static_cast<float>(nnz32) / static_cast<float>(L.height(i));
-    }
-  build_dense(L);
   return L;
 }
 
@@ -223,6 +267,7 @@ bool task_skipped(const Layout& L, const bbtc_task& x) {
 
 void task_bytes(const Layout& L, const bbtc_task* tasks, uint64_t n, uint32_t o, uint64_t* per_task,
                 uint64_t* total, uint64_t* wedges) {
+  require_host_arrays(L, "bbtc_tasks_bytes");
   uint64_t tot = 0, wtot = 0;
   const uint32_t* rp = L.rowptr.data();
   const uint32_t* cl = L.cols.data();
@@ -259,6 +304,7 @@ void task_bytes(const Layout& L, const bbtc_task* tasks, uint64_t n, uint32_t o,
 }
 
 void task_stats(const Layout& L, const bbtc_task* tasks, uint64_t n, uint64_t* out, uint32_t sample_units) {
+  require_host_arrays(L, "bbtc_tasks_stats");
   const uint32_t* rp = L.rowptr.data();
   const uint32_t* cl = L.cols.data();
 #pragma omp parallel for schedule(dynamic, 1)

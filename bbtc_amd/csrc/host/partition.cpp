@@ -252,29 +252,37 @@ namespace {
 // Returns the sampled entries as (row offsets, cols).
 void sparsify(const Csr& g, double keep, int seed, std::vector<u32>& sp_off, std::vector<u32>& sp_col) {
   const size_t N = size_t(u32(g.m));  // the reference's nnz is a 32-bit Ordinal
-  std::vector<bool> mask(N, false);
+  // positions of the kept entries, increasing: the draws of the mask, one per
+  // entry or one per gap between kept entries
+  std::vector<uint64_t> kept;
   std::mt19937_64 gen(seed);
   std::uniform_real_distribution<double> rnd;
   if (keep >= 1.0) {
-    mask.assign(N, true);
+    kept.resize(N);
+    for (size_t i = 0; i < N; ++i) kept[i] = i;
   } else if (keep > 1.0 / 3) {
-    for (size_t i = 0; i < N; ++i) mask[i] = rnd(gen) <= keep;
+    for (size_t i = 0; i < N; ++i)
+      if (rnd(gen) <= keep) kept.push_back(i);
   } else {
     const double Q = std::log(1 - keep);
     for (size_t i = std::numeric_limits<size_t>::max();;) {
       const size_t fc = size_t(std::log(1 - rnd(gen)) / Q);
-      i += std::min(fc, mask.size());
-      if (++i < mask.size()) mask[i] = true;
+      i += std::min(fc, N);
+      if (++i < N) kept.push_back(i);
       else break;
     }
   }
+  // the row of each kept entry; rows keep their entries in column order
   sp_off.assign(size_t(g.n) + 1, 0);
-  sp_col.clear();
-  for (u32 r = 0; r < g.n; ++r) {
-    sp_off[r + 1] = sp_off[r];
-    for (uint64_t x = g.off[r]; x < g.off[r + 1]; ++x)
-      if (x < N && mask[x]) { sp_col.push_back(g.col[x]); sp_off[r + 1]++; }
+  sp_col.resize(kept.size());
+  const uint64_t* off = g.off.data();
+  for (size_t s = 0; s < kept.size(); ++s) {
+    const uint64_t x = kept[s];
+    const u32 r = u32(std::upper_bound(off, off + g.n + 1, x) - off - 1);
+    sp_off[size_t(r) + 1]++;
+    sp_col[s] = g.col[x];
   }
+  for (size_t r = 0; r < g.n; ++r) sp_off[r + 1] += sp_off[r];
 }
 
 // Exact dominance counts over the sample: query(i, j) = #samples with

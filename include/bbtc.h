@@ -152,6 +152,16 @@ bbtc_status bbtc_layout_load_bcsr(const char* path, bbtc_hgraph** graph_out, bbt
 
 /* Bytes the layout occupies once uploaded. */
 uint64_t bbtc_layout_device_bytes(const bbtc_layout* L);
+/* The layout's three arrays as the device holds them (read-only, owned by
+ * L): per upper tile its h + 1 tile-local row offsets, its tile-local
+ * columns (each tile from a 32-entry boundary, padding zero) and its
+ * non-empty local rows, tiles in (i, k) order, i <= k. Sets every pointer to
+ * NULL and every size to 0 for a layout built on the device without its host
+ * arrays. No counterpart in the reference (its BlockCSR keeps (N+P)*P
+ * global row offsets and no row lists). */
+bbtc_status bbtc_layout_arrays(const bbtc_layout* L, const uint32_t** rowptr, uint64_t* rowptr_n,
+                               const uint32_t** cols, uint64_t* cols_n, const uint32_t** nzrows,
+                               uint64_t* nzrows_n);
 void bbtc_layout_free(bbtc_layout* L);
 
 /* ------------------------------------------------------------------------ */
@@ -277,6 +287,28 @@ bbtc_status bbtc_hgraph_gen_gnm_device(bbtc_ctx* ctx, uint32_t n, uint64_t m, ui
 bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* L, bbtc_dgraph** out);
 uint64_t bbtc_graph_bytes(const bbtc_dgraph* g);
 void bbtc_graph_release(bbtc_dgraph* g);
+
+/* Replaces BlockCSR(cuts, x_adj, adj) (block_csr.cpp:286-376) followed by
+ * CopyTilesToGpu (bb_tc_helper.cpp:75-172): the CSR is copied to the device
+ * once and the block layout is built there by data-parallel kernels (row
+ * counts, per-tile scans, column scatter), then stays resident. The device
+ * arrays are those bbtc_layout_build + bbtc_graph_upload make, bit for bit,
+ * and so are the work statistics the AUTO rule reads (computed on the
+ * device). *layout_out receives the host side: the cuts and every per-tile
+ * field (bbtc_layout_tile, bbtc_tasks_build). With BBTC_BUILD_HOST_ARRAYS
+ * the arrays are also copied back, so every entry point that takes a
+ * bbtc_layout works on it; without, those that read the arrays
+ * (bbtc_tasks_bytes, bbtc_tasks_stats, bbtc_count_cpu, the host tail of
+ * bbtc_count_hybrid, bbtc_graph_upload, bbtc_layout_save_bcsr, batches and
+ * the server) return BBTC_ERR_INVALID. Requires every column of g below
+ * cuts[p]. Free both outputs (bbtc_layout_free, bbtc_graph_release). */
+#define BBTC_BUILD_HOST_ARRAYS 1u
+bbtc_status bbtc_graph_build(bbtc_ctx* ctx, const bbtc_hgraph* g, const uint32_t* cuts, uint32_t p,
+                             uint32_t flags, bbtc_layout** layout_out, bbtc_dgraph** graph_out);
+/* bbtc_tasks_stats computed on the device from a resident graph: the same
+ * six numbers per task, bit for bit. No counterpart in the reference. */
+bbtc_status bbtc_graph_tasks_stats(bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n, uint32_t sample_units,
+                                   uint64_t* stats);
 
 /* A plan binds a graph to a task list and a kernel choice: it uploads task
  * descriptors and the work-item table once, so repeated launches do no host

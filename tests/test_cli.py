@@ -77,6 +77,7 @@ def test_flags_are_case_insensitive(c1):
     (["--partitioner", "xyz"], "Wrong partitioning algorithm."),
     (["--kernel", "xyz"], "Wrong kernel."),
     (["--bogus", "1"], "Wrong argument '--bogus' given."),
+    (["--device-layout"], "--device-layout builds one graph's layout on a GPU"),
 ])
 def test_validation(c1, args, msg):
     _, path = c1

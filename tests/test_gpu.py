@@ -212,7 +212,8 @@ def test_cli_gpu(tmp_path):
     for run in d["runs"]:
         for extra in ([], ["--border", "-1"], ["--border", "7", "--nstream", "2"], ["--copy-first"],
                       ["--kernel", "tuned", "--border", "-1"], ["--kernel", "rdense"], ["--kernel", "dense"],
-                      ["--kernel", "lane"]):
+                      ["--kernel", "lane"], ["--device-layout"], ["--device-layout", "--border", "-1"],
+                      ["--device-layout", "--kernel", "tuned", "--border", "5"]):
             out = tmp_path / "pt.txt"
             cmd = [B.CLI_PATH, "--graph", path, "--partitioner", run["partitioner"], "--ncut", str(run["ncut"]),
                    "--repeat", "2", "--ngpu", "1", "--per-task-out", str(out)] + extra
@@ -220,6 +221,7 @@ def test_cli_gpu(tmp_path):
             assert r.returncode == 0, r.stderr
             assert f"Number of Triangles: {run['total']}" in r.stdout
             assert "Number of GPUs: 1" in r.stdout
+            assert ("Layout construction on the device" in r.stdout) == ("--device-layout" in extra)
             got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
             assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
 

@@ -175,3 +175,18 @@ def test_task_stats(p, kind):
         R, W, A, Pbc = (st[:, c].astype(np.int64) for c in range(4))
         assert np.array_equal(per.astype(np.int64), 4 * o * R + 4 * W + 4 * A + 2 * o * W + 4 * Pbc)
         assert wedges == int(W.sum())
+
+
+def test_column_beyond_last_cut_is_refused():
+    """A graph whose column count exceeds its rows may hold a column at or
+    beyond cuts[P]; the layout refuses it instead of placing it in a tile that
+    does not exist."""
+    off = np.array([0, 2, 3, 3, 3], dtype=np.uint64)
+    cols = np.array([1, 8, 2], dtype=np.uint32)
+    g = B.Graph.from_csr(off, cols, n=4, ncols=10)
+    with pytest.raises(B.BbtcError, match="beyond"):
+        B.Layout(g, [0, 2, 5])
+    L = B.Layout(g, [0, 2, 9])
+    assert L.tile(0, 0).nnz == 1 and L.tile(0, 1).nnz == 2 and L.tile(1, 1).nnz == 0
+    rp, cl, nz = L.arrays()
+    assert list(nz) == [0, 0, 1]  # tile (0,0): row 0; tile (0,1): rows 0 and 1
