@@ -105,7 +105,8 @@ struct SlotCtl {
   unsigned long long chunk;  // items per chunk (a multiple of kItemBatch)
   unsigned long long shift;  // log2(chunk), or 63 with one chunk per part
   unsigned long long spill;  // (records << kSpillShift) | chunks reserved
-  unsigned long long pad[kCtlStride - 4];
+  unsigned long long front;  // largest front of the shared task queue seen (CountArgs::queue)
+  unsigned long long pad[kCtlStride - 5];
 };
 static_assert(sizeof(SlotCtl) % 128 == 0, "SlotCtl layout");
 static_assert(kListChunkItems % kItemBatch == 0, "claims must not straddle chunks");
@@ -203,6 +204,15 @@ struct CountArgs {
   // the device wall clock when a graph's last unit finished.
   unsigned long long* left;
   unsigned long long* done;
+  // Task queue shared with host workers (NULL: none; one segment): the
+  // word (front << 32) | back in host-coherent memory. Tasks [front, back)
+  // are unclaimed; the device claims from the front with system-scope
+  // atomics, host workers from the back. Tasks below queue_lo are the
+  // device's alone; a claim reaches about queue_ahead items past the start
+  // of the task that needs it.
+  unsigned long long* queue;
+  uint32_t queue_lo;
+  uint32_t queue_ahead;
 };
 
 }  // namespace bbtc

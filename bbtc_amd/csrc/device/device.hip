@@ -302,7 +302,8 @@ void zero(bbtc_plan* p, hipStream_t s) { enqueue_zero(p->per_task, p->ntasks, s)
 // Counts tasks [t0, t1) of the plan on stream s with work counter `slot`.
 // Launches on one slot must be ordered (same stream); distinct slots may run
 // concurrently.
-void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t s) {
+void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t s,
+                  unsigned long long* queue = nullptr, uint32_t queue_lo = 0, uint32_t queue_ahead = 0) {
   const uint64_t i0 = p->hprefix[t0], i1 = p->hprefix[t1];
   p->last_stream = s;
   if (i1 <= i0) return;
@@ -322,6 +323,9 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   a.spill_cap = p->spill_cap;
   a.left = p->track_left;
   a.done = p->track_done;
+  a.queue = queue;
+  a.queue_lo = queue_lo;
+  a.queue_ahead = queue_ahead;
   enqueue_count(p->ctx, a, i0, i1 - i0, p->lane_only, p->chunk, s);
 }
 
@@ -610,6 +614,48 @@ bool plan_slot_idle(bbtc_plan* p, int slot) {
 }
 
 void plan_slot_wait(bbtc_plan* p, int slot) { HIP_OK(hipStreamSynchronize(slot_stream(p->ctx, slot))); }
+
+bool host_atomics(bbtc_ctx* ctx) {
+  if (ctx->host_atomics < 0) {
+    int v = 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipDeviceGetAttribute(&v, hipDeviceAttributeHostNativeAtomicSupported, ctx->device));
+    ctx->host_atomics = v ? 1 : 0;
+  }
+  return ctx->host_atomics == 1;
+}
+
+uint64_t* queue_alloc(bbtc_ctx* ctx) {
+  void* p = nullptr;
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipHostMalloc(&p, 128, hipHostMallocMapped | hipHostMallocCoherent));
+  return static_cast<uint64_t*>(p);
+}
+
+void queue_free(uint64_t* q) {
+  if (q) (void)hipHostFree(q);
+}
+
+uint64_t plan_contested_from(const bbtc_plan* p) {
+  const uint64_t len = p->hprefix[p->ntasks];
+  if (len == 0 || p->chunk != 0) return p->ntasks;  // nothing to count, or chunks dealt round robin
+  // the last part with items starts at item q * C; the task holding that item
+  // may also have items in the part before, so the contest starts after it
+  const unsigned long long C = part_chunk(len, 0);
+  const uint64_t first = (len - 1) / C * C;
+  const uint64_t t = uint64_t(std::upper_bound(p->hprefix.begin(), p->hprefix.end(), first) - p->hprefix.begin()) - 1;
+  return t + 1;
+}
+
+void plan_launch_shared(bbtc_plan* p, uint64_t* queue, uint32_t queue_lo) {
+  HIP_OK(hipSetDevice(p->ctx->device));
+  void* dq = nullptr;
+  HIP_OK(hipHostGetDevicePointer(&dq, queue, 0));
+  // a claim reaches one item batch per resident wave past the task that needs it
+  const uint64_t ahead = resident_blocks(p->ctx, p->lane_only) * kWavesPerBlock * kItemBatch;
+  launch_range(p, 0, p->ntasks, 0, p->ctx->stream, static_cast<unsigned long long*>(dq), queue_lo,
+               uint32_t(std::min<uint64_t>(ahead, 0xFFFFFFFFull)));
+}
 
 void plan_copy_counts(bbtc_plan* p, uint64_t* out) {
   HIP_OK(hipSetDevice(p->ctx->device));

@@ -197,6 +197,35 @@ __global__ void reset_slot(SlotCtl* __restrict__ c, unsigned long long i0, unsig
   c->chunk = C;
   c->shift = part_shift(len, chunk);
   c->spill = 0;
+  c->front = 0;
+}
+
+// True when task t (t >= queue_lo) is the device's: below the queue's front,
+// or claimed now by moving the front past it (through about queue_ahead
+// more items) while it is below the back. False once a host worker has it.
+// Lane 0 does the atomics; every CAS that fails means another claimant
+// moved the front or the back, so the loop ends.
+__device__ __forceinline__ uint32_t claim_shared(const CountArgs& a, const uint64_t* prefix, uint32_t ntasks,
+                                              uint32_t t) {
+  const gptr<uint64_t> ip = as_global(prefix);
+  unsigned long long cur = __hip_atomic_load(a.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  for (;;) {
+    const uint32_t f = uint32_t(cur >> 32), b = uint32_t(cur & 0xFFFFFFFFull);
+    if (t < f) {
+      atomicMax(&a.ctl->front, (unsigned long long)f);
+      return 1;
+    }
+    if (t >= b) return 0;
+    const uint64_t want = ip[t] + a.queue_ahead;
+    uint32_t t1 = want >= ip[b] ? b : find_task(ip, ntasks, want) + 1;
+    t1 = t1 < t + 1 ? t + 1 : (t1 > b ? b : t1);
+    const unsigned long long next = (uint64_t(t1) << 32) | b;
+    if (__hip_atomic_compare_exchange_strong(a.queue, &cur, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_SYSTEM)) {
+      atomicMax(&a.ctl->front, (unsigned long long)t1);
+      return 1;
+    }
+  }
 }
 
 // One persistent launch per phase. Phase 0 drains the first-phase items
@@ -337,6 +366,20 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
       const uint64_t e = ip[t_cur + 1];
       t_lo = S.launch0 + ((t_first > S.item0 ? t_first : S.item0) - S.item0);
       t_hi = S.launch0 + ((e < S.item1 ? e : S.item1) - S.item0);
+      if (a.queue && a.phase == 0 && t_cur >= a.queue_lo) {
+        uint32_t own = 0;
+        if (lane == 0) own = t_cur < a.ctl->front ? 1u : claim_shared(a, S.item_prefix, S.ntasks, t_cur);
+        if (!__builtin_amdgcn_readfirstlane(own)) {
+          // a host worker has this task and every later one: the rest of
+          // this part is theirs
+          t_lo = 1;
+          t_hi = 0;
+          item = batch_end;
+          part = part + 1 == kQueues ? 0 : part + 1;
+          --parts_left;
+          continue;
+        }
+      }
     }
     const SegDesc& S = a.segs[s_cur];
     const unsigned long long t_start = S.cycles ? clock64() : 0;

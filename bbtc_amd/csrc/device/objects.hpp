@@ -31,6 +31,7 @@ struct bbtc_ctx {
   uint64_t budget = 0;
   uint64_t in_use = 0;
   unsigned char* scratch = nullptr;  // this device's address of the scratch array (bbtc_count_noalloc)
+  int host_atomics = -1;             // device atomics on host memory supported (-1: not asked yet)
 };
 
 struct bbtc_dgraph {

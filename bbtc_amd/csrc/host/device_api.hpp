@@ -24,6 +24,18 @@ void plan_slot_wait(bbtc_plan* p, int slot);
 // Waits for all slots, copies all per-task counters to host.
 void plan_copy_counts(bbtc_plan* p, uint64_t* out);
 int max_slots();
+// Task queue shared by the device and host workers (CountArgs::queue):
+// true when the device's atomics reach host memory over its link.
+bool host_atomics(bbtc_ctx* ctx);
+// A host-coherent word for the queue (128 bytes), and its release.
+uint64_t* queue_alloc(bbtc_ctx* ctx);
+void queue_free(uint64_t* q);
+// The first task the host may share: every item of it and of the tasks
+// after it lies in the last part of a whole-plan launch (ntasks: none).
+uint64_t plan_contested_from(const bbtc_plan* p);
+// Enqueues the count of the whole plan on slot 0 with the device claiming
+// tasks [queue_lo, n) from the queue word's front (queue: host pointer).
+void plan_launch_shared(bbtc_plan* p, uint64_t* queue, uint32_t queue_lo);
 // Estimated work of each task (algorithmic bytes with 4-byte offsets) from the
 // work statistics the graph gathered at upload; false when it has none.
 bool graph_task_work(const bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n, double* out);

@@ -3,12 +3,15 @@
 // The reference (bbtc.cu:54-62; bb_tc.cpp:142-197) gives tasks
 // [0, Separator) to GPU worker threads only, and lets GPU workers (from the
 // front, one launch per task) and CPU workers (from the back, one task at a
-// time) race for [Separator, |T|) with a per-task atomic_flag. Here one host
-// thread feeds the device launches of tasks from the front of the list, up
-// to `nstreams` in flight on separate streams; the other OpenMP threads
-// claim single tasks from the back. Both sides claim from one packed 64-bit
-// word (next front task, one past the next back task), so every task is
-// counted exactly once.
+// time) race for [Separator, |T|) with a per-task atomic_flag. Here both
+// sides claim from one packed 64-bit word (next front task, one past the
+// next back task), so every task is counted exactly once, and the other
+// OpenMP threads of the caller's team claim single tasks from the back.
+// Where the device's atomics reach host memory (run_shared) the word lives
+// in host-coherent memory and the device's own waves move the front inside
+// one persistent launch; otherwise (run_hybrid) one host thread moves it on
+// the device's behalf, feeding launches of tasks from the front, up to
+// `nstreams` in flight on separate streams.
 //
 // Each task's work is estimated from its sampled work statistics (the
 // algorithmic bytes they imply). A host worker takes the back task only
@@ -64,6 +67,112 @@ std::atomic<double> last_device_rate(kDeviceRate), last_host_rate(kHostRate);
 constexpr double kHostShare = 0.5;
 constexpr uint32_t kStatUnits = 16;  // units of 64 rows read per task for the estimates
 
+// The device and the host workers claim from one word in host-coherent
+// memory, (front << 32) | back: the device's waves move the front with
+// system-scope atomics as they reach tasks [lo, n) (the last part of the
+// launch), the host workers take single tasks from the back under the same
+// rule as above. One launch counts the whole list; a wave that meets a task
+// a host worker took leaves the rest of that part to the host.
+template <class Work>
+HybridResult run_shared(bbtc_ctx* ctx, bbtc_plan* plan, const Layout& L, const bbtc_task* tasks, uint64_t n,
+                        uint64_t lo, int T, const Work& work, std::vector<uint64_t>& cpu_counts,
+                        uint64_t* per_task) {
+  struct Word {
+    uint64_t* p;
+    ~Word() { dev::queue_free(p); }
+  } word{dev::queue_alloc(ctx)};
+  uint64_t* W = word.p;
+  __atomic_store_n(W, pack(lo, n), __ATOMIC_SEQ_CST);
+  using Clock = std::chrono::steady_clock;
+  const double dev_guess = last_device_rate.load(), host_guess = last_host_rate.load();
+  std::atomic<double> host_work(0.0), host_secs(0.0), dev_secs(0.0);
+  std::atomic<bool> dev_finished(false);
+  std::exception_ptr err;
+  std::mutex errm;
+  Clock::time_point t0{};
+  auto host_rate = [&]() {
+    const double s2 = host_secs.load();
+    return s2 > 0.01 ? host_work.load() / s2 : host_guess;
+  };
+#pragma omp parallel num_threads(T)
+  {
+    if (omp_get_thread_num() == 0) {
+      try {
+        t0 = Clock::now();
+        dev::plan_launch_shared(plan, W, uint32_t(lo));
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(errm);
+        if (!err) err = std::current_exception();
+        __atomic_store_n(W, pack(0, 0), __ATOMIC_SEQ_CST);
+      }
+    }
+#pragma omp barrier
+    if (omp_get_thread_num() == 0) {
+      try {
+        if (!err) dev::plan_slot_wait(plan, 0);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(errm);
+        if (!err) err = std::current_exception();
+      }
+      dev_secs.store(std::chrono::duration<double>(Clock::now() - t0).count());
+      dev_finished.store(true);
+    } else {
+      CpuWorker w;
+      for (;;) {
+        uint64_t cur = __atomic_load_n(W, __ATOMIC_SEQ_CST);
+        uint64_t q = 0;
+        bool got = false;
+        for (;;) {
+          const uint64_t f = cur >> 32, b = cur & 0xFFFFFFFFull;
+          if (b <= f || b <= lo || dev_finished.load()) break;
+          q = b - 1;
+          // the device's work still ahead of q: from where its rate puts it
+          // now (at most its front) to q
+          const double el = std::chrono::duration<double>(Clock::now() - t0).count();
+          uint64_t from = f;
+          {
+            const double done = el * dev_guess;
+            uint64_t a = 0, z = f;  // largest position with work(0, pos) <= done, capped at f
+            while (a < z) {
+              const uint64_t mid = (a + z + 1) / 2;
+              if (work(0, mid) <= done) a = mid;
+              else z = mid - 1;
+            }
+            from = std::min(a, q);
+          }
+          if (work(q, q + 1) / host_rate() > kHostShare * work(from, q) / dev_guess) break;
+          if (__atomic_compare_exchange_n(W, &cur, pack(f, q), false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+            got = true;
+            break;
+          }
+        }
+        if (!got) break;
+        const Clock::time_point ts = Clock::now();
+        cpu_counts[q] = count_task_cpu(L, tasks[q], w);
+        add_to(host_secs, std::chrono::duration<double>(Clock::now() - ts).count());
+        add_to(host_work, work(q, q + 1));
+      }
+    }
+  }
+  if (err) std::rethrow_exception(err);
+  const uint64_t fin = __atomic_load_n(W, __ATOMIC_SEQ_CST);
+  const uint64_t front = fin >> 32, back = fin & 0xFFFFFFFFull;
+  if (dev_secs.load() > 1e-4) last_device_rate.store(work(0, front) / dev_secs.load());
+  if (host_secs.load() > 0.01) last_host_rate.store(host_work.load() / host_secs.load());
+  std::vector<uint64_t> gpu_counts(n, 0);
+  dev::plan_copy_counts(plan, gpu_counts.data());
+  HybridResult r;
+  // tasks [0, back) are the device's (those in [front, back) have no items),
+  // [back, n) the host workers'
+  r.gpu_tasks = back;
+  for (uint64_t q = 0; q < n; ++q) {
+    const uint64_t c = q < back ? gpu_counts[q] : cpu_counts[q];
+    if (per_task) per_task[q] = c;
+    r.total += c;
+  }
+  return r;
+}
+
 HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bbtc_task* tasks,
                         uint64_t n, uint64_t sep, int nthreads, int nstreams, bbtc_kernel kernel,
                         const uint8_t* choice, uint64_t* per_task) {
@@ -97,6 +206,10 @@ HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bb
   auto work = [&](uint64_t a, uint64_t b) { return pre[b] - pre[a]; };
 
   std::vector<uint64_t> cpu_counts(n, 0);
+  if (T > 1 && sep < n && dev::host_atomics(ctx)) {
+    const uint64_t lo = std::max<uint64_t>(sep, dev::plan_contested_from(plan));
+    if (lo < n) return run_shared(ctx, plan, L, tasks, n, lo, T, work, cpu_counts, per_task);
+  }
   std::atomic<uint64_t> state(pack(0, n));
   const int S = std::max(1, std::min(nstreams, dev::max_slots()));
   const uint64_t target = std::max<uint64_t>(1, dev::plan_batch_items(plan));

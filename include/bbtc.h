@@ -444,12 +444,18 @@ void bbtc_batch_destroy(bbtc_batch* b);
 
 /* Replaces BBTc::NoOfTriangles of the GPU build with its --border semantics
  * (bbtc.cu:54-62; bb_tc.cpp:142-252): tasks [0, separator) run on the device
- * only; tasks [separator, n) are shared, the device taking batches from the
- * front (one persistent launch per batch, up to nstreams batches in flight)
- * while nthreads-1 host workers take single tasks from the back. ctx and g
- * NULL: every task on the host (--ngpu 0). choice (may be NULL) gives
- * per-task device forms as in bbtc_plan_create_tuned. gpu_tasks_out (may be
- * NULL) receives how many leading tasks the device counted. */
+ * only; tasks [separator, n) are shared, the device taking them from the
+ * front while nthreads-1 host workers take single tasks from the back, each
+ * only while it would finish before the device reaches it at the measured
+ * rates. Where the device's atomics reach host memory, both sides claim from
+ * one word in host-coherent memory: one persistent launch counts the whole
+ * list and its waves move the front with system-scope atomics as they reach
+ * the shared tasks (those of the launch's last part). Otherwise a host thread
+ * claims device batches from the front on the device's behalf (up to
+ * nstreams launches in flight). ctx and g NULL: every task on the host
+ * (--ngpu 0). choice (may be NULL) gives per-task device forms as in
+ * bbtc_plan_create_tuned. gpu_tasks_out (may be NULL) receives how many
+ * leading tasks the device counted. */
 bbtc_status bbtc_count_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_layout* L,
                               const bbtc_task* tasks, uint64_t n, uint64_t separator, int nthreads,
                               int nstreams, bbtc_kernel kernel, const uint8_t* choice,
