@@ -115,6 +115,29 @@ def test_heavy_items_split(dev, part, p):
         assert tot == rtot and np.array_equal(per, rper), form
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_lane_slots_wide_bands(dev, p):
+    # lane-only plans read short bc(v) rows from 16-byte row slots; bands of
+    # 2^20 + 8, 2^19 + 5 and about 2^18.4 columns, rows of at most 8 entries
+    n = (1 << 20) + 7
+    g = B.Graph.gnm(n, 3 << 20, 77)
+    cuts = g.partition("uni", p)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    log("case lane slots", p, "widest band", int(np.max(np.diff(cuts.astype(np.int64)))))
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    for form in ("lane", "auto"):
+        pl = dg.plan(t, form)
+        assert (pl.kinds() == 5).all(), form  # every task in the lane form
+        pl.launch()
+        per, tot = pl.results()
+        pl.destroy()
+        assert tot == rtot and np.array_equal(per, rper), form
+    dg.release()
+
+
 def test_complete_graph_rows_wider_than_lds(dev):
     # K_2100: out-degrees up to 2099 exceed the 1792-word LDS slice, so the
     # sorted-list forms search those rows in HBM
