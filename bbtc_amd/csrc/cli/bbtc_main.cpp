@@ -82,8 +82,9 @@ void usage() {
             << "  --border <int>        Work queue border for GPUs                 (0)\n"
             << "                          Note: if set to -1, GPUs process all.\n"
             << "  --copy-first          If set then copy graph to GPUs first       (false)\n"
-            << "  --device-layout       Build the block layout on the GPU from the CSR; it\n"
-            << "                          stays resident for every repeat (no tile copies)\n"
+            << "  --device-layout       Build the block layout on the GPU from the CSR (timed\n"
+            << "                          with partitioning); it stays resident for every\n"
+            << "                          repeat, so no tile copies follow\n"
             << "  --repeat <int>        Run bbTC algorithm '--repeat' times        (5)\n"
             << "  --kernel <str>        auto | merge | bitmap | dense | rdense | lane | tuned (auto)\n"
             << "  --threads <int>       Host worker threads                        (all)\n"
@@ -622,6 +623,13 @@ int main(int argc, char** argv) {
     bbtc_hgraph_info(g, &n, &ncols, &m);
     std::cerr << "# Graph read.. N: " << n << ", NNZ: " << m << std::endl;
 
+    if (P.device_layout) {
+      // the layout is built on the device: its context comes first
+      F.begin("HIP Initial Memory Allocations and Stream Creation");
+      st = bbtc_ctx_create(0, nullptr, &ctx);
+      if (st) die(st, "creating the device context");
+      F.end();
+    }
     F.begin("Partitioning and Layout construction (s)");
     std::vector<uint32_t> cuts(p + 1);
     if (!P.cuts_file.empty()) {
@@ -636,17 +644,12 @@ int main(int argc, char** argv) {
       if (st) die(st, "partitioning");
     }
     if (P.device_layout) {
-      F.end();
-      F.begin("HIP Initial Memory Allocations and Stream Creation");
-      st = bbtc_ctx_create(0, nullptr, &ctx);
-      if (st) die(st, "creating the device context");
-      F.end();
-      F.begin("Layout construction on the device (s)");
       // the host arrays only where host code reads them: the host tail of
       // the task queue and the .bcsr writer
       const bool host = separator_of(bbtc_task_count(p)) < bbtc_task_count(p) || !P.save_bcsr.empty();
       st = bbtc_graph_build(ctx, g, cuts.data(), p, host ? BBTC_BUILD_HOST_ARRAYS : 0u, &L, &dg);
       if (st) die(st, "building the block layout on the device");
+      std::cerr << "# Layout built on the device" << (host ? " (host arrays copied back)" : "") << std::endl;
     } else {
       st = bbtc_layout_build(g, cuts.data(), p, &L);
       if (st) die(st, "building the block layout");

@@ -244,7 +244,8 @@ def test_cli_gpu(tmp_path):
             assert r.returncode == 0, r.stderr
             assert f"Number of Triangles: {run['total']}" in r.stdout
             assert "Number of GPUs: 1" in r.stdout
-            assert ("Layout construction on the device" in r.stdout) == ("--device-layout" in extra)
+            assert ("# Layout built on the device" in r.stderr) == ("--device-layout" in extra)
+            assert "Partitioning and Layout construction (s)" in r.stdout
             got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
             assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
 
