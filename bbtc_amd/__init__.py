@@ -27,7 +27,7 @@ CLI_PATH = os.path.join(_HERE, "bbtc")
 
 PART = {"uni": 0, "n1d": 1, "pal": 2, "rac": 3}
 ORDER = {"lex": 0, "desc": 1, "asc": 2}
-KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4, "lane": 5}
+KERNEL = {"auto": 0, "merge": 1, "bitmap": 2, "dense": 3, "rdense": 4, "lane": 5, "rows": 6}
 BUILD_HOST_ARRAYS = 1  # bbtc_graph_build: copy the layout's arrays back to the host
 
 

@@ -13,7 +13,7 @@
 //   * argument errors exit with status 2 (the reference exits 0);
 //   * "Cuda Initial Memory Allocations and Stream Creation" is reported as
 //     "HIP Initial Memory Allocations and Stream Creation";
-//   * extra flags: --kernel auto|merge|bitmap|dense|rdense|lane|tuned, --threads n, --cuts-file path,
+//   * extra flags: --kernel auto|merge|bitmap|dense|rdense|lane|rows|tuned, --threads n, --cuts-file path,
 //     --per-task-out path, --keep-duplicates, --save-bcsr path (a .bcsr
 //     --graph is read back with its own cuts), --forms-out / --forms-in path
 //     (the autotuned per-task forms saved for, and reused by, later runs);
@@ -86,7 +86,7 @@ void usage() {
             << "                          with partitioning); it stays resident for every\n"
             << "                          repeat, so no tile copies follow\n"
             << "  --repeat <int>        Run bbTC algorithm '--repeat' times        (5)\n"
-            << "  --kernel <str>        auto | merge | bitmap | dense | rdense | lane | tuned (auto)\n"
+            << "  --kernel <str>        auto | merge | bitmap | dense | rdense | lane | rows | tuned (auto)\n"
             << "  --threads <int>       Host worker threads                        (all)\n"
             << "  --cuts-file <path>    Read the p+1 cuts from a text file instead of partitioning\n"
             << "  --per-task-out <path> Write 'i j k count' for every task\n"
@@ -174,7 +174,7 @@ Params parse(int argc, char** argv) {
     } else if (!strcasecmp(a, "--kernel")) {
       p.kernel = val("--kernel");
       if (p.kernel != "auto" && p.kernel != "merge" && p.kernel != "bitmap" && p.kernel != "dense" &&
-          p.kernel != "rdense" && p.kernel != "lane" && p.kernel != "tuned")
+          p.kernel != "rdense" && p.kernel != "lane" && p.kernel != "rows" && p.kernel != "tuned")
         bad("Wrong kernel.");
     } else if (!strcasecmp(a, "--threads")) {
       p.threads = to_int(val("--threads"), "--threads");
@@ -334,6 +334,7 @@ bbtc_kernel kernel_kind(const std::string& s) {
          : s == "dense"  ? BBTC_KERNEL_DENSE
          : s == "rdense" ? BBTC_KERNEL_RDENSE
          : s == "lane"   ? BBTC_KERNEL_LANE
+         : s == "rows"   ? BBTC_KERNEL_ROWS
                          : BBTC_KERNEL_AUTO;
 }
 

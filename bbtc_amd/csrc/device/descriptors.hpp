@@ -58,6 +58,7 @@ constexpr uint32_t kKindBitmap = 1;
 constexpr uint32_t kKindDense = 2;
 constexpr uint32_t kKindRdense = 3;
 constexpr uint32_t kKindLane = 4;
+constexpr uint32_t kKindRows = 5;  // reported per task only: the row form counts a whole plan
 // A per-task choice byte (bbtc_plan_create_tuned): the form in the low bits
 // (0 the kernel's rule, 1-5 a form), and in bits 4-5 the rows per work item
 // as a right shift of the form's default (64, or as many bitmaps as fit).
@@ -189,6 +190,55 @@ static_assert(sizeof(SegDesc) == 112, "SegDesc layout");
 constexpr uint32_t kSlotEntries = 3;
 // Row slots are built for a graph's lane-form plans only up to this size.
 constexpr uint64_t kSlotMaxBytes = 8ull << 30;
+
+// The row form (a single graph whose tasks all admit the lane form): every
+// global row v gets a slot of S words (S a power of two, 4..kRowSlotMax)
+// holding N+(v) over all bands, global ids ascending, padded with kRowPad,
+// and its length in a separate array. Each row u is enumerated once: a
+// wedge (u, v), v in N+(u), merges the part of u's slot after v with v's
+// slot, and each common entry w adds one to task (band(u), band(v),
+// band(w)). So N+(v) is read once per wedge for every k, instead of once
+// per bc tile, and N+(u) once per row instead of once per band pair.
+constexpr uint32_t kRowSlotMax = 64;
+constexpr uint32_t kRowPad = 0xFFFFFFFFu;
+constexpr uint64_t kRowTableMaxBytes = 8ull << 30;
+constexpr uint32_t kRowMaxCuts = 256;  // band starts are kept in LDS
+constexpr uint32_t kRowItemBatch = 1;  // items (64 rows each) a wave claims per atomic
+
+// Index of task (i, j, k), i <= j <= k < P, in lexicographic order:
+// sum over i' < i of (P - i')(P - i' + 1)/2, then sum over j' in [i, j) of
+// P - j', then k - j.
+constexpr uint64_t lex_task(uint32_t P, uint32_t i, uint32_t j, uint32_t k) {
+  const uint64_t p = P, a = uint64_t(P) - i, d = uint64_t(j) - i;
+  return (p * (p + 1) * (p + 2) - a * (a + 1) * (a + 2)) / 6 + d * p - (uint64_t(i) * 2 + d - 1) * d / 2 +
+         (uint64_t(k) - j);
+}
+
+// lex_task against the enumeration itself (i, then j >= i, then k >= j).
+constexpr bool lex_task_enumerates(uint32_t P) {
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < P; ++i)
+    for (uint32_t j = i; j < P; ++j)
+      for (uint32_t k = j; k < P; ++k)
+        if (lex_task(P, i, j, k) != n++) return false;
+  return true;
+}
+static_assert(lex_task_enumerates(1) && lex_task_enumerates(2) && lex_task_enumerates(7) &&
+                  lex_task_enumerates(32),
+              "lex_task order");
+
+struct RowArgs {
+  const uint32_t* rows;         // row table: S words per global row
+  const uint32_t* rowlen;       // entries per global row
+  const uint32_t* cuts;         // P + 1 band starts
+  const uint32_t* lex2plan;     // lexicographic task -> plan task (~0: not in the plan)
+  unsigned long long* per_task;
+  SlotCtl* ctl;                 // claim heads, set by reset_slot
+  uint32_t nrows;
+  uint32_t P;
+  uint32_t s_shift;             // log2(S)
+  uint32_t pad;
+};
 
 struct CountArgs {
   const SegDesc* __restrict__ segs;  // launch0 increasing

@@ -66,7 +66,7 @@ void* scratch_of_device(int device) {
 
 bool valid_kernel(bbtc_kernel k) {
   return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
-         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE || k == BBTC_KERNEL_LANE;
+         k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE || k == BBTC_KERNEL_LANE || k == BBTC_KERNEL_ROWS;
 }
 
 std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n) {
@@ -181,6 +181,8 @@ PlanDesc describe_plan(const std::vector<TileDesc>& tiles, const std::vector<flo
                        const std::vector<TaskRef>& refs, bbtc_kernel kernel, const uint8_t* choice,
                        bool allow_auto_lanes, const std::vector<TaskWork>* work) {
   const uint64_t n = refs.size();
+  // per task, the row form is the lane form (a graph-wide launch takes it)
+  if (kernel == BBTC_KERNEL_ROWS) kernel = BBTC_KERNEL_AUTO;
   PlanDesc P;
   P.td.assign(std::max<uint64_t>(n, 1), TaskDesc{});
   P.pref.assign(n + 1, 0);
@@ -277,6 +279,8 @@ void release_graph(bbtc_dgraph* g) {
   dev_free(g->ctx, g->dense);
   dev_free(g->ctx, g->tiles);
   dev_free(g->ctx, g->slots);
+  dev_free(g->ctx, g->rows);
+  dev_free(g->ctx, g->rowlen);
   if (g->ctx) g->ctx->in_use -= std::min(g->ctx->in_use, g->bytes);
   delete g;
 }
@@ -293,6 +297,8 @@ void release_plan(bbtc_plan* p) {
     dev_free(p->ctx, sp.base);
   }
   dev_free(p->ctx, p->cycles);
+  dev_free(p->ctx, p->lex2plan);
+  dev_free(p->ctx, p->cuts);
   if (p->ctx) p->ctx->in_use -= std::min(p->ctx->in_use, p->bytes);
   delete p;
 }
@@ -329,9 +335,34 @@ void launch_range(bbtc_plan* p, uint64_t t0, uint64_t t1, int slot, hipStream_t 
   enqueue_count(p->ctx, a, i0, i1 - i0, p->lane_only, p->chunk, s);
 }
 
+// The row form's launch over every task of the plan (slot 0).
+void launch_rows(bbtc_plan* p, hipStream_t s) {
+  p->last_stream = s;
+  const bbtc_dgraph* g = p->g;
+  RowArgs a{};
+  a.rows = g->rows;
+  a.rowlen = g->rowlen;
+  a.cuts = p->cuts;
+  a.lex2plan = p->lex2plan;
+  a.per_task = p->per_task;
+  a.ctl = p->ctl;
+  a.nrows = g->cuts[g->P];
+  a.P = g->P;
+  a.s_shift = uint32_t(__builtin_ctz(g->rows_S));
+  const uint64_t items = (uint64_t(a.nrows) + kWave - 1) / kWave;
+  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, 0ull, (unsigned long long)items, 0ull);
+  HIP_OK(hipGetLastError());
+  const uint64_t want = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+  const unsigned blocks =
+      unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(p->ctx, true))));
+  hipLaunchKernelGGL(count_rows, dim3(blocks), dim3(kBlock), 0, s, a);
+  HIP_OK(hipGetLastError());
+}
+
 void launch(bbtc_plan* p, hipStream_t s) {
   zero(p, s);
-  launch_range(p, 0, p->ntasks, 0, s);
+  if (p->rows_form && !p->profiling) launch_rows(p, s);
+  else launch_range(p, 0, p->ntasks, 0, s);
 }
 
 }  // namespace
@@ -383,6 +414,96 @@ void build_graph_slots(bbtc_ctx* ctx, bbtc_dgraph* g) {
   g->bytes += acct - job_bytes;
 }
 
+// The row table of a single resident graph (the row form), built from the
+// resident tiles once: the longest row sets the slot size; none when a row
+// holds more than kRowSlotMax entries, the table would exceed
+// kRowTableMaxBytes or the budget, or the graph has more than kRowMaxCuts
+// bands.
+void build_graph_rows(bbtc_ctx* ctx, bbtc_dgraph* g) {
+  g->rows_tried = true;
+  const uint32_t P = g->P;
+  if (P == 0 || P > kRowMaxCuts) return;
+  g->cuts.assign(size_t(P) + 1, 0);
+  for (uint32_t b = 0; b < P; ++b) {
+    const uint64_t e = uint64_t(g->cuts[b]) + g->htiles[size_t(b) * P + b].h;
+    if (e >= 0xFFFFFFFFull) return;  // ids must stay below kRowPad
+    g->cuts[b + 1] = uint32_t(e);
+  }
+  const uint64_t nrows = g->cuts[P];
+  if (nrows == 0) return;
+  uint64_t acct = 0;
+  uint32_t* cuts = nullptr;
+  uint32_t* maxlen = nullptr;
+  uint32_t* rows = nullptr;
+  uint32_t* rowlen = nullptr;
+  try {
+    cuts = dev_alloc<uint32_t>(ctx, P + 1, &acct);
+    maxlen = dev_alloc<uint32_t>(ctx, 1, &acct);
+    HIP_OK(hipMemcpyAsync(cuts, g->cuts.data(), (P + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t zero = 0;
+    HIP_OK(hipMemcpyAsync(maxlen, &zero, 4, hipMemcpyHostToDevice, ctx->stream));
+    const unsigned blocks = unsigned((nrows + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(row_lengths, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->rowptr, g->tiles, cuts, P, maxlen);
+    HIP_OK(hipGetLastError());
+    uint32_t longest = 0;
+    HIP_OK(hipMemcpyAsync(&longest, maxlen, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    uint32_t S = 4;
+    while (S < longest) S <<= 1;
+    const uint64_t bytes = nrows * S * 4;
+    if (longest <= kRowSlotMax && bytes <= kRowTableMaxBytes && !(ctx->budget && ctx->in_use + bytes > ctx->budget)) {
+      uint64_t racct = 0;
+      rows = dev_alloc<uint32_t>(ctx, nrows * S, &racct);
+      rowlen = dev_alloc<uint32_t>(ctx, nrows, &racct);
+      hipLaunchKernelGGL(build_rows, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->rowptr, g->cols, g->tiles, cuts,
+                         P, uint32_t(__builtin_ctz(S)), reinterpret_cast<u32x4*>(rows), rowlen);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipStreamSynchronize(ctx->stream));
+      g->rows = rows;
+      g->rowlen = rowlen;
+      g->rows_S = S;
+      g->bytes += racct;
+      rows = nullptr;
+      rowlen = nullptr;
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, rows);
+    dev_free(ctx, rowlen);
+    dev_free(ctx, cuts);
+    dev_free(ctx, maxlen);
+    ctx->in_use -= std::min(ctx->in_use, acct);
+    throw;
+  }
+  dev_free(ctx, cuts);
+  dev_free(ctx, maxlen);
+  ctx->in_use -= std::min(ctx->in_use, acct);
+}
+
+// Sets up the row form for plan p over refs (a lane-only plan of a graph
+// with a row table): the map from lexicographic task index to plan task.
+// Leaves p unchanged when the plan lists a task twice.
+void plan_rows(bbtc_ctx* ctx, bbtc_plan* p, const std::vector<TaskRef>& refs) {
+  const bbtc_dgraph* g = p->g;
+  const uint32_t P = g->P;
+  const uint64_t ntask = lex_task(P, P - 1, P - 1, P - 1) + 1;
+  if (ntask >= 0xFFFFFFFFull) return;
+  std::vector<uint32_t> lex2plan(ntask, ~0u);
+  for (size_t q = 0; q < refs.size(); ++q) {
+    const uint32_t i = refs[q].ab / P, j = refs[q].ab % P, k = refs[q].bc % P;
+    uint32_t& slot = lex2plan[lex_task(P, i, j, k)];
+    if (slot != ~0u) return;  // listed twice
+    slot = uint32_t(q);
+  }
+  p->lex2plan = dev_alloc<uint32_t>(ctx, lex2plan.size(), &p->bytes);
+  p->cuts = dev_alloc<uint32_t>(ctx, P + 1, &p->bytes);
+  HIP_OK(hipMemcpyAsync(p->lex2plan, lex2plan.data(), lex2plan.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(hipMemcpyAsync(p->cuts, g->cuts.data(), (P + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+  p->rows_form = true;
+  for (uint64_t q = 0; q < p->ntasks; ++q)
+    if (p->hprefix[q + 1] > p->hprefix[q]) p->hkind[q] = kKindRows + 1;
+}
+
 // Work statistics of the tasks of a single resident graph, from the table
 // gathered at upload (empty when it has none).
 std::vector<TaskWork> graph_work(const bbtc_dgraph* g, const std::vector<TaskRef>& refs) {
@@ -414,6 +535,9 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
       if (!own.empty()) work = &own;
     }
     PlanDesc D = describe_plan(g->htiles, g->deg, refs, kernel, choice, true, work);
+    const bool want_rows = (kernel == BBTC_KERNEL_AUTO || kernel == BBTC_KERNEL_ROWS) && !choice && D.lane_only &&
+                           n > 0 && g->P > 0;
+    if (want_rows && !g->rows_tried) build_graph_rows(ctx, g);
     if (D.lane_only && g->P > 0 && !g->slots) build_graph_slots(ctx, g);
     std::vector<TaskDesc>& td = D.td;
     std::vector<uint64_t>& pref = D.pref;
@@ -453,6 +577,7 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
     HIP_OK(hipMemcpyAsync(p->tasks, td.data(), td.size() * sizeof(TaskDesc), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(p->item_prefix, pref.data(), pref.size() * 8, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(p->segs, sd, sizeof(sd), hipMemcpyHostToDevice, ctx->stream));
+    if (want_rows && g->rows) plan_rows(ctx, p, refs);
     HIP_OK(hipStreamSynchronize(ctx->stream));
   } catch (...) {
     release_plan(p);

@@ -778,6 +778,209 @@ __global__ __launch_bounds__(kBlock) void build_slots(const uint32_t* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Row form. The row table holds N+(v) of every global row v in a slot of
+// S = 2^s_shift words (global ids ascending, then kRowPad), gathered from
+// the tiles (j, k), k >= j, of v's band j.
+// ---------------------------------------------------------------------------
+
+// Band of global row or column id x: the last b < P with cuts[b] <= x.
+__device__ __forceinline__ uint32_t band_of(const uint32_t* cuts, uint32_t P, uint32_t x) {
+  uint32_t lo = 0, hi = P;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cuts[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Last entry of a block of a row slot (its first entry is an entry).
+__device__ __forceinline__ uint32_t last_entry(const u32x4& b) {
+  return b.w != kRowPad ? b.w : b.z != kRowPad ? b.z : b.y != kRowPad ? b.y : b.x;
+}
+
+// Entries of each global row v < nrows over all its tiles; the longest goes
+// to *maxlen (atomic max).
+__global__ __launch_bounds__(kBlock) void row_lengths(const uint32_t* __restrict__ rowptr,
+                                                      const TileDesc* __restrict__ tiles,
+                                                      const uint32_t* __restrict__ cuts_g, uint32_t P,
+                                                      uint32_t* __restrict__ maxlen) {
+  __shared__ uint32_t cuts[kRowMaxCuts + 1];
+  for (uint32_t x = threadIdx.x; x <= P; x += kBlock) cuts[x] = cuts_g[x];
+  __syncthreads();
+  const uint32_t nrows = cuts[P];
+  const uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t len = 0;
+  if (v < nrows) {
+    const uint32_t j = band_of(cuts, P, uint32_t(v));
+    const uint32_t r = uint32_t(v) - cuts[j];
+    for (uint32_t k = j; k < P; ++k) {
+      const TileDesc T = gload(tiles + uint64_t(j) * P + k);
+      if (T.nzr) len += rowptr[T.rp + r + 1] - rowptr[T.rp + r];
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(len, off, kWave);
+    len = o > len ? o : len;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && len) atomicMax(maxlen, len);
+}
+
+// Writes the row table (S = 2^s_shift words per row, S >= every row's
+// length): one thread per global row, 16 bytes per store.
+__global__ __launch_bounds__(kBlock) void build_rows(const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ cols,
+                                                     const TileDesc* __restrict__ tiles,
+                                                     const uint32_t* __restrict__ cuts_g, uint32_t P,
+                                                     uint32_t s_shift, u32x4* __restrict__ rows,
+                                                     uint32_t* __restrict__ rowlen) {
+  __shared__ uint32_t cuts[kRowMaxCuts + 1];
+  for (uint32_t x = threadIdx.x; x <= P; x += kBlock) cuts[x] = cuts_g[x];
+  __syncthreads();
+  const uint32_t nrows = cuts[P];
+  const uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (v >= nrows) return;
+  const uint32_t j = band_of(cuts, P, uint32_t(v));
+  const uint32_t r = uint32_t(v) - cuts[j];
+  u32x4* out = rows + (v << (s_shift - 2));
+  const uint32_t nblk = 1u << (s_shift - 2);
+  uint32_t buf[4] = {kRowPad, kRowPad, kRowPad, kRowPad};
+  uint32_t fill = 0, blk = 0;
+  for (uint32_t k = j; k < P; ++k) {
+    const TileDesc T = gload(tiles + uint64_t(j) * P + k);
+    if (!T.nzr) continue;
+    const uint32_t b = rowptr[T.rp + r], e = rowptr[T.rp + r + 1];
+    for (uint32_t x = b; x < e; ++x) {
+      const uint32_t c = cols[T.col + x] + cuts[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (uint32_t(q) == fill) buf[q] = c;
+      if (++fill == 4) {
+        out[blk++] = u32x4{buf[0], buf[1], buf[2], buf[3]};
+        fill = 0;
+      }
+    }
+  }
+  if (fill) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (uint32_t(q) >= fill) buf[q] = kRowPad;
+    out[blk++] = u32x4{buf[0], buf[1], buf[2], buf[3]};
+  }
+  rowlen[v] = blk * 4 - (fill ? 4 - fill : 0);
+  for (; blk < nblk; ++blk) out[blk] = u32x4{kRowPad, kRowPad, kRowPad, kRowPad};
+}
+
+// The row form's count. An item is 64 consecutive global rows u, claimed
+// like the first phase of count_probe from the launch's eight parts
+// (contiguous eighths of the rows), kRowItemBatch items per atomic. The wave flattens the rows' entries into wedges (u, v)
+// with a prefix sum, so a lane takes one wedge: v is entry x of u's slot,
+// and the entries after it are merged with v's slot 16 bytes at a time (all
+// 16 pairs of the two current blocks compared, the block with the smaller
+// last entry moving on). Each common entry w adds one to task (band(u),
+// band(v), band(w)); a lane keeps one running count per band (w ascends, so
+// bands do not decrease) and adds it to the task's counter when the band
+// changes and at the end of the wedge.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void count_rows(RowArgs a) {
+  __shared__ uint32_t cuts[kRowMaxCuts + 1];
+  for (uint32_t x = threadIdx.x; x <= a.P; x += kBlock) cuts[x] = a.cuts[x];
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t nblk = 1u << (a.s_shift - 2);
+  const gptr<u32x4> rows = (gptr<u32x4>)a.rows;
+  const gptr<uint32_t> words = as_global(a.rows);
+
+  unsigned long long item = 0, batch_end = 0;
+  uint32_t part = blockIdx.x % kQueues, parts_left = kQueues;
+  const unsigned long long base = a.ctl->base, chunk = a.ctl->chunk;
+  const unsigned shift = unsigned(a.ctl->shift);
+  for (;;) {
+    if (item >= batch_end) {
+      unsigned long long old = 0, pe = 0;
+      while (parts_left) {  // wave-uniform loop; lane 0 does the atomic
+        pe = a.ctl->end[part];
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(&a.ctl->head[part * kCtlStride], (unsigned long long)kRowItemBatch);
+        old = uniform64(__shfl(got, 0, kWave));
+        if (old < pe) break;
+        part = part + 1 == kQueues ? 0 : part + 1;
+        --parts_left;
+      }
+      if (!parts_left) break;
+      item = part_item(base, part, old, chunk, shift);
+      batch_end = item + ((old + kRowItemBatch < pe ? old + kRowItemBatch : pe) - old);
+    }
+    const uint64_t u = (item++) * kWave + uint64_t(lane);
+    uint32_t d = 0, bu = 0;
+    if (u < a.nrows) {
+      d = as_global(a.rowlen)[u];
+      if (d) bu = band_of(cuts, a.P, uint32_t(u));
+    }
+    const uint32_t incl = wave_incl_scan(d, lane);
+    const uint32_t excl = incl - d;
+    const uint32_t W = __builtin_amdgcn_readlane(incl, kWave - 1);
+    for (uint32_t wb = 0; wb < W; wb += kWave) {
+      const uint32_t w = wb + lane;
+      const int q = owner_of(incl, w);
+      const uint32_t q_ex = __shfl(excl, q, kWave);
+      const uint32_t q_d = __shfl(d, q, kWave);
+      const uint32_t q_bu = __shfl(bu, q, kWave);
+      const uint64_t q_u = uint64_t(__shfl(uint32_t(u), q, kWave));
+      // v is entry x0 - 1 of u's slot; a v that is u's last entry has no w
+      // after it
+      const uint32_t x0 = w - q_ex + 1;
+      if (w >= W || x0 == q_d) continue;
+      const uint64_t xs = q_u << (a.s_shift - 2);
+      const uint32_t v = words[(q_u << a.s_shift) + x0 - 1];
+      const uint64_t ys = uint64_t(v) << (a.s_shift - 2);
+      u32x4 y = rows[ys];
+      uint32_t bx = x0 >> 2;
+      u32x4 x = rows[xs + bx];
+      if (y.x == kRowPad) continue;  // N+(v) is empty
+      const uint32_t bv = band_of(cuts, a.P, v);
+      const uint64_t lex_uv = lex_task(a.P, q_bu, bv, bv);
+      uint32_t by = 0, kc = 0, cc = 0;
+      for (;;) {
+        const uint32_t xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t c = xv[e];
+          if (c != kRowPad && (c == y.x || c == y.y || c == y.z || c == y.w)) {
+            const uint32_t k = band_of(cuts, a.P, c);
+            if (k != kc && cc) {
+              const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_uv + (kc - bv)] : ~0u;
+              if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
+              cc = 0;
+            }
+            kc = k;
+            ++cc;
+          }
+        }
+        // the block whose last entry is smaller moves on (both when equal); a
+        // block holding its list's end finishes the merge when it would move
+        const uint32_t lx = last_entry(x), ly = last_entry(y);
+        const bool na = lx <= ly, nb = ly <= lx;
+        if (na) {
+          if (x.w == kRowPad || ++bx == nblk) break;
+          x = rows[xs + bx];
+        }
+        if (nb) {
+          if (y.w == kRowPad || ++by == nblk) break;
+          y = rows[ys + by];
+        }
+      }
+      if (cc) {
+        // in an upper-triangular graph every common entry w > v >= cuts[bv],
+        // so kc >= bv
+        const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_uv + (kc - bv)] : ~0u;
+        if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
+      }
+    }
+  }
+}
+
 // Start of a batch launch: stamp the device wall clock and reset each
 // graph's outstanding item count (graphs without items finish at once).
 __global__ void batch_begin(const unsigned long long* __restrict__ left0, uint64_t ngraphs,

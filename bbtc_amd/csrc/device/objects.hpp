@@ -53,6 +53,14 @@ struct bbtc_dgraph {
   std::vector<uint64_t> lex_ij;
   uint64_t rowptr_words = 0;
   uint32_t* slots = nullptr;  // row slots of the lane form (4 words per rowptr word), built on first need
+  // Row table of the row form (rows_S words per global row), built on first
+  // need; rows_tried: the build was attempted (rows stays NULL when a row is
+  // longer than kRowSlotMax or the table would be too large).
+  uint32_t* rows = nullptr;
+  uint32_t* rowlen = nullptr;  // entries per global row
+  uint32_t rows_S = 0;
+  bool rows_tried = false;
+  std::vector<uint32_t> cuts;  // band starts (P + 1), from the band heights
 };
 
 struct bbtc_batch;
@@ -70,6 +78,12 @@ struct bbtc_plan {
   std::vector<uint64_t> hprefix;        // host copy of item_prefix
   std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense, 5 lane)
   bool lane_only = false;               // every task with items uses the lane form
+  // Row form (a lane-only plan of a graph with a row table): launch() counts
+  // every task in one count_rows launch over the graph's rows; task ranges,
+  // profiling and the shared queue keep the lane form.
+  bool rows_form = false;
+  uint32_t* lex2plan = nullptr;         // lexicographic task -> plan task
+  uint32_t* cuts = nullptr;             // the graph's band starts (P + 1)
   uint64_t chunk = 0;                   // first-phase chunk (0: one per part; batches: list order)
   unsigned long long* per_task = nullptr;
   bbtc::SlotCtl* ctl = nullptr;         // one control block per launch slot

@@ -206,7 +206,7 @@ def count_c2(B, dev, steps, cpu=True):
            "ms_per_count": mean_s * 1e3, "triangles": int(total), "tasks": int(len(tasks)),
            "algorithmic_bytes": int(total_bytes), "offset_bytes": 4,
            "roofline_frac": total_bytes / mean_s / 1e9 / HBM_PEAK_GBS,
-           "forms_merge_bitmap_dense_rdense_lane": np.bincount(kinds, minlength=6)[1:].tolist()}
+           "forms_merge_bitmap_dense_rdense_lane_rows": np.bincount(kinds, minlength=7)[1:].tolist()}
     if cpu:
         out["cpu_baseline"] = cpu_baseline(g, cuts, tasks, per, g.m, 0, full=True)
         log(f"c2 cpu baseline: {out['cpu_baseline']['seconds_per_count']} s per count")
@@ -313,7 +313,7 @@ def main():
     dev.synchronize()
     wall = time.perf_counter() - t0
     per, total = plan.results()
-    forms = np.bincount(plan.kinds(), minlength=6)[1:].tolist()
+    forms = np.bincount(plan.kinds(), minlength=7)[1:].tolist()
     ms_per_step = wall * 1e3 / args.steps
     ev_mean_s = float(np.mean(ms)) * 1e-3
     ev_median_ms = float(np.median(ms))
@@ -347,7 +347,7 @@ def main():
             "n": n, "m": m, "ncut": args.ncut, "tasks": int(len(tasks)),
             "partitioner": args.partitioner, "kernel": args.kernel,
             "triangles": int(total), "autotune_s": round(tune_s, 2),
-            "forms_merge_bitmap_dense_rdense_lane": forms,
+            "forms_merge_bitmap_dense_rdense_lane_rows": forms,
         },
         "end_to_end_s": {
             "generate": round(gen_s, 3), "partition": round(part_s, 3),

@@ -233,10 +233,11 @@ typedef struct bbtc_dgraph bbtc_dgraph;
 typedef struct bbtc_plan bbtc_plan;
 
 typedef enum bbtc_kernel {
-  BBTC_KERNEL_AUTO = 0,    /* without measurements: LANE for every task when every task with
-                              work admits it; otherwise per task RDENSE where the bc tile has a
-                              bitmap form, else BITMAP where a row of the k band fits the LDS
-                              slice, else MERGE (bbtc_autotune measures instead) */
+  BBTC_KERNEL_AUTO = 0,    /* per task the form of least estimated cost from the tasks' work
+                              statistics (a linear model of measured clocks); LANE for every task
+                              when every task with work admits it, and then ROWS for a single
+                              graph whose rows fit its row table (bbtc_autotune measures
+                              instead) */
   BBTC_KERNEL_MERGE = 1,   /* bc(v) entries binary-searched in ac(u) (LDS copy) */
   BBTC_KERNEL_BITMAP = 2,  /* bc(v) entries tested against an LDS bitmap of ac(u), where the
                               band fits the slice */
@@ -244,10 +245,15 @@ typedef enum bbtc_kernel {
                               bc tile has one, else as BITMAP */
   BBTC_KERNEL_RDENSE = 4,  /* ac(u) entries tested against the bitmap row of bc(v) where the
                               bc tile has one, else as MERGE */
-  BBTC_KERNEL_LANE = 5     /* one lane per row u merges bc(v) with ac(u) in HBM, the
+  BBTC_KERNEL_LANE = 5,    /* one lane per row u merges bc(v) with ac(u) in HBM, the
                               per-thread row merge of SparseKernel (gpu_sparse_kernel.cu:6-49),
                               where no row of the three tiles holds more than 32 entries,
                               else as MERGE */
+  BBTC_KERNEL_ROWS = 6     /* as AUTO, and a plan of one graph whose tasks all take the LANE
+                              form counts every task in one launch over whole rows: a wedge
+                              (u, v) merges N+(u) with N+(v) over all bands once (rows of at
+                              most 64 entries, kept in a row table built on first use) and
+                              each common w adds to task (band(u), band(v), band(w)) */
 } bbtc_kernel;
 
 typedef struct bbtc_opts {
@@ -355,7 +361,8 @@ bbtc_status bbtc_plan_profile(bbtc_plan* plan, uint64_t* cycles_per_task);
  * and the fixed 256-thread row blocks of bb_tc.cpp:96-105. */
 bbtc_status bbtc_autotune(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                           uint8_t* choice_out, uint64_t* cycles_out);
-/* Form the plan chose per task: 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane. */
+/* Form the plan chose per task: 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane, 6 rows (the
+ * plan's tasks with work are counted by the row form's launch). */
 bbtc_status bbtc_plan_task_kinds(const bbtc_plan* plan, uint8_t* kinds);
 /* Number of tasks the plan routed to an LDS bitmap form (bitmap or dense). */
 uint64_t bbtc_plan_bitmap_tasks(const bbtc_plan* plan);

@@ -21,7 +21,7 @@ from tests.test_count_cpu import CLOSED
 
 pytestmark = pytest.mark.gpu
 
-FORMS = ["merge", "bitmap", "dense", "rdense", "lane", "auto"]
+FORMS = ["merge", "bitmap", "dense", "rdense", "lane", "auto", "rows"]
 GOLDEN = load_golden()
 COUNTED = [(d, r) for d in GOLDEN for r in d["runs"] if "tasks" in r]
 
@@ -130,11 +130,88 @@ def test_lane_slots_wide_bands(dev, p):
     dg = dev.upload(L)
     for form in ("lane", "auto"):
         pl = dg.plan(t, form)
-        assert (pl.kinds() == 5).all(), form  # every task in the lane form
+        # every task in the lane form; AUTO then counts them by whole rows
+        assert (pl.kinds() == (5 if form == "lane" else 6)).all(), form
         pl.launch()
         per, tot = pl.results()
         pl.destroy()
         assert tot == rtot and np.array_equal(per, rper), form
+    dg.release()
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_rows_form_vs_oracle(dev, case):
+    # the row form: every task of a lane-only plan counted by one launch over
+    # whole rows (N+(u) merged with N+(v) across all bands), each common
+    # entry added to its own task; sparse G(n, m) and RMAT graphs with 1-24
+    # cuts (uni bands leave some tiles empty), per task against the oracle
+    rng = np.random.default_rng(900 + case)
+    if case % 4 == 3:
+        g = B.Graph.rmat(int(rng.integers(8, 13)), 2, seed=case)
+    else:
+        n = int(rng.integers(2000, 200000))
+        g = B.Graph.gnm(n, int(rng.integers(n, 12 * n)), 300 + case)
+    part = ["uni", "pal", "n1d"][case % 3]
+    p = int(rng.integers(1, 25))
+    cuts = g.partition(part, p)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    log("case rows", case, "n", g.n, "m", g.m, part, p)
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    pl = dg.plan(t, "rows")
+    kinds = pl.kinds()
+    for _ in range(2):  # relaunch: counters cleared each time
+        pl.launch()
+        per, tot = pl.results()
+        assert tot == rtot and np.array_equal(per, rper), kinds
+    if (kinds == 6).any():
+        # task ranges (the shared-queue and host-driven --border paths) keep
+        # the lane form
+        per_h, tot_h, _ = dg.count_hybrid(t, len(t) // 2, nthreads=2)
+        assert tot_h == rtot and np.array_equal(per_h, rper)
+    pl.destroy()
+    # a subset of the tasks, in another order: the others' triangles are not counted
+    sub = t[::-2].copy()
+    rsub, rstot = R.count(np.array(sub.tolist(), dtype=np.uint32))
+    per, tot = dg.count(sub, "rows")
+    assert tot == rstot and np.array_equal(per, rsub)
+    dg.release()
+
+
+def test_rows_form_declined(dev):
+    # rows of more than 64 entries (G(n, m) of average degree 160): no row
+    # table, the plan keeps the lane or other forms; a task listed twice: the
+    # row form is declined for that plan
+    g = B.Graph.gnm(4000, 320000, 5)
+    cuts = g.partition("uni", 8)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    pl = dg.plan(t, "rows")
+    assert not (pl.kinds() == 6).any()
+    pl.launch()
+    per, tot = pl.results()
+    pl.destroy()
+    assert tot == rtot and np.array_equal(per, rper)
+    dg.release()
+    g = B.Graph.gnm(30000, 60000, 6)
+    cuts = g.partition("pal", 4)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    t2 = np.concatenate([t, t[:3]])
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t2.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    pl = dg.plan(t2, "rows")
+    assert not (pl.kinds() == 6).any() and (pl.kinds() == 5).any()
+    pl.launch()
+    per, tot = pl.results()
+    pl.destroy()
+    assert tot == rtot and np.array_equal(per, rper)
     dg.release()
 
 

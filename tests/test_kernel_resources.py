@@ -1,7 +1,7 @@
 """Register budget of the counting kernels: count_probe must keep 4 waves per
 SIMD (at most 128 VGPRs: 1,024 resident blocks of 256 threads) and its
-lane-form instantiation 8, since both are latency-bound and their speed
-falls with occupancy; neither may spill. Read from the compiler's resource
+lane-form instantiation 8, as must the row form's count_rows, since all are
+latency-bound and their speed falls with occupancy; none may spill. Read from the compiler's resource
 remarks for gfx950, with the library's own flags; no GPU needed."""
 import os
 import re
@@ -33,5 +33,7 @@ def test_counting_kernels_keep_their_occupancy(tmp_path):
             report.setdefault(name, {})[m.group(1)] = int(m.group(2))
     probe = report["_ZN4bbtc11count_probeENS_9CountArgsE"]
     lanes = report["_ZN4bbtc17count_probe_lanesENS_9CountArgsE"]
+    rows = report["_ZN4bbtc10count_rowsENS_7RowArgsE"]
     assert probe["Occupancy [waves/SIMD]"] >= 4 and probe["VGPRs Spill"] == 0, probe
     assert lanes["Occupancy [waves/SIMD]"] >= 8 and lanes["VGPRs Spill"] == 0, lanes
+    assert rows["Occupancy [waves/SIMD]"] >= 8 and rows["VGPRs Spill"] == 0, rows
