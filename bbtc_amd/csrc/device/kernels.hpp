@@ -935,41 +935,70 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
       const uint64_t xs = q_u << (a.s_shift - 2);
       const uint32_t v = words[(q_u << a.s_shift) + x0 - 1];
       const uint64_t ys = uint64_t(v) << (a.s_shift - 2);
-      u32x4 y = rows[ys];
+      // N+(v) is read four blocks (64 bytes) at a time, all four loads in
+      // flight together: a slot's blocks share one cache line, which a later
+      // load of the same line could find evicted by the other lanes' lines.
+      // u's entries after v are read a block at a time (u's last block is
+      // known from its length).
+      const u32x4 pads = {kRowPad, kRowPad, kRowPad, kRowPad};
+      u32x4 Y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Y[q] = uint32_t(q) < nblk ? rows[ys + q] : pads;
       uint32_t bx = x0 >> 2;
+      const uint32_t ex = (q_d - 1) >> 2;
       u32x4 x = rows[xs + bx];
-      if (y.x == kRowPad) continue;  // N+(v) is empty
+      if (Y[0].x == kRowPad) continue;  // N+(v) is empty
       const uint32_t bv = band_of(cuts, a.P, v);
       const uint64_t lex_uv = lex_task(a.P, q_bu, bv, bv);
-      uint32_t by = 0, kc = 0, cc = 0;
-      for (;;) {
-        const uint32_t xv[4] = {x.x, x.y, x.z, x.w};
+      uint32_t kc = 0, cc = 0;
+      bool done = false;
+      for (uint32_t r = 0;;) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const uint32_t c = xv[e];
-          if (c != kRowPad && (c == y.x || c == y.y || c == y.z || c == y.w)) {
-            const uint32_t k = band_of(cuts, a.P, c);
-            if (k != kc && cc) {
-              const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_uv + (kc - bv)] : ~0u;
-              if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
-              cc = 0;
+        for (int q = 0; q < 4; ++q) {
+          const u32x4 y = Y[q];
+          if (done) break;
+          if (y.x == kRowPad) {
+            done = true;
+            break;
+          }
+          const uint32_t ly = last_entry(y);
+          for (;;) {
+            const uint32_t xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const uint32_t c = xv[e];
+              if (c != kRowPad && (c == y.x || c == y.y || c == y.z || c == y.w)) {
+                const uint32_t k = band_of(cuts, a.P, c);
+                if (k != kc && cc) {
+                  const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_uv + (kc - bv)] : ~0u;
+                  if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
+                  cc = 0;
+                }
+                kc = k;
+                ++cc;
+              }
             }
-            kc = k;
-            ++cc;
+            // the block whose last entry is smaller moves on (both when
+            // equal); a list whose last block moves on is done
+            const uint32_t lx = last_entry(x);
+            const bool na = lx <= ly, nb = ly <= lx;
+            if (na) {
+              if (bx == ex) {
+                done = true;
+                break;
+              }
+              x = rows[xs + ++bx];
+            }
+            if (nb) {
+              if (y.w == kRowPad) done = true;
+              break;
+            }
           }
         }
-        // the block whose last entry is smaller moves on (both when equal); a
-        // block holding its list's end finishes the merge when it would move
-        const uint32_t lx = last_entry(x), ly = last_entry(y);
-        const bool na = lx <= ly, nb = ly <= lx;
-        if (na) {
-          if (x.w == kRowPad || ++bx == nblk) break;
-          x = rows[xs + bx];
-        }
-        if (nb) {
-          if (y.w == kRowPad || ++by == nblk) break;
-          y = rows[ys + by];
-        }
+        r += 4;
+        if (done || r >= nblk) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Y[q] = r + q < nblk ? rows[ys + r + q] : pads;
       }
       if (cc) {
         // in an upper-triangular graph every common entry w > v >= cuts[bv],

@@ -25,9 +25,11 @@ Prints one JSON line:
   cpu_baseline  the CPU oracle (restated reference CPU path) on a
                 systematic 1/s sample of every task's rows, scaled by s
                 (--cpu-full: every task once, measured)
-  c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), AUTO forms:
-                edges/s and roofline fraction of the same count, and the
-                oracle over all of its tasks
+  c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), AUTO forms
+                (the row form: one count_rows launch over whole rows):
+                edges/s, roofline fraction by the tiled algorithmic bytes
+                and by the row form's own bytes, HBM traffic from the
+                committed counters, and the oracle over all of its tasks
   batched       4,096 uneven graphs (70% RMAT s10-18, 30% G(n,m)) resident,
                 one captured persistent launch vs the plain per-graph launch,
                 both with the AUTO forms (and the batch with autotuned forms
@@ -68,6 +70,22 @@ def measured_traffic(workload: str, forms: str):
     if t.get("workload") != workload or t.get("forms") != forms:
         return None, None
     return int(t["traffic_bytes"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+
+
+C2_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "count_rows_gnm-n22-m26_pal16_traffic.json")
+
+
+def measured_c2_traffic(kernel: str):
+    """HBM bytes per C2 count from the committed rocprofv3 counters of this
+    kernel (scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(C2_TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if t.get("kernel") != kernel:
+        return None, None
+    return int(t["traffic_bytes"]), os.path.relpath(C2_TRAFFIC_FILE, ROOT)
 
 
 CPU_FULL_FILE = os.path.join(ROOT, "profiles", "cpu_baseline_rmat-s26-ef16_pal32.json")
@@ -202,11 +220,28 @@ def count_c2(B, dev, steps, cpu=True):
     mean_s = float(np.mean(ms)) * 1e-3
     log(f"c2: G(n=2^22, m=2^26) pal 16: {total} triangles, events ms {np.round(ms, 3).tolist()} "
         f"(prepared in {prep:.1f}s)")
+    rows_form = bool((kinds == 6).any())
     out = {"workload": "gnm-n22-m26 pal ncut16", "metric": "edges/s", "value": g.m / mean_s,
            "ms_per_count": mean_s * 1e3, "triangles": int(total), "tasks": int(len(tasks)),
+           "kernel": "count_rows" if rows_form else "count_probe_lanes",
            "algorithmic_bytes": int(total_bytes), "offset_bytes": 4,
            "roofline_frac": total_bytes / mean_s / 1e9 / HBM_PEAK_GBS,
            "forms_merge_bitmap_dense_rdense_lane_rows": np.bincount(kinds, minlength=7)[1:].tolist()}
+    if rows_form:
+        # what the row form itself reads: each row's length and entries once,
+        # and N+(v) once per wedge (u, v) (4-byte ids; the tiled figure above
+        # reads bc(v) once per task)
+        off = g.offsets().astype(np.int64)
+        d = np.diff(off)
+        own = 4 * g.n + 4 * g.m + 4 * int(d[g.cols()].sum())
+        out["row_form_bytes"] = own
+        out["row_form_frac"] = own / mean_s / 1e9 / HBM_PEAK_GBS
+    traffic, src = measured_c2_traffic(out["kernel"])
+    if traffic:
+        out["hbm_traffic_bytes"] = traffic
+        out["hbm_frac"] = traffic / mean_s / 1e9 / HBM_PEAK_GBS
+        out["traffic_over_algorithmic"] = traffic / total_bytes
+        out["traffic_source"] = src
     if cpu:
         out["cpu_baseline"] = cpu_baseline(g, cuts, tasks, per, g.m, 0, full=True)
         log(f"c2 cpu baseline: {out['cpu_baseline']['seconds_per_count']} s per count")
