@@ -47,12 +47,19 @@
 
 namespace bbtc {
 
+// Inclusive prefix sum over the wave's 64 lanes with DPP moves (no LDS
+// round trips): shifts by 1, 2, 4 and 8 lanes within each row of 16 (a lane
+// whose source lies before its row adds 0), then lane 15 of rows 0 and 2
+// added to rows 1 and 3 (row_bcast:15), then lane 31 to rows 2 and 3
+// (row_bcast:31).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off, kWave);
-    if (lane >= off) x += y;
-  }
+  (void)lane;
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xf, 0xf, false));  // row_shr:1
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xf, 0xf, false));  // row_shr:2
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xf, 0xf, false));  // row_shr:4
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xf, 0xf, false));  // row_shr:8
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xc, 0xf, false));  // row_bcast:31
   return x;
 }
 
