@@ -159,6 +159,10 @@ __device__ __forceinline__ uint32_t intersect_from(gptr<uint32_t> A, uint32_t a,
   return n;
 }
 
+__device__ __forceinline__ uint32_t and_popc(u32x4 a, u32x4 b) {
+  return __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+}
+
 // Largest t with prefix[t] <= item (prefix[n] > item).
 __device__ __forceinline__ uint32_t find_task(gptr<uint64_t> prefix, uint32_t n,
                                               uint64_t item) {
@@ -505,13 +509,24 @@ __device__ __forceinline__ void count_units(const CountArgs& a, uint32_t* lds_al
             const uint32_t v = ab_c[q_ab0 + (w - q_ex)];
             const gptr<u32x4> brow = (gptr<u32x4>)(bc_d + uint64_t(v) * T.wpr);
             const u32x4* arow = reinterpret_cast<const u32x4*>(lds + q_seg);
+            // four 16-byte loads of the bitmap row in flight per lane (half a
+            // line: a line read a quarter at a time is often evicted by the
+            // other lanes' rows before its next quarter is read); the LDS
+            // words are read two at a time, so the registers of the four
+            // loads stay within the kernel's budget for 4 waves per SIMD
             uint32_t c = 0;
-#pragma unroll 2
-            for (uint32_t j = 0; j < T.wpr / 4; ++j) {
-              const u32x4 bv = brow[j];
-              const u32x4 av = arow[j];
-              c += __popc(av.x & bv.x) + __popc(av.y & bv.y) + __popc(av.z & bv.z) + __popc(av.w & bv.w);
+            const uint32_t n4 = T.wpr / 4;
+            uint32_t j = 0;
+            for (; j + 4 <= n4; j += 4) {
+              const u32x4 b0 = brow[j], b1 = brow[j + 1], b2 = brow[j + 2], b3 = brow[j + 3];
+              __builtin_amdgcn_sched_barrier(0);
+              c += and_popc(arow[j], b0);
+              c += and_popc(arow[j + 1], b1);
+              __builtin_amdgcn_sched_barrier(0);
+              c += and_popc(arow[j + 2], b2);
+              c += and_popc(arow[j + 3], b3);
             }
+            for (; j < n4; ++j) c += and_popc(arow[j], brow[j]);
             lcnt += c;
           }
           steps += 1 + T.wpr / 4;
