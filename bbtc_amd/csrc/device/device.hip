@@ -64,6 +64,61 @@ void* scratch_of_device(int device) {
   return scratch_addr[device];
 }
 
+// ---------------------------------------------------------------------------
+// Pinned staging of host-to-device copies.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kStageBytes = 64ull << 20;
+
+struct Staging {
+  std::mutex mu;
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool pending[2] = {false, false};  // a copy from the buffer may still be in flight
+};
+
+void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, uint64_t bytes) {
+  if (bytes < kStageMinBytes) {
+    if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!ctx->h2d) ctx->h2d = new Staging;
+  }
+  Staging& st = *ctx->h2d;
+  std::lock_guard<std::mutex> lk(st.mu);
+  for (int b = 0; b < 2; ++b) {
+    if (!st.buf[b]) HIP_OK(hipHostMalloc(&st.buf[b], kStageBytes, hipHostMallocDefault));
+    if (!st.ev[b]) HIP_OK(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+  }
+  const uint64_t nchunks = (bytes + kStageBytes - 1) / kStageBytes;
+  for (uint64_t c = 0; c < nchunks; ++c) {
+    const int b = int(c & 1);
+    if (st.pending[b]) {
+      // the buffer's previous chunk (this call's or an earlier call's) has left
+      HIP_OK(hipEventSynchronize(st.ev[b]));
+      st.pending[b] = false;
+    }
+    const uint64_t o = c * kStageBytes, len = std::min(kStageBytes, bytes - o);
+    parallel_copy(st.buf[b], static_cast<const char*>(src) + o, len);
+    HIP_OK(hipMemcpyAsync(static_cast<char*>(dst) + o, st.buf[b], len, hipMemcpyHostToDevice, s));
+    HIP_OK(hipEventRecord(st.ev[b], s));
+    st.pending[b] = true;
+  }
+}
+
+void staging_release(bbtc_ctx* ctx) {
+  if (!ctx->h2d) return;
+  Staging* st = ctx->h2d;
+  for (int b = 0; b < 2; ++b) {
+    if (st->pending[b]) (void)hipEventSynchronize(st->ev[b]);
+    if (st->ev[b]) (void)hipEventDestroy(st->ev[b]);
+    if (st->buf[b]) (void)hipHostFree(st->buf[b]);
+  }
+  delete st;
+  ctx->h2d = nullptr;
+}
+
 bool valid_kernel(bbtc_kernel k) {
   return k == BBTC_KERNEL_AUTO || k == BBTC_KERNEL_MERGE || k == BBTC_KERNEL_BITMAP ||
          k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE || k == BBTC_KERNEL_LANE || k == BBTC_KERNEL_ROWS;
@@ -867,6 +922,7 @@ void bbtc_ctx_destroy(bbtc_ctx* ctx) {
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
   }
+  bbtc::staging_release(ctx);
   delete ctx;
 }
 
@@ -956,9 +1012,9 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
     g->nzrows = bbtc::dev_alloc<uint32_t>(ctx, L.nzrows.size(), &g->bytes);
     g->dense = bbtc::dev_alloc<uint32_t>(ctx, L.dense_words, &g->bytes);
     g->tiles = bbtc::dev_alloc<bbtc::TileDesc>(ctx, T, &g->bytes);
-    HIP_OK(hipMemcpyAsync(g->rowptr, L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipMemcpyAsync(g->cols, L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipMemcpyAsync(g->nzrows, L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    bbtc::copy_to_device(ctx, ctx->stream, g->rowptr, L.rowptr.data(), L.rowptr.size() * 4);
+    bbtc::copy_to_device(ctx, ctx->stream, g->cols, L.cols.data(), L.cols.size() * 4);
+    bbtc::copy_to_device(ctx, ctx->stream, g->nzrows, L.nzrows.data(), L.nzrows.size() * 4);
     HIP_OK(hipMemcpyAsync(g->tiles, g->htiles.data(), T * sizeof(bbtc::TileDesc), hipMemcpyHostToDevice, ctx->stream));
     // bitmap rows of the dense tiles, built from the uploaded tiles
     uint64_t rows = 0, acct = 0;
@@ -1389,9 +1445,9 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
           A->deg[tile0[gi] + t] = L.deg[t];
         }
       hipStream_t s = ctx->stream;
-      HIP_OK(hipMemcpyAsync(A->rowptr + rp0[gi], L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(A->cols + col0[gi], L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
+      bbtc::copy_to_device(ctx, s, A->rowptr + rp0[gi], L.rowptr.data(), L.rowptr.size() * 4);
+      bbtc::copy_to_device(ctx, s, A->cols + col0[gi], L.cols.data(), L.cols.size() * 4);
+      bbtc::copy_to_device(ctx, s, A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4);
       uint64_t rows = 0;
       const std::vector<bbtc::DenseJob> gj = bbtc::dense_jobs(L, rp0[gi], col0[gi], dn0[gi], job_rows, &rows);
       jobs.insert(jobs.end(), gj.begin(), gj.end());

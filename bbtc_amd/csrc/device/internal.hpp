@@ -87,6 +87,17 @@ inline uint64_t resident_blocks(const bbtc_ctx* ctx, bool lane_only = false) {
 void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t items, bool lane_only,
                    uint64_t chunk, hipStream_t s, uint32_t defer_steps = 0);
 
+// Host to device copy on stream s. Copies of at least kStageMinBytes go
+// through the context's two pinned staging buffers: the host threads fill
+// one buffer while the other's chunk is in flight over PCIe (pinned copies
+// run at about twice the rate of pageable ones on MI355X hosts), so src may
+// be reused on return; smaller copies are plain hipMemcpyAsync calls. Calls
+// on one context are serialised on its staging buffers.
+constexpr uint64_t kStageMinBytes = 8ull << 20;
+void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, uint64_t bytes);
+// Frees the context's staging buffers (after its streams are idle).
+void staging_release(bbtc_ctx* ctx);
+
 // Enqueues p[0..n) = 0 on s.
 void enqueue_zero(unsigned long long* p, uint64_t n, hipStream_t s);
 

@@ -435,8 +435,8 @@ void build_on_device(bbtc_ctx* ctx, const Csr& g, const std::vector<uint32_t>& c
       dg->rowptr = dev_alloc<uint32_t>(ctx, L.rp_words, &dg->bytes);
       dg->rowptr_words = L.rp_words;
       dg->tiles = dev_alloc<TileDesc>(ctx, T, &dg->bytes);
-      HIP_OK(hipMemcpyAsync(off, g.off.data(), (uint64_t(g.n) + 1) * 8, hipMemcpyHostToDevice, s));
-      if (g.m) HIP_OK(hipMemcpyAsync(col, g.col.data(), g.m * 4, hipMemcpyHostToDevice, s));
+      copy_to_device(ctx, s, off, g.off.data(), (uint64_t(g.n) + 1) * 8);
+      copy_to_device(ctx, s, col, g.col.data(), g.m * 4);
       HIP_OK(hipMemcpyAsync(cuts_d, cuts.data(), (P + 1) * 4, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(ch, chunks.data(), nch * sizeof(ScanChunk), hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(first_d, first.data(), (ntiles + 1) * 8, hipMemcpyHostToDevice, s));

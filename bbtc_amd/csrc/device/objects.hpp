@@ -21,6 +21,10 @@
 // ---------------------------------------------------------------------------
 // host objects
 // ---------------------------------------------------------------------------
+namespace bbtc {
+struct Staging;  // pinned host-to-device staging buffers (device.hip)
+}
+
 struct bbtc_ctx {
   int device = 0;
   int num_cu = 0;
@@ -32,6 +36,7 @@ struct bbtc_ctx {
   uint64_t in_use = 0;
   unsigned char* scratch = nullptr;  // this device's address of the scratch array (bbtc_count_noalloc)
   int host_atomics = -1;             // device atomics on host memory supported (-1: not asked yet)
+  bbtc::Staging* h2d = nullptr;      // created on the first large upload
 };
 
 struct bbtc_dgraph {

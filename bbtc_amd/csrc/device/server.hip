@@ -319,9 +319,9 @@ void admit_loop(bbtc_server* S) {
       const Layout& L = *r->L;
       char* b = S->heap_base + r->heap_off;
       hipStream_t s = S->copy;
-      HIP_OK(hipMemcpyAsync(b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(b + r->o_col, L.cols.data(), L.cols.size() * 4, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(b + r->o_nz, L.nzrows.data(), L.nzrows.size() * 4, hipMemcpyHostToDevice, s));
+      copy_to_device(S->ctx, s, b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4);
+      copy_to_device(S->ctx, s, b + r->o_col, L.cols.data(), L.cols.size() * 4);
+      copy_to_device(S->ctx, s, b + r->o_nz, L.nzrows.data(), L.nzrows.size() * 4);
 
       HIP_OK(hipMemcpyAsync(b + r->o_tiles, r->tiles.data(), r->tiles.size() * sizeof(TileDesc),
                             hipMemcpyHostToDevice, s));
