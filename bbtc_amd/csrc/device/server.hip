@@ -124,7 +124,8 @@ struct Request {
   uint64_t job_rows = 0;
   uint64_t bytes = 0;
   uint64_t heap_off = ~0ull;
-  hipEvent_t uploaded = nullptr;
+  hipEvent_t copied = nullptr;    // its copies have landed (copy stream)
+  hipEvent_t uploaded = nullptr;  // and its bitmap rows are built (build stream)
   SegDesc seg{};
   uint64_t next = 0;  // next item to put in a launch
   // estimated work of items [0, x): wpref[t] before task t (the reference's
@@ -186,7 +187,10 @@ struct bbtc_server {
 
   char* heap_base = nullptr;
   bbtc::Heap heap;
-  hipStream_t copy = nullptr, compute = nullptr, readback = nullptr;
+  // copies in (copy), the bitmap rows of uploaded requests (build, so the
+  // next request's copies need not wait behind those kernels), counting
+  // launches (compute), counters out (readback)
+  hipStream_t copy = nullptr, build = nullptr, compute = nullptr, readback = nullptr;
   bbtc::SlotCtl* ctl = nullptr;
   bbtc::SpillRec* spill = nullptr;
   uint64_t* spill_base = nullptr;
@@ -316,6 +320,7 @@ void admit_loop(bbtc_server* S) {
         }
       }
       // copy the block CSR and the plan into the block
+      const auto c0 = Clock::now();
       const Layout& L = *r->L;
       char* b = S->heap_base + r->heap_off;
       hipStream_t s = S->copy;
@@ -329,13 +334,18 @@ void admit_loop(bbtc_server* S) {
                             hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(b + r->o_pref, r->plan.pref.data(), r->plan.pref.size() * 8, hipMemcpyHostToDevice, s));
       auto* cnt = reinterpret_cast<unsigned long long*>(b + r->o_cnt);
-      enqueue_zero(cnt, r->ntasks, s);
-      // the bitmap rows of the dense tiles, built from the uploaded tiles
       auto* jd = reinterpret_cast<DenseJob*>(b + r->o_jobs);
       if (!r->jobs.empty())
         HIP_OK(hipMemcpyAsync(jd, r->jobs.data(), r->jobs.size() * sizeof(DenseJob), hipMemcpyHostToDevice, s));
+      HIP_OK(hipEventCreateWithFlags(&r->copied, hipEventDisableTiming));
+      HIP_OK(hipEventRecord(r->copied, s));
+      // the counters and the bitmap rows of the dense tiles, built from the
+      // uploaded tiles on the build stream
+      hipStream_t bs = S->build;
+      HIP_OK(hipStreamWaitEvent(bs, r->copied, 0));
+      enqueue_zero(cnt, r->ntasks, bs);
       enqueue_dense_build(reinterpret_cast<const uint32_t*>(b + r->o_rp), reinterpret_cast<const uint32_t*>(b + r->o_col),
-                          reinterpret_cast<uint32_t*>(b + r->o_dense), L.dense_words, jd, r->jobs.size(), r->job_rows, s);
+                          reinterpret_cast<uint32_t*>(b + r->o_dense), L.dense_words, jd, r->jobs.size(), r->job_rows, bs);
       SegDesc& g = r->seg;
       g.rowptr = reinterpret_cast<const uint32_t*>(b + r->o_rp);
       g.cols = reinterpret_cast<const uint32_t*>(b + r->o_col);
@@ -348,9 +358,10 @@ void admit_loop(bbtc_server* S) {
       g.cycles = nullptr;
       g.ntasks = uint32_t(r->ntasks);
       HIP_OK(hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming));
-      HIP_OK(hipEventRecord(r->uploaded, s));
+      HIP_OK(hipEventRecord(r->uploaded, bs));
       {
         std::lock_guard<std::mutex> lk(S->mu);
+        S->stats.upload_s += std::chrono::duration<double>(Clock::now() - c0).count();
         S->ready.push_back(r);
       }
       S->cv_sched.notify_all();
@@ -375,6 +386,8 @@ void finish(bbtc_server* S, Request& r) {
   if (r.per_task_out && r.ntasks) std::copy(per.begin(), per.begin() + r.ntasks, r.per_task_out);
   if (r.uploaded) (void)hipEventDestroy(r.uploaded);
   r.uploaded = nullptr;
+  if (r.copied) (void)hipEventDestroy(r.copied);
+  r.copied = nullptr;
   std::lock_guard<std::mutex> lk(S->mu);
   r.done_s = S->now();
   S->heap.release(r.heap_off, r.bytes);
@@ -429,11 +442,15 @@ void sched_loop(bbtc_server* S) {
           finish(S, *r);
           act.erase(std::find(act.begin(), act.end(), r));
         }
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, l.start, l.done));
+        {
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->stats.busy_s += ms * 1e-3;
+        }
         if (S->adaptive) {
           // next budget: the work this launch would have done in the target
           // time at its measured rate, at most doubling or halving
-          float ms = 0.f;
-          HIP_OK(hipEventElapsedTime(&ms, l.start, l.done));
           if (ms > 0.01f && l.work > 0) {
             const double want = l.work * S->launch_ms / double(ms);
             double& w = S->work_budget[l.kind];
@@ -602,7 +619,7 @@ void server_free(bbtc_server* S) {
   if (S->ctl) (void)hipFree(S->ctl);
   if (S->spill) (void)hipFree(S->spill);
   if (S->spill_base) (void)hipFree(S->spill_base);
-  for (hipStream_t s : {S->copy, S->compute, S->readback})
+  for (hipStream_t s : {S->copy, S->build, S->compute, S->readback})
     if (s) (void)hipStreamDestroy(s);
   if (S->ctx) S->ctx->in_use -= std::min(S->ctx->in_use, S->heap.size());
   delete S;
@@ -692,6 +709,7 @@ bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc
       S->seg_host.push_back(h);
     }
     HIP_OK(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&S->build, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&S->compute, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&S->readback, hipStreamNonBlocking));
     S->epoch = bbtc::Clock::now();

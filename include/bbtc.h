@@ -518,6 +518,8 @@ typedef struct bbtc_server_stats {
   uint64_t launches, segments, items;
   uint64_t peak_resident_bytes, heap_bytes;
   uint32_t peak_active, peak_segments;
+  double busy_s;    /* sum of the counting launches' device times (HIP events) */
+  double upload_s;  /* host time the upload thread spent copying requests in */
 } bbtc_server_stats;
 
 bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc_server** out);
