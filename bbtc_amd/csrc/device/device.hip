@@ -535,6 +535,26 @@ void build_graph_rows(bbtc_ctx* ctx, bbtc_dgraph* g) {
   ctx->in_use -= std::min(ctx->in_use, acct);
 }
 
+// Whether BBTC_KERNEL_AUTO takes the row form for a graph whose tasks all
+// admit the lane form. Measured on G(n, m) graphs of 2^12-2^22 vertices and
+// average degree 4-64 with 1-16 cuts: the row form wins on large graphs
+// (n >= 2^20: 2-2.7x on G(n=2^22, m=2^26), 2x on G(n=2^20, m=2^24)), with
+// one or two cuts, and at average degree up to 24; on smaller graphs of
+// higher degree with 4 or more cuts the tiled lane form is faster (its
+// tiles stay in the caches and its per-task lists are shorter, so its waves
+// diverge less), by up to 7x at degree 63 with 8 cuts.
+bool rows_pay(const bbtc_dgraph* g) {
+  const uint32_t P = g->P;
+  if (P == 0) return false;
+  double n = 0, m = 0;
+  for (uint32_t i = 0; i < P; ++i) {
+    const double h = g->htiles[size_t(i) * P + i].h;
+    n += h;
+    for (uint32_t k = i; k < P; ++k) m += double(g->deg[size_t(i) * P + k]) * h;  // nnz of tile (i, k)
+  }
+  return n >= double(1u << 20) || P <= 2 || 2.0 * m <= 24.0 * n;
+}
+
 // Sets up the row form for plan p over refs (a lane-only plan of a graph
 // with a row table): the map from lexicographic task index to plan task.
 // Leaves p unchanged when the plan lists a task twice.
@@ -590,8 +610,8 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
       if (!own.empty()) work = &own;
     }
     PlanDesc D = describe_plan(g->htiles, g->deg, refs, kernel, choice, true, work);
-    const bool want_rows = (kernel == BBTC_KERNEL_AUTO || kernel == BBTC_KERNEL_ROWS) && !choice && D.lane_only &&
-                           n > 0 && g->P > 0;
+    const bool want_rows = (kernel == BBTC_KERNEL_ROWS || (kernel == BBTC_KERNEL_AUTO && rows_pay(g))) && !choice &&
+                           D.lane_only && n > 0 && g->P > 0;
     if (want_rows && !g->rows_tried) build_graph_rows(ctx, g);
     if (D.lane_only && g->P > 0 && !g->slots) build_graph_slots(ctx, g);
     std::vector<TaskDesc>& td = D.td;
