@@ -101,8 +101,16 @@ void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, ui
     }
     const uint64_t o = c * kStageBytes, len = std::min(kStageBytes, bytes - o);
     parallel_copy(st.buf[b], static_cast<const char*>(src) + o, len);
-    HIP_OK(hipMemcpyAsync(static_cast<char*>(dst) + o, st.buf[b], len, hipMemcpyHostToDevice, s));
-    HIP_OK(hipEventRecord(st.ev[b], s));
+    try {
+      HIP_OK(hipMemcpyAsync(static_cast<char*>(dst) + o, st.buf[b], len, hipMemcpyHostToDevice, s));
+      HIP_OK(hipEventRecord(st.ev[b], s));
+    } catch (...) {
+      // a copy from either buffer may still be in flight: wait for the stream
+      // before the buffers can be refilled
+      (void)hipStreamSynchronize(s);
+      st.pending[0] = st.pending[1] = false;
+      throw;
+    }
     st.pending[b] = true;
   }
 }
@@ -504,7 +512,7 @@ void build_graph_rows(bbtc_ctx* ctx, bbtc_dgraph* g) {
     HIP_OK(hipMemcpyAsync(&longest, maxlen, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     uint32_t S = 4;
-    while (S < longest) S <<= 1;
+    while (S < longest && S < kRowSlotMax) S <<= 1;
     const uint64_t bytes = nrows * S * 4;
     if (longest <= kRowSlotMax && bytes <= kRowTableMaxBytes && !(ctx->budget && ctx->in_use + bytes > ctx->budget)) {
       uint64_t racct = 0;
