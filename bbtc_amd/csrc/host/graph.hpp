@@ -97,8 +97,9 @@ Layout layout_frame(uint32_t n, uint64_t m, const std::vector<uint32_t>& cuts);
 // ...and, once nnz, nz_rows and max_row of every tile are known, the column
 // and row-list bases, the tile degrees and the bitmap placement.
 void layout_place(Layout& L);
-// memcpy by all OpenMP threads (the destination's pages are touched in
-// parallel, which a fresh host array needs to fill at memory speed).
+// memcpy by all OpenMP threads in 1 MiB pieces taken dynamically (the
+// destination's pages are touched in parallel, which a fresh host array
+// needs to fill at memory speed).
 void parallel_copy(void* dst, const void* src, size_t bytes);
 // One past the largest column of g (0 when it has no entries); a layout
 // needs it at most cuts[P].

@@ -112,7 +112,11 @@ void require_host_arrays(const Layout& L, const char* what) {
 void parallel_copy(void* dst, const void* src, size_t bytes) {
   constexpr size_t kPiece = 1 << 20;
   const int64_t pieces = int64_t((bytes + kPiece - 1) / kPiece);
-#pragma omp parallel for schedule(static)
+  // pieces taken dynamically: a copy made from a thread other than the
+  // caller's main one (the server's upload thread) shares the cores with
+  // other busy threads, and a statically split copy waits for its slowest
+  // thread
+#pragma omp parallel for schedule(dynamic, 1)
   for (int64_t q = 0; q < pieces; ++q) {
     const size_t o = size_t(q) * kPiece;
     std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(kPiece, bytes - o));
