@@ -327,6 +327,27 @@ def test_cli_gpu(tmp_path):
             assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
 
 
+def test_cli_rows_form(tmp_path):
+    # a G(n, m) graph whose tasks all admit the lane form: --kernel rows
+    # counts it with the row form (and with --border, the contested task
+    # ranges in the lane form); per-task output equals the golden vectors
+    d = load_golden("small_gnm_2k")[0]
+    g = graph_from_spec(d["graph"])
+    path = str(tmp_path / "g.bbtc")
+    g.save(path)
+    for run in d["runs"]:
+        for extra in (["--kernel", "rows"], ["--kernel", "rows", "--border", "3"],
+                      ["--kernel", "rows", "--device-layout", "--border", "-1"]):
+            out = tmp_path / "pt.txt"
+            cmd = [B.CLI_PATH, "--graph", path, "--partitioner", run["partitioner"], "--ncut", str(run["ncut"]),
+                   "--repeat", "2", "--ngpu", "1", "--per-task-out", str(out)] + extra
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            assert f"Number of Triangles: {run['total']}" in r.stdout
+            got = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in out.read_text().splitlines()}
+            assert got == {tuple(x[:3]): x[3] for x in run["tasks"]}
+
+
 def test_cli_forms_out_and_in(tmp_path):
     # the forms measured by --kernel tuned, written once and reused: the same
     # per-task counts without autotuning; a file for other cuts is refused
