@@ -970,9 +970,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
       const uint32_t ex = (q_d - 1) >> 2;
       u32x4 x = rows[xs + bx];
       if (Y[0].x == kRowPad) continue;  // N+(v) is empty
-      const uint32_t bv = band_of(cuts, a.P, v);
-      const uint64_t lex_uv = lex_task(a.P, q_bu, bv, bv);
+      // count cc of common entries in band kc not yet added to its task;
+      // the task's slot (through band(v)) is looked up only when there is a
+      // count to add, which is rare
       uint32_t kc = 0, cc = 0;
+      auto add = [&]() {
+        const uint32_t bv = band_of(cuts, a.P, v);
+        // in an upper-triangular graph every common entry w > v >= cuts[bv],
+        // so kc >= bv
+        const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_task(a.P, q_bu, bv, kc)] : ~0u;
+        if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
+        cc = 0;
+      };
+      uint32_t lx = last_entry(x);
       bool done = false;
       for (uint32_t r = 0;;) {
 #pragma unroll
@@ -985,24 +995,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
           }
           const uint32_t ly = last_entry(y);
           for (;;) {
-            const uint32_t xv[4] = {x.x, x.y, x.z, x.w};
+            // x entries equal to an entry of y (a pad matches only a pad,
+            // which is excluded below)
+            const uint32_t m = ((x.x == y.x) | (x.x == y.y) | (x.x == y.z) | (x.x == y.w)) |
+                               ((x.y == y.x) | (x.y == y.y) | (x.y == y.z) | (x.y == y.w)) << 1 |
+                               ((x.z == y.x) | (x.z == y.y) | (x.z == y.z) | (x.z == y.w)) << 2 |
+                               ((x.w == y.x) | (x.w == y.y) | (x.w == y.z) | (x.w == y.w)) << 3;
+            if (m) {
+              const uint32_t xv[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const uint32_t c = xv[e];
-              if (c != kRowPad && (c == y.x || c == y.y || c == y.z || c == y.w)) {
-                const uint32_t k = band_of(cuts, a.P, c);
-                if (k != kc && cc) {
-                  const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_uv + (kc - bv)] : ~0u;
-                  if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
-                  cc = 0;
+              for (int e = 0; e < 4; ++e) {
+                const uint32_t c = xv[e];
+                if (((m >> e) & 1u) && c != kRowPad) {
+                  const uint32_t k = band_of(cuts, a.P, c);
+                  if (k != kc && cc) add();
+                  kc = k;
+                  ++cc;
                 }
-                kc = k;
-                ++cc;
               }
             }
             // the block whose last entry is smaller moves on (both when
             // equal); a list whose last block moves on is done
-            const uint32_t lx = last_entry(x);
             const bool na = lx <= ly, nb = ly <= lx;
             if (na) {
               if (bx == ex) {
@@ -1010,6 +1023,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                 break;
               }
               x = rows[xs + ++bx];
+              lx = last_entry(x);
             }
             if (nb) {
               if (y.w == kRowPad) done = true;
@@ -1022,12 +1036,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 #pragma unroll
         for (int q = 0; q < 4; ++q) Y[q] = r + q < nblk ? rows[ys + r + q] : pads;
       }
-      if (cc) {
-        // in an upper-triangular graph every common entry w > v >= cuts[bv],
-        // so kc >= bv
-        const uint32_t t = kc >= bv ? as_global(a.lex2plan)[lex_uv + (kc - bv)] : ~0u;
-        if (t != ~0u) atomicAdd(a.per_task + t, (unsigned long long)cc);
-      }
+      if (cc) add();
     }
   }
 }
