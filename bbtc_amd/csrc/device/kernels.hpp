@@ -965,7 +965,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
       const u32x4 pads = {kRowPad, kRowPad, kRowPad, kRowPad};
       u32x4 Y[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Y[q] = uint32_t(q) < nblk ? rows[ys + q] : pads;
+      for (int yb = 0; yb < 4; ++yb) Y[yb] = uint32_t(yb) < nblk ? rows[ys + yb] : pads;
       uint32_t bx = x0 >> 2;
       const uint32_t ex = (q_d - 1) >> 2;
       u32x4 x = rows[xs + bx];
@@ -986,8 +986,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
       bool done = false;
       for (uint32_t r = 0;;) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const u32x4 y = Y[q];
+        for (int yb = 0; yb < 4; ++yb) {
+          const u32x4 y = Y[yb];
           if (done) break;
           if (y.x == kRowPad) {
             done = true;
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         r += 4;
         if (done || r >= nblk) break;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Y[q] = r + q < nblk ? rows[ys + r + q] : pads;
+        for (int yb = 0; yb < 4; ++yb) Y[yb] = r + yb < nblk ? rows[ys + r + yb] : pads;
       }
       if (cc) add();
     }
