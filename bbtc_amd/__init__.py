@@ -573,7 +573,7 @@ class Plan:
 
     def kinds(self) -> np.ndarray:
         """Form per task: 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane, 6 rows
-        (counted by the row form's launch), 0 no work."""
+        (a task with work counted by the row form's launch)."""
         k = np.zeros(len(self.tasks), dtype=np.uint8)
         _check(lib().bbtc_plan_task_kinds(self._h, _ptr(k)))
         return k
