@@ -180,6 +180,26 @@ def test_rows_form_vs_oracle(dev, case):
     dg.release()
 
 
+def test_rows_form_many_bands(dev):
+    # 160 uni bands (695,520 tasks, the band starts past the rule's
+    # statistics limit of 128 cuts): every band lookup and task slot of the
+    # row form against the oracle
+    g = B.Graph.gnm(20000, 150000, 21)
+    cuts = g.partition("uni", 160)
+    L = B.Layout(g, cuts)
+    t = L.tasks("lex")
+    R = O.RefLayout(g.offsets(), g.cols(), cuts)
+    rper, rtot = R.count(np.array(t.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    pl = dg.plan(t, "rows")
+    assert (pl.kinds() == 6).any()
+    pl.launch()
+    per, tot = pl.results()
+    pl.destroy()
+    dg.release()
+    assert tot == rtot and np.array_equal(per, rper)
+
+
 def test_rows_form_declined(dev):
     # rows of more than 64 entries (G(n, m) of average degree 160): no row
     # table, the plan keeps the lane or other forms; a task listed twice: the
