@@ -118,6 +118,8 @@ def _declare(lib):
         "bbtc_ctx_synchronize": (st, [vp]),
         "bbtc_ctx_read_bandwidth": (st, [vp, u64, i32, vp]),
         "bbtc_graph_upload": (st, [vp, vp, pvp]),
+        "bbtc_layout_pin": (st, [vp, vp]),
+        "bbtc_layout_pinned": (i32, [vp]),
         "bbtc_graph_build": (st, [vp, vp, vp, u32, u32, pvp, pvp]),
         "bbtc_graph_tasks_stats": (st, [vp, vp, u64, u32, vp]),
         "bbtc_graph_bytes": (u64, [vp]),
@@ -339,6 +341,11 @@ class Layout:
     def device_bytes(self) -> int:
         return lib().bbtc_layout_device_bytes(self._h)
 
+    @property
+    def pinned(self) -> bool:
+        """Whether the arrays are in page-locked host memory (Device.pin)."""
+        return bool(lib().bbtc_layout_pinned(self._h))
+
     def arrays(self):
         """Copies of (rowptr, cols, nzrows) (bbtc_layout_arrays); None when
         the layout holds no host arrays."""
@@ -428,6 +435,14 @@ class Device:
 
     def upload(self, layout: Layout) -> "DeviceGraph":
         return DeviceGraph(self, layout)
+
+    def pin(self, layout: Layout) -> Layout:
+        """Moves the layout's block-CSR arrays into page-locked host memory
+        (bbtc_layout_pin, as the reference's BlockCSR holds its tiles), so
+        uploads, batches and served requests copy them straight over PCIe.
+        Returns the layout."""
+        _check(lib().bbtc_layout_pin(self._h, layout._h))
+        return layout
 
     def build(self, graph: Graph, cuts, host_arrays: bool = True) -> "DeviceGraph":
         """The block layout of graph built on this device (bbtc_graph_build):

@@ -67,20 +67,43 @@ void* scratch_of_device(int device) {
 // ---------------------------------------------------------------------------
 // Pinned staging of host-to-device copies.
 // ---------------------------------------------------------------------------
-constexpr uint64_t kStageBytes = 64ull << 20;
+constexpr uint64_t kStageBytes = kStageImageBytes;
 
 struct Staging {
   std::mutex mu;
   void* buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool pending[2] = {false, false};  // a copy from the buffer may still be in flight
+  int next = 0;                      // the buffer the next chunk fills
+
+  // The next buffer, once its previous chunk has left (st.mu held).
+  int take() {
+    const int b = next;
+    next ^= 1;
+    if (pending[b]) {
+      HIP_OK(hipEventSynchronize(ev[b]));
+      pending[b] = false;
+    }
+    return b;
+  }
+  // Copies buf[b][0, len) to dst on s and marks the buffer busy until then.
+  void send(int b, void* dst, uint64_t len, hipStream_t s) {
+    try {
+      HIP_OK(hipMemcpyAsync(dst, buf[b], len, hipMemcpyHostToDevice, s));
+      HIP_OK(hipEventRecord(ev[b], s));
+    } catch (...) {
+      // a copy from either buffer may still be in flight: wait for the stream
+      // before the buffers can be refilled
+      (void)hipStreamSynchronize(s);
+      pending[0] = pending[1] = false;
+      throw;
+    }
+    pending[b] = true;
+  }
 };
 
-void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, uint64_t bytes) {
-  if (bytes < kStageMinBytes) {
-    if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    return;
-  }
+namespace {
+Staging& staging_of(bbtc_ctx* ctx) {
   {
     std::lock_guard<std::mutex> lk(scratch_mu);
     if (!ctx->h2d) ctx->h2d = new Staging;
@@ -91,28 +114,61 @@ void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, ui
     if (!st.buf[b]) HIP_OK(hipHostMalloc(&st.buf[b], kStageBytes, hipHostMallocDefault));
     if (!st.ev[b]) HIP_OK(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
   }
+  return st;
+}
+}  // namespace
+
+void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, uint64_t bytes, bool src_pinned) {
+  if (bytes < kStageMinBytes || src_pinned) {
+    if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  Staging& st = staging_of(ctx);
+  std::lock_guard<std::mutex> lk(st.mu);
   const uint64_t nchunks = (bytes + kStageBytes - 1) / kStageBytes;
   for (uint64_t c = 0; c < nchunks; ++c) {
-    const int b = int(c & 1);
-    if (st.pending[b]) {
-      // the buffer's previous chunk (this call's or an earlier call's) has left
-      HIP_OK(hipEventSynchronize(st.ev[b]));
-      st.pending[b] = false;
-    }
+    const int b = st.take();
     const uint64_t o = c * kStageBytes, len = std::min(kStageBytes, bytes - o);
     parallel_copy(st.buf[b], static_cast<const char*>(src) + o, len);
-    try {
-      HIP_OK(hipMemcpyAsync(static_cast<char*>(dst) + o, st.buf[b], len, hipMemcpyHostToDevice, s));
-      HIP_OK(hipEventRecord(st.ev[b], s));
-    } catch (...) {
-      // a copy from either buffer may still be in flight: wait for the stream
-      // before the buffers can be refilled
-      (void)hipStreamSynchronize(s);
-      st.pending[0] = st.pending[1] = false;
-      throw;
-    }
-    st.pending[b] = true;
+    st.send(b, static_cast<char*>(dst) + o, len, s);
   }
+}
+
+void copy_image_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const CopyPiece* pieces, size_t npieces,
+                          uint64_t copy_bytes) {
+  uint64_t end = 0;
+  for (size_t q = 0; q < npieces; ++q) end = std::max(end, pieces[q].offset + pieces[q].bytes);
+  if (end > copy_bytes || copy_bytes > kStageBytes)
+    fail(BBTC_ERR_INTERNAL, "copy image of " + std::to_string(copy_bytes) + " bytes does not hold its pieces or "
+                            "exceeds a staging buffer");
+  if (!copy_bytes) return;
+  Staging& st = staging_of(ctx);
+  std::lock_guard<std::mutex> lk(st.mu);
+  const int b = st.take();
+  char* img = static_cast<char*>(st.buf[b]);
+  for (size_t q = 0; q < npieces; ++q)
+    if (pieces[q].bytes) std::memcpy(img + pieces[q].offset, pieces[q].src, pieces[q].bytes);
+  st.send(b, dst, copy_bytes, s);
+}
+
+namespace {
+void pinned_free(void* p) { (void)hipHostFree(p); }
+
+template <class T>
+void pin_buf(Buf<T>& b) {
+  if (b.pinned() || b.size() == 0) return;
+  const size_t bytes = b.size() * sizeof(T);
+  void* p = nullptr;
+  HIP_OK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  parallel_copy(p, b.data(), bytes);
+  b.adopt(static_cast<T*>(p), b.size(), &pinned_free);
+}
+}  // namespace
+
+void pin_layout(Layout& L) {
+  pin_buf(L.rowptr);
+  pin_buf(L.cols);
+  pin_buf(L.nzrows);
 }
 
 void staging_release(bbtc_ctx* ctx) {
@@ -1009,6 +1065,24 @@ bbtc_status bbtc_ctx_read_bandwidth(bbtc_ctx* ctx, uint64_t bytes, int iters, do
   DEV_CATCH
 }
 
+bbtc_status bbtc_layout_pin(bbtc_ctx* ctx, bbtc_layout* Lh) {
+  DEV_REQUIRE(ctx && Lh, "null argument");
+  DEV_TRY
+  bbtc::require_host_arrays(Lh->L, "bbtc_layout_pin");
+  HIP_OK(hipSetDevice(ctx->device));
+  bbtc::pin_layout(Lh->L);
+  DEV_CATCH
+}
+
+int bbtc_layout_pinned(const bbtc_layout* Lh) {
+  if (!Lh || !Lh->L.host_arrays) return 0;
+  const bbtc::Layout& L = Lh->L;
+  return (L.rowptr.size() == 0 || L.rowptr.pinned()) && (L.cols.size() == 0 || L.cols.pinned()) &&
+                 (L.nzrows.size() == 0 || L.nzrows.pinned())
+             ? 1
+             : 0;
+}
+
 bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph** out) {
   DEV_REQUIRE(ctx && Lh && out, "null argument");
   DEV_TRY
@@ -1040,9 +1114,9 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
     g->nzrows = bbtc::dev_alloc<uint32_t>(ctx, L.nzrows.size(), &g->bytes);
     g->dense = bbtc::dev_alloc<uint32_t>(ctx, L.dense_words, &g->bytes);
     g->tiles = bbtc::dev_alloc<bbtc::TileDesc>(ctx, T, &g->bytes);
-    bbtc::copy_to_device(ctx, ctx->stream, g->rowptr, L.rowptr.data(), L.rowptr.size() * 4);
-    bbtc::copy_to_device(ctx, ctx->stream, g->cols, L.cols.data(), L.cols.size() * 4);
-    bbtc::copy_to_device(ctx, ctx->stream, g->nzrows, L.nzrows.data(), L.nzrows.size() * 4);
+    bbtc::copy_to_device(ctx, ctx->stream, g->rowptr, L.rowptr.data(), L.rowptr.size() * 4, L.rowptr.pinned());
+    bbtc::copy_to_device(ctx, ctx->stream, g->cols, L.cols.data(), L.cols.size() * 4, L.cols.pinned());
+    bbtc::copy_to_device(ctx, ctx->stream, g->nzrows, L.nzrows.data(), L.nzrows.size() * 4, L.nzrows.pinned());
     HIP_OK(hipMemcpyAsync(g->tiles, g->htiles.data(), T * sizeof(bbtc::TileDesc), hipMemcpyHostToDevice, ctx->stream));
     // bitmap rows of the dense tiles, built from the uploaded tiles
     uint64_t rows = 0, acct = 0;
@@ -1063,6 +1137,8 @@ bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* Lh, bbtc_dgraph*
     bbtc::dev_free(ctx, jd);
     ctx->in_use -= std::min(ctx->in_use, acct);
   } catch (...) {
+    // copies straight from a pinned layout may still be in flight
+    (void)hipStreamSynchronize(ctx->stream);
     bbtc::release_graph(g);
     throw;
   }
@@ -1473,9 +1549,9 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
           A->deg[tile0[gi] + t] = L.deg[t];
         }
       hipStream_t s = ctx->stream;
-      bbtc::copy_to_device(ctx, s, A->rowptr + rp0[gi], L.rowptr.data(), L.rowptr.size() * 4);
-      bbtc::copy_to_device(ctx, s, A->cols + col0[gi], L.cols.data(), L.cols.size() * 4);
-      bbtc::copy_to_device(ctx, s, A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4);
+      bbtc::copy_to_device(ctx, s, A->rowptr + rp0[gi], L.rowptr.data(), L.rowptr.size() * 4, L.rowptr.pinned());
+      bbtc::copy_to_device(ctx, s, A->cols + col0[gi], L.cols.data(), L.cols.size() * 4, L.cols.pinned());
+      bbtc::copy_to_device(ctx, s, A->nzrows + nz0[gi], L.nzrows.data(), L.nzrows.size() * 4, L.nzrows.pinned());
       uint64_t rows = 0;
       const std::vector<bbtc::DenseJob> gj = bbtc::dense_jobs(L, rp0[gi], col0[gi], dn0[gi], job_rows, &rows);
       jobs.insert(jobs.end(), gj.begin(), gj.end());
@@ -1544,6 +1620,8 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
     b->start = bbtc::dev_alloc<unsigned long long>(ctx, 1, &b->bytes);
     bbtc::batch_replan(b, kernel, nullptr);
   } catch (...) {
+    // copies straight from pinned layouts may still be in flight
+    (void)hipStreamSynchronize(ctx->stream);
     bbtc::batch_release(b);
     throw;
   }

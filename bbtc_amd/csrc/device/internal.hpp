@@ -92,9 +92,36 @@ void enqueue_count(bbtc_ctx* ctx, CountArgs a, unsigned long long i0, uint64_t i
 // one buffer while the other's chunk is in flight over PCIe (pinned copies
 // run at about twice the rate of pageable ones on MI355X hosts), so src may
 // be reused on return; smaller copies are plain hipMemcpyAsync calls. Calls
-// on one context are serialised on its staging buffers.
+// on one context are serialised on its staging buffers. With src_pinned (src
+// is page-locked memory, as after bbtc_layout_pin) the copy is one
+// hipMemcpyAsync straight from src, which must then stay valid until the
+// stream has passed it.
 constexpr uint64_t kStageMinBytes = 8ull << 20;
-void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, uint64_t bytes);
+void copy_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const void* src, uint64_t bytes,
+                    bool src_pinned = false);
+// Moves a layout's block-CSR arrays into page-locked host memory (no-op for
+// arrays already there).
+void pin_layout(Layout& L);
+
+// Host-to-device copies below this size ran as blit kernels on the compute
+// units (a copy trace on MI355X: 4 KiB copies did, 64 KiB and larger ran on
+// the SDMA engines), and such a kernel waits for a free CU slot behind a
+// persistent count launch that holds them all, stalling the copies queued
+// after it on its stream.
+constexpr uint64_t kSdmaMinBytes = 64ull << 10;
+constexpr uint64_t kStageImageBytes = 64ull << 20;  // one staging buffer
+struct CopyPiece {
+  uint64_t offset;  // from the image's start
+  const void* src;
+  uint64_t bytes;
+};
+// Packs the pieces into one staging buffer at their offsets and copies its
+// first copy_bytes (at least every piece's end, at most kStageImageBytes)
+// to dst in one hipMemcpyAsync on s: many small copies become one that the
+// SDMA engines take when copy_bytes >= kSdmaMinBytes. The bytes between the
+// pieces are undefined on the device. The sources may be reused on return.
+void copy_image_to_device(bbtc_ctx* ctx, hipStream_t s, void* dst, const CopyPiece* pieces, size_t npieces,
+                          uint64_t copy_bytes);
 // Frees the context's staging buffers (after its streams are idle).
 void staging_release(bbtc_ctx* ctx);
 

@@ -120,6 +120,10 @@ struct Request {
   double arrival = 0;
   // placement of the arrays inside the request's heap block
   uint64_t o_rp = 0, o_col = 0, o_nz = 0, o_dense = 0, o_tiles = 0, o_tasks = 0, o_pref = 0, o_cnt = 0, o_jobs = 0;
+  // the block's first image_end bytes (the tables, and the graph arrays when
+  // image_graph) are copied in as one packed image (0: piece by piece)
+  uint64_t image_end = 0;
+  bool image_graph = false;
   std::vector<DenseJob> jobs;  // bitmap rows built on the device after the upload
   uint64_t job_rows = 0;
   uint64_t bytes = 0;
@@ -232,7 +236,12 @@ void set_fatal(bbtc_server* S, bbtc_status st, const std::string& msg) {
   S->cv_done.notify_all();
 }
 
-// Places the request's arrays in one block and returns its size.
+// Places the request's arrays in one block and returns its size: the small
+// tables first, then the graph arrays, the counters and the bitmap rows. The
+// tables, with the graph arrays of a graph of at most kStageMinBytes, are
+// copied in as one packed image of at least kSdmaMinBytes, and the counters
+// are read back kSdmaMinBytes at least, so that neither becomes a blit
+// kernel queued behind a count launch; the block is large enough for both.
 uint64_t place(Request& r) {
   const Layout& L = *r.L;
   uint64_t o = 0;
@@ -240,16 +249,21 @@ uint64_t place(Request& r) {
     at = o;
     o += align_up(std::max<uint64_t>(bytes, 1));
   };
-  put(r.o_rp, L.rowptr.size() * 4);
-  put(r.o_col, L.cols.size() * 4);
-  put(r.o_nz, L.nzrows.size() * 4);
-  put(r.o_dense, L.dense_words * 4);
   put(r.o_tiles, r.tiles.size() * sizeof(TileDesc));
   put(r.o_tasks, r.plan.td.size() * sizeof(TaskDesc));
   put(r.o_pref, r.plan.pref.size() * 8);
-  put(r.o_cnt, std::max<uint64_t>(r.ntasks, 1) * 8);
   put(r.o_jobs, r.jobs.size() * sizeof(DenseJob));
-  return o;
+  const uint64_t tables_end = o;
+  put(r.o_rp, L.rowptr.size() * 4);
+  put(r.o_col, L.cols.size() * 4);
+  put(r.o_nz, L.nzrows.size() * 4);
+  const uint64_t graph_bytes = (L.rowptr.size() + L.cols.size() + L.nzrows.size()) * 4;
+  r.image_graph = graph_bytes <= kStageMinBytes && o <= kStageImageBytes;
+  r.image_end = r.image_graph ? o : tables_end;
+  if (r.image_end > kStageImageBytes) r.image_end = 0;  // tables beyond one staging buffer
+  put(r.o_cnt, std::max<uint64_t>(r.ntasks, 1) * 8);
+  put(r.o_dense, L.dense_words * 4);
+  return std::max(o, r.o_cnt + kSdmaMinBytes);
 }
 
 void push_result(bbtc_server* S, const Request& r) {
@@ -324,19 +338,34 @@ void admit_loop(bbtc_server* S) {
       const Layout& L = *r->L;
       char* b = S->heap_base + r->heap_off;
       hipStream_t s = S->copy;
-      copy_to_device(S->ctx, s, b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4);
-      copy_to_device(S->ctx, s, b + r->o_col, L.cols.data(), L.cols.size() * 4);
-      copy_to_device(S->ctx, s, b + r->o_nz, L.nzrows.data(), L.nzrows.size() * 4);
-
-      HIP_OK(hipMemcpyAsync(b + r->o_tiles, r->tiles.data(), r->tiles.size() * sizeof(TileDesc),
-                            hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(b + r->o_tasks, r->plan.td.data(), r->plan.td.size() * sizeof(TaskDesc),
-                            hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(b + r->o_pref, r->plan.pref.data(), r->plan.pref.size() * 8, hipMemcpyHostToDevice, s));
+      const uint64_t tile_b = r->tiles.size() * sizeof(TileDesc), task_b = r->plan.td.size() * sizeof(TaskDesc);
+      const uint64_t pref_b = r->plan.pref.size() * 8, job_b = r->jobs.size() * sizeof(DenseJob);
+      if (r->image_end) {
+        // the image first: its padding may cover the graph arrays' places,
+        // which the copies after it then fill
+        std::vector<CopyPiece> img = {{r->o_tiles, r->tiles.data(), tile_b},
+                                      {r->o_tasks, r->plan.td.data(), task_b},
+                                      {r->o_pref, r->plan.pref.data(), pref_b},
+                                      {r->o_jobs, r->jobs.data(), job_b}};
+        if (r->image_graph) {
+          img.push_back({r->o_rp, L.rowptr.data(), L.rowptr.size() * 4});
+          img.push_back({r->o_col, L.cols.data(), L.cols.size() * 4});
+          img.push_back({r->o_nz, L.nzrows.data(), L.nzrows.size() * 4});
+        }
+        copy_image_to_device(S->ctx, s, b, img.data(), img.size(), std::max(r->image_end, kSdmaMinBytes));
+      } else {
+        HIP_OK(hipMemcpyAsync(b + r->o_tiles, r->tiles.data(), tile_b, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b + r->o_tasks, r->plan.td.data(), task_b, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b + r->o_pref, r->plan.pref.data(), pref_b, hipMemcpyHostToDevice, s));
+        if (job_b) HIP_OK(hipMemcpyAsync(b + r->o_jobs, r->jobs.data(), job_b, hipMemcpyHostToDevice, s));
+      }
+      if (!r->image_graph) {
+        copy_to_device(S->ctx, s, b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4, L.rowptr.pinned());
+        copy_to_device(S->ctx, s, b + r->o_col, L.cols.data(), L.cols.size() * 4, L.cols.pinned());
+        copy_to_device(S->ctx, s, b + r->o_nz, L.nzrows.data(), L.nzrows.size() * 4, L.nzrows.pinned());
+      }
       auto* cnt = reinterpret_cast<unsigned long long*>(b + r->o_cnt);
       auto* jd = reinterpret_cast<DenseJob*>(b + r->o_jobs);
-      if (!r->jobs.empty())
-        HIP_OK(hipMemcpyAsync(jd, r->jobs.data(), r->jobs.size() * sizeof(DenseJob), hipMemcpyHostToDevice, s));
       HIP_OK(hipEventCreateWithFlags(&r->copied, hipEventDisableTiming));
       HIP_OK(hipEventRecord(r->copied, s));
       // the counters and the bitmap rows of the dense tiles, built from the
@@ -359,6 +388,9 @@ void admit_loop(bbtc_server* S) {
       g.ntasks = uint32_t(r->ntasks);
       HIP_OK(hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming));
       HIP_OK(hipEventRecord(r->uploaded, bs));
+      // copies straight from a pinned layout return at once: wait for them,
+      // so the upload time counts the copies, as the staged path's does
+      if (L.cols.pinned()) HIP_OK(hipEventSynchronize(r->copied));
       {
         std::lock_guard<std::mutex> lk(S->mu);
         S->stats.upload_s += std::chrono::duration<double>(Clock::now() - c0).count();
@@ -375,9 +407,11 @@ void admit_loop(bbtc_server* S) {
 
 // Reads back a finished request's counters and frees its block.
 void finish(bbtc_server* S, Request& r) {
-  std::vector<uint64_t> per(std::max<uint64_t>(r.ntasks, 1), 0);
+  // kSdmaMinBytes at least (the block holds them past the counters)
+  const uint64_t rd = std::max<uint64_t>(r.ntasks * 8, kSdmaMinBytes);
+  std::vector<uint64_t> per(rd / 8, 0);
   if (r.ntasks) {
-    HIP_OK(hipMemcpyAsync(per.data(), r.seg.per_task, r.ntasks * 8, hipMemcpyDeviceToHost, S->readback));
+    HIP_OK(hipMemcpyAsync(per.data(), r.seg.per_task, rd, hipMemcpyDeviceToHost, S->readback));
     HIP_OK(hipStreamSynchronize(S->readback));
   }
   uint64_t tot = 0;

@@ -31,19 +31,34 @@ struct Error : std::runtime_error {
 
 // Uninitialised, 256-byte aligned array of trivially copyable T. Large graph
 // arrays (billions of entries) must not pay for std::vector's zero fill.
+// adopt() hands it memory of another allocator (page-locked host memory from
+// the device library) with the function that frees it; pinned() says so.
 template <class T>
 class Buf {
  public:
+  using FreeFn = void (*)(void*);
   Buf() = default;
   explicit Buf(size_t n) { resize(n); }
   Buf(const Buf&) = delete;
   Buf& operator=(const Buf&) = delete;
-  Buf(Buf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  Buf(Buf&& o) noexcept : p_(o.p_), n_(o.n_), free_(o.free_) { o.p_ = nullptr; o.n_ = 0; o.free_ = nullptr; }
   Buf& operator=(Buf&& o) noexcept {
-    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    if (this != &o) {
+      release();
+      p_ = o.p_; n_ = o.n_; free_ = o.free_;
+      o.p_ = nullptr; o.n_ = 0; o.free_ = nullptr;
+    }
     return *this;
   }
   ~Buf() { release(); }
+
+  void adopt(T* p, size_t n, FreeFn free_fn) {
+    release();
+    p_ = p;
+    n_ = n;
+    free_ = free_fn;
+  }
+  bool pinned() const { return free_ != nullptr; }
 
   void resize(size_t n) {
     release();
@@ -67,9 +82,19 @@ class Buf {
   const T* end() const { return p_ + n_; }
 
  private:
-  void release() { std::free(p_); p_ = nullptr; n_ = 0; }
+  void release() {
+    if (free_) {
+      if (p_) free_(p_);
+    } else {
+      std::free(p_);
+    }
+    p_ = nullptr;
+    n_ = 0;
+    free_ = nullptr;
+  }
   T* p_ = nullptr;
   size_t n_ = 0;
+  FreeFn free_ = nullptr;
 };
 
 // Degree-ordered, upper-triangular CSR with sorted rows: the content of a

@@ -187,9 +187,12 @@ def poisson_arrivals(count: int, rate: float, seed: int = 7):
     return np.cumsum(gaps) - gaps[0]  # the first request arrives at 0
 
 
-def prepare_requests(dev: Device, specs, log=print, cands=(1, 2, 4, 8, 16), nsample: int = 8):
+def prepare_requests(dev: Device, specs, log=print, cands=(1, 2, 4, 8, 16), nsample: int = 8,
+                     pin: bool = True):
     """Graphs, per-graph cut counts from a measured rule, layouts and task
-    lists (descending order) of every request."""
+    lists (descending order) of every request; with pin, the layouts'
+    arrays in page-locked host memory (Device.pin), as the reference's
+    BlockCSR holds its tiles."""
     t0 = time.time()
     graphs = [make_graph(dev, s) for s in specs]
     ms = [g.m for g in graphs]
@@ -212,6 +215,11 @@ def prepare_requests(dev: Device, specs, log=print, cands=(1, 2, 4, 8, 16), nsam
     tasks = [L.tasks("desc") for L in layouts]
     log(f"serving: layouts in {time.time() - t0:.1f}s, host block CSR "
         f"{sum(L.device_bytes for L in layouts) / 1e9:.1f} GB")
+    if pin:
+        t0 = time.time()
+        for L in layouts:
+            dev.pin(L)
+        log(f"serving: layouts pinned in {time.time() - t0:.1f}s")
     return graphs, layouts, tasks, ncut
 
 
@@ -274,13 +282,13 @@ def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200
 
 
 def run_serving(dev: Device, specs, log=print, rate=None, load: float = 0.8, heap_bytes: int = 200_000_000_000,
-                **server_opts):
+                pin: bool = True, **server_opts):
     """Serving workload: the plain per-graph launch and the server, first
     saturated (every request at time 0: capacity), then at Poisson arrivals of
     `rate` requests/s (default: `load` times the plain capacity). Per-task
     counts must agree bit for bit between the two. server_opts: Server
     options (launch_us, oldest_pct, part_chunks, ...)."""
-    graphs, layouts, tasks, ncut = prepare_requests(dev, specs, log)
+    graphs, layouts, tasks, ncut = prepare_requests(dev, specs, log, pin=pin)
     del graphs
     n = len(layouts)
     # warm code paths on the smallest request
@@ -329,6 +337,8 @@ def _serving_runs(dev, layouts, tasks, srv, n, p_tot, p_per, p_cap, rate, load, 
         server_p50_ms=percentile(s_lat, 50) * 1e3, server_p99_ms=percentile(s_lat, 99) * 1e3,
         server_launches=st2["launches"], server_peak_active=st2["peak_active"],
         server_peak_resident_gb=st2["peak_resident_bytes"] / 1e9,
+        server_busy_s=st["busy_s"], server_upload_s=st["upload_s"],
+        pinned_layouts=all(L.pinned for L in layouts),
     )
     log(f"serving at {lam:.2f} req/s: server p50/p99 {res['server_p50_ms']:.1f}/{res['server_p99_ms']:.1f} ms, "
         f"plain p50/p99 {res['plain_p50_ms']:.1f}/{res['plain_p99_ms']:.1f} ms; capacity "

@@ -291,6 +291,18 @@ bbtc_status bbtc_hgraph_gen_gnm_device(bbtc_ctx* ctx, uint32_t n, uint64_t m, ui
 /* Replaces CopyTilesToGpu / CopyTileToGpu (bb_tc_helper.cpp:75-172, the
  * --copy-first path): the whole block CSR becomes device resident once. */
 bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* L, bbtc_dgraph** out);
+/* Moves L's block-CSR arrays (row offsets, columns, row lists) into
+ * page-locked host memory, as the reference allocates its BlockCSR's tiles
+ * with cudaMallocHost (block_csr.cpp:296-299): uploads, batches and served
+ * requests then copy them straight over PCIe (the arrays of an unpinned
+ * layout pass through the context's two pinned staging buffers, filled by the
+ * host threads). The contents do not change; a pinned layout must not be
+ * freed while a copy from it may be in flight (a served request until it is
+ * polled). Costs one host copy of the arrays; BBTC_ERR_HIP when the memory
+ * cannot be locked (the layout is then unchanged or partly pinned, and valid
+ * either way). bbtc_layout_pinned: 1 when every array is pinned. */
+bbtc_status bbtc_layout_pin(bbtc_ctx* ctx, bbtc_layout* L);
+int bbtc_layout_pinned(const bbtc_layout* L);
 uint64_t bbtc_graph_bytes(const bbtc_dgraph* g);
 void bbtc_graph_release(bbtc_dgraph* g);
 

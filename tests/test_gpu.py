@@ -624,6 +624,47 @@ def _serving_requests():
     return reqs
 
 
+def test_pinned_layouts(dev):
+    # layouts moved to page-locked memory (bbtc_layout_pin): the upload, a
+    # batch and the server copy straight from their arrays, small ones and
+    # ones past the 8 MiB staging threshold (RMAT s18 columns: 15 MB); every
+    # count against the oracle per task
+    reqs = _serving_requests()[:4]
+    g = B.Graph.rmat(18, 16, seed=5)
+    cuts = g.partition("pal", 4)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    rper, _ = O.RefLayout(g.offsets(), g.cols(), cuts).count(np.array(t.tolist(), dtype=np.uint32))
+    reqs.append((L, t, rper))
+    for L, _, _ in reqs:
+        assert not L.pinned
+        assert dev.pin(L) is L and L.pinned
+        dev.pin(L)  # a second pin leaves it as it is
+        assert L.pinned
+    for L, t, cpu in reqs:
+        dg = dev.upload(L)
+        per, tot = dg.count(t, "auto")
+        dg.release()
+        assert np.array_equal(per, cpu) and tot == int(cpu.sum())
+    srv = B.Server(dev, heap_bytes=1 << 30)
+    ids = [srv.submit(L, t, 0.0) for L, t, _ in reqs]
+    got = {r.id: r for r in srv.drain()}
+    srv.destroy()
+    for i, (_, _, cpu) in zip(ids, reqs):
+        assert got[i].status == 0 and np.array_equal(got[i].per_task, cpu)
+    Ls = [L for L, _, _ in reqs]
+    b = B.Batch(dev, Ls, "auto")
+    gidx, bt = b.tasks()
+    b.launch()
+    pg, pt, _ = b.results()
+    b.destroy()
+    assert np.array_equal(pg, np.array([int(c.sum()) for _, _, c in reqs], dtype=np.uint64))
+    index = [{tuple(x): q for q, x in enumerate(t.tolist())} for _, t, _ in reqs]
+    want = np.array([reqs[gi][2][index[gi][tuple(x)]] for gi, x in zip(gidx.tolist(), bt.tolist())],
+                    dtype=np.uint64)
+    assert np.array_equal(pt, want)
+
+
 def test_server_alone_vs_mix(dev):
     reqs = _serving_requests()
     # alone: one request in the server at a time
