@@ -119,6 +119,7 @@ def _declare(lib):
         "bbtc_ctx_read_bandwidth": (st, [vp, u64, i32, vp]),
         "bbtc_graph_upload": (st, [vp, vp, pvp]),
         "bbtc_layout_pin": (st, [vp, vp]),
+        "bbtc_tasks_shard": (st, [vp, vp, u64, u32, vp, vp]),
         "bbtc_layout_pinned": (i32, [vp]),
         "bbtc_graph_build": (st, [vp, vp, vp, u32, u32, pvp, pvp]),
         "bbtc_graph_tasks_stats": (st, [vp, vp, u64, u32, vp]),
@@ -504,6 +505,16 @@ class DeviceGraph:
         out = np.zeros((len(t), 6), dtype=np.uint64)
         _check(lib().bbtc_graph_tasks_stats(self._h, _ptr(t), len(t), sample_units, _ptr(out)))
         return out
+
+    def shard(self, tasks, shards: int):
+        """(share of each task, its estimated cost) for `shards` GPUs
+        (bbtc_tasks_shard: longest estimated first onto the least loaded
+        share)."""
+        t = tasks_array(tasks)
+        out = np.zeros(len(t), dtype=np.uint32)
+        cost = np.zeros(len(t), dtype=np.float64)
+        _check(lib().bbtc_tasks_shard(self._h, _ptr(t), len(t), shards, _ptr(out), _ptr(cost)))
+        return out, cost
 
     def count(self, tasks, kernel: str = "auto"):
         t = tasks_array(tasks)

@@ -41,6 +41,11 @@ struct TaskRef {
 };
 
 std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n);
+// The AUTO rule's form for a task of work statistics s (ac(u) bitmaps of wpr
+// words per row, fit rows of them in a wave's LDS slice, a bitmap form of the
+// bc tile or not) and, in *cost, its estimated shader clocks (0 without
+// wedges).
+uint32_t auto_kind(const TaskWork& s, uint32_t wpr, uint32_t fit, bool has_dense, double* cost);
 
 // Device tile table of a layout (P*P entries, offsets into its arrays).
 std::vector<TileDesc> tile_table(const Layout& L);

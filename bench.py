@@ -40,6 +40,11 @@ Prints one JSON line:
                 through the continuous-batching server vs the plain
                 sequential launch: capacity, and p50/p99 latency at Poisson
                 arrivals of 80% of the plain capacity; bit-exact counts
+  shards        the RMAT s26 task list split for 2, 4 and 8 GPUs (LPT on the
+                AUTO rule's cost estimates, and the reference's task q to
+                GPU q mod G): each share counted alone on this GPU with its
+                per-task counts checked against the whole run; the slowest
+                share, the imbalance, and the tile bytes a GPU would hold
 """
 from __future__ import annotations
 
@@ -248,6 +253,51 @@ def count_c2(B, dev, steps, cpu=True):
     return out
 
 
+def shard_shares(dg, tasks, per, whole_ms, shards=(2, 4, 8), reps=2):
+    """Each GPU's part of a G-GPU run, counted alone on this GPU at its real
+    shape (all tiles resident, only the share's tasks in the launch): the
+    LPT shares of bbtc_tasks_shard against the reference's dealing of task q
+    to GPU q mod G over the same ordered list. A G-GPU count takes about the
+    slowest share; per-task counts of every share equal the whole run's."""
+    L = dg.layout
+    tile_bytes = {}
+
+    def tbytes(i, k):
+        if (i, k) not in tile_bytes:
+            t = L.tile(i, k)
+            tile_bytes[(i, k)] = 4 * t.nnz + 8 * t.nz_rows + 4
+        return tile_bytes[(i, k)]
+
+    tl = tasks.tolist()
+    whole_tiles = {x for i, j, k in tl for x in ((i, j), (j, k), (i, k))}
+    res = {"whole_ms": whole_ms, "whole_tile_bytes": sum(tbytes(*x) for x in whole_tiles)}
+    for G in shards:
+        lpt, _ = dg.shard(tasks, G)
+        row = {}
+        for name, assign in (("lpt", lpt), ("mod_g", np.arange(len(tasks)) % G)):
+            ms, foot = [], []
+            for s in range(G):
+                sel = assign == s
+                p = dg.plan(tasks[sel], "auto")
+                p.time(1)
+                t = p.time(reps)
+                pp, _ = p.results()
+                p.destroy()
+                if not np.array_equal(pp, per[sel]):
+                    raise RuntimeError(f"share {s} of {G} ({name}) differs from the whole run per task")
+                ms.append(float(np.median(t)))
+                tiles = {x for (i, j, k), y in zip(tl, sel) if y for x in ((i, j), (j, k), (i, k))}
+                foot.append(sum(tbytes(*x) for x in tiles))
+            row[name] = {"share_ms": [round(x, 2) for x in ms], "max_ms": max(ms),
+                         "imbalance": max(ms) / float(np.mean(ms)), "speedup": whole_ms / max(ms),
+                         "max_tile_bytes": max(foot)}
+        res[f"g{G}"] = row
+        log(f"shards of {G}: LPT slowest share {row['lpt']['max_ms']:.1f} ms (imbalance "
+            f"{row['lpt']['imbalance']:.3f}, {row['lpt']['speedup']:.2f}x), task q to GPU q mod {G}: "
+            f"{row['mod_g']['max_ms']:.1f} ms ({row['mod_g']['speedup']:.2f}x)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +322,8 @@ def main():
     ap.add_argument("--no-batch-tune", action="store_true",
                     help="batched workload with the AUTO forms only (no autotuned batch beside it)")
     ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
+    ap.add_argument("--no-shards", action="store_true",
+                    help="skip the per-GPU shares of 2, 4 and 8 GPUs counted on this GPU")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -409,6 +461,8 @@ def main():
         "cpu_baseline": cpu,
     }
     plan.destroy()
+    if not args.no_shards:
+        out["shards"] = shard_shares(dg, tasks, per, ev_median_ms)
     dg.release()
     del plan, dg, L, g
 

@@ -291,6 +291,18 @@ bbtc_status bbtc_hgraph_gen_gnm_device(bbtc_ctx* ctx, uint32_t n, uint64_t m, ui
 /* Replaces CopyTilesToGpu / CopyTileToGpu (bb_tc_helper.cpp:75-172, the
  * --copy-first path): the whole block CSR becomes device resident once. */
 bbtc_status bbtc_graph_upload(bbtc_ctx* ctx, const bbtc_layout* L, bbtc_dgraph** out);
+/* Splits tasks[0..n) of a resident graph into `shards` shares, one per GPU
+ * of a multi-GPU run, and writes each task's share to shard_out[q]: longest
+ * estimated time first onto the least loaded share (LPT). The estimate
+ * (cost_out, may be NULL) is the
+ * AUTO rule's shader clocks for the task's form, or the reference's task
+ * weight (tasks.hpp:21-23) where the graph has no statistics (more than 128
+ * cuts); tasks with an empty tile cost 0. Replaces the reference's dealing
+ * of task q to GPU q mod G (bb_tc.cpp:142-181, thread_id % n_gpu with a
+ * stride of all workers). One share's tasks counted alone on one GPU are
+ * that GPU's part of the run at its real shape. */
+bbtc_status bbtc_tasks_shard(const bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n, uint32_t shards,
+                             uint32_t* shard_out, double* cost_out);
 /* Moves L's block-CSR arrays (row offsets, columns, row lists) into
  * page-locked host memory, as the reference allocates its BlockCSR's tiles
  * with cudaMallocHost (block_csr.cpp:296-299): uploads, batches and served

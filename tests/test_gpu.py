@@ -624,6 +624,39 @@ def _serving_requests():
     return reqs
 
 
+@pytest.mark.parametrize("ncut", [8, 160])
+def test_tasks_shard(dev, ncut):
+    # shares for 1-8 GPUs: every task in exactly one share; each share's
+    # load within the greedy (LPT) bound total/G + the largest task; each
+    # share counted alone equals
+    # the oracle on its tasks; 160 cuts: no statistics, the reference's
+    # task weight as the estimate
+    g = B.Graph.rmat(14, 16, seed=11)
+    cuts = g.partition("uni" if ncut > 128 else "pal", ncut)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    if ncut > 128:
+        t = t[:4000]
+    rper, _ = O.RefLayout(g.offsets(), g.cols(), cuts).count(np.array(t.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    for G in (1, 3, 8):
+        sh, cost = dg.shard(t, G)
+        assert sh.shape == (len(t),) and (sh < G).all() and (cost >= 0).all()
+        if G == 1:
+            assert (sh == 0).all()
+        load = np.bincount(sh, weights=cost, minlength=G)
+        assert load.max() <= cost.sum() / G + cost.max() * (1 + 1e-9)
+        got = np.zeros(len(t), dtype=np.uint64)
+        for s in range(G):
+            sel = sh == s
+            if sel.any():
+                got[sel], _ = dg.count(t[sel], "auto")
+        assert np.array_equal(got, rper), G
+    with pytest.raises(B.BbtcError):
+        dg.shard(t, 0)
+    dg.release()
+
+
 def test_pinned_layouts(dev):
     # layouts moved to page-locked memory (bbtc_layout_pin): the upload, a
     # batch and the server copy straight from their arrays, small ones and
