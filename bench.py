@@ -322,6 +322,8 @@ def main():
     ap.add_argument("--no-batch-tune", action="store_true",
                     help="batched workload with the AUTO forms only (no autotuned batch beside it)")
     ap.add_argument("--serving-requests", type=int, default=64, help="0: skip the serving workload")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving workload (--serving-requests requests); prints its object as the line")
     ap.add_argument("--no-shards", action="store_true",
                     help="skip the per-GPU shares of 2, 4 and 8 GPUs counted on this GPU")
     args = ap.parse_args()
@@ -342,6 +344,10 @@ def main():
         from bbtc_amd import workloads as W
         print(json.dumps(W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log,
                                        tune=not args.no_batch_tune)), flush=True)
+        return
+    if args.serving_only:
+        from bbtc_amd import workloads as W
+        print(json.dumps(W.run_serving(dev, W.serving_specs(args.serving_requests), log=log)), flush=True)
         return
     read_gbs = dev.read_bandwidth(8 << 30, 5)
     stream_gbs = float(np.median(read_gbs))
