@@ -67,7 +67,7 @@ class bbtc_server_stats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("segments", C.c_uint64), ("items", C.c_uint64),
                 ("peak_resident_bytes", C.c_uint64), ("heap_bytes", C.c_uint64),
                 ("peak_active", C.c_uint32), ("peak_segments", C.c_uint32),
-                ("busy_s", C.c_double), ("upload_s", C.c_double)]
+                ("busy_s", C.c_double), ("upload_s", C.c_double), ("row_form_requests", C.c_uint64)]
 
 
 TASK_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("k", np.uint32)])

@@ -188,6 +188,36 @@ bool valid_kernel(bbtc_kernel k) {
          k == BBTC_KERNEL_DENSE || k == BBTC_KERNEL_RDENSE || k == BBTC_KERNEL_LANE || k == BBTC_KERNEL_ROWS;
 }
 
+void enqueue_rows(bbtc_ctx* ctx, const RowArgs& a, hipStream_t s) {
+  const uint64_t items = (uint64_t(a.nrows) + kWave - 1) / kWave;
+  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, 0ull, (unsigned long long)items, 0ull);
+  HIP_OK(hipGetLastError());
+  const uint64_t want = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+  const unsigned blocks = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(ctx, true))));
+  hipLaunchKernelGGL(count_rows, dim3(blocks), dim3(kBlock), 0, s, a);
+  HIP_OK(hipGetLastError());
+}
+
+void enqueue_build_rows(const uint32_t* rowptr, const uint32_t* cols, const TileDesc* tiles, const uint32_t* cuts,
+                        uint32_t P, uint32_t nrows, uint32_t S, uint32_t* rows, uint32_t* rowlen, hipStream_t s) {
+  if (nrows == 0) return;
+  const unsigned blocks = unsigned((uint64_t(nrows) + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(build_rows, dim3(blocks), dim3(kBlock), 0, s, rowptr, cols, tiles, cuts, P,
+                     uint32_t(__builtin_ctz(S)), reinterpret_cast<u32x4*>(rows), rowlen);
+  HIP_OK(hipGetLastError());
+}
+
+bool rows_pay(uint32_t P, const std::vector<TileDesc>& tiles, const std::vector<float>& deg) {
+  if (P == 0) return false;
+  double n = 0, m = 0;
+  for (uint32_t i = 0; i < P; ++i) {
+    const double h = tiles[size_t(i) * P + i].h;
+    n += h;
+    for (uint32_t k = i; k < P; ++k) m += double(deg[size_t(i) * P + k]) * h;  // nnz of tile (i, k)
+  }
+  return n >= double(1u << 20) || P <= 2 || 2.0 * m <= 24.0 * n;
+}
+
 uint32_t auto_kind(const TaskWork& s, uint32_t wpr, uint32_t fit, bool has_dense, double* cost) {
   // the form of least estimated cost among those that apply
   uint32_t kind = kKindSearch;
@@ -478,14 +508,7 @@ void launch_rows(bbtc_plan* p, hipStream_t s) {
   a.nrows = g->cuts[g->P];
   a.P = g->P;
   a.s_shift = uint32_t(__builtin_ctz(g->rows_S));
-  const uint64_t items = (uint64_t(a.nrows) + kWave - 1) / kWave;
-  hipLaunchKernelGGL(reset_slot, dim3(1), dim3(1), 0, s, a.ctl, 0ull, (unsigned long long)items, 0ull);
-  HIP_OK(hipGetLastError());
-  const uint64_t want = (items + kWavesPerBlock - 1) / kWavesPerBlock;
-  const unsigned blocks =
-      unsigned(std::max<uint64_t>(1, std::min<uint64_t>(want, resident_blocks(p->ctx, true))));
-  hipLaunchKernelGGL(count_rows, dim3(blocks), dim3(kBlock), 0, s, a);
-  HIP_OK(hipGetLastError());
+  enqueue_rows(p->ctx, a, s);
 }
 
 void launch(bbtc_plan* p, hipStream_t s) {
@@ -617,17 +640,7 @@ void build_graph_rows(bbtc_ctx* ctx, bbtc_dgraph* g) {
 // higher degree with 4 or more cuts the tiled lane form is faster (its
 // tiles stay in the caches and its per-task lists are shorter, so its waves
 // diverge less), by up to 7x at degree 63 with 8 cuts.
-bool rows_pay(const bbtc_dgraph* g) {
-  const uint32_t P = g->P;
-  if (P == 0) return false;
-  double n = 0, m = 0;
-  for (uint32_t i = 0; i < P; ++i) {
-    const double h = g->htiles[size_t(i) * P + i].h;
-    n += h;
-    for (uint32_t k = i; k < P; ++k) m += double(g->deg[size_t(i) * P + k]) * h;  // nnz of tile (i, k)
-  }
-  return n >= double(1u << 20) || P <= 2 || 2.0 * m <= 24.0 * n;
-}
+bool rows_pay(const bbtc_dgraph* g) { return ::bbtc::rows_pay(g->P, g->htiles, g->deg); }
 
 // Sets up the row form for plan p over refs (a lane-only plan of a graph
 // with a row table): the map from lexicographic task index to plan task.

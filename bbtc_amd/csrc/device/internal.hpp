@@ -46,6 +46,15 @@ std::vector<TaskRef> refs_of(uint32_t P, const bbtc_task* tasks, uint64_t n);
 // bc tile or not) and, in *cost, its estimated shader clocks (0 without
 // wedges).
 uint32_t auto_kind(const TaskWork& s, uint32_t wpr, uint32_t fit, bool has_dense, double* cost);
+// The row form: its launch (the claim counter reset, then count_rows over
+// every row), and its table of N+(v) in S-word slots built from a block CSR
+// on the device (band starts cuts[0..P]).
+void enqueue_rows(bbtc_ctx* ctx, const RowArgs& a, hipStream_t s);
+void enqueue_build_rows(const uint32_t* rowptr, const uint32_t* cols, const TileDesc* tiles, const uint32_t* cuts,
+                        uint32_t P, uint32_t nrows, uint32_t S, uint32_t* rows, uint32_t* rowlen, hipStream_t s);
+// Whether BBTC_KERNEL_AUTO takes the row form for a graph of P bands with
+// these tiles and tile degrees whose tasks all admit the lane form.
+bool rows_pay(uint32_t P, const std::vector<TileDesc>& tiles, const std::vector<float>& deg);
 
 // Device tile table of a layout (P*P entries, offsets into its arrays).
 std::vector<TileDesc> tile_table(const Layout& L);

@@ -396,6 +396,8 @@ void device_task_stats(bbtc_dgraph* g, const std::vector<float>& deg, const bbtc
 void build_on_device(bbtc_ctx* ctx, const Csr& g, const std::vector<uint32_t>& cuts, bool host_arrays,
                      Layout& L, bbtc_dgraph*& dg_out) {
   L = layout_frame(g.n, g.m, cuts);
+  L.longest_row = longest_row(g);
+  L.longest_known = true;
   const uint32_t P = L.P;
   if (column_end(g) > cuts[P]) fail(BBTC_ERR_INVALID, "a column lies at or beyond cuts[P]");
   if (uint64_t(P) * P >= (1ull << 32)) fail(BBTC_ERR_INVALID, "too many cuts");

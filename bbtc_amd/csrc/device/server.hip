@@ -24,9 +24,12 @@
 // with them; a large request is slowed by later arrivals by at most about
 // 100 / oldest_pct. With `lane_form`, requests whose tasks all take the lane
 // form run in launches of their own (the lane kernel), the two kinds taking
-// turns while both have work. A request finishes with the launch that holds
-// its last slice. Its counters are read back and its heap block returns to
-// the heap, which lets the next arrival in.
+// turns while both have work. A request that the plain path would count in
+// the row form gets its row table built after its upload and one count_rows
+// launch of its own, taking turns with the shared launches. A request
+// finishes with the launch that holds its last slice. Its counters are read
+// back and its heap block returns to the heap, which lets the next arrival
+// in.
 //
 // This replaces the reference's per-run task loop (bb_tc.cpp:142-181,
 // 200-252), which counts one graph per process.
@@ -38,6 +41,7 @@
 #include <cstdio>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -124,6 +128,15 @@ struct Request {
   // image_graph) are copied in as one packed image (0: piece by piece)
   uint64_t image_end = 0;
   bool image_graph = false;
+  // The row form, where the plain path's AUTO takes it: one count_rows
+  // launch over the whole request, with the table of N+(v) in rows_S-word
+  // slots built on the build stream after the upload; rcuts are the band
+  // starts, lex2plan maps a lexicographic task to the request's task.
+  bool rows = false;
+  uint32_t rows_S = 0, nrows = 0;
+  std::vector<uint32_t> rcuts, lex2plan;
+  uint64_t o_rows = 0, o_rowlen = 0, o_rcuts = 0, o_lex2plan = 0;
+  RowArgs ra{};
   std::vector<DenseJob> jobs;  // bitmap rows built on the device after the upload
   uint64_t job_rows = 0;
   uint64_t bytes = 0;
@@ -167,7 +180,7 @@ struct Launch {
   hipEvent_t start = nullptr, done = nullptr;  // around the launch on the compute stream
   uint64_t items = 0;
   double work = 0;  // estimated work of its items
-  int kind = 0;  // 1: lane-form requests only
+  int kind = 0;  // 1: lane-form requests only; 2: one request in the row form
   int ring = 0;
   std::vector<std::shared_ptr<Request>> finishing;
 };
@@ -184,6 +197,7 @@ struct bbtc_server {
   uint32_t max_inflight = 0, max_segments = 0;
   bool sequential = false;
   bool lane_turn = false;  // kind of the next launch when both kinds have work
+  bool rows_turn = false;  // a row-form request's launch goes next (shared launches had theirs)
   bool lane_form = false;  // AUTO requests may take the lane form
   uint32_t oldest_pct = 90;  // share of each launch's budget kept by the oldest request
   bool list_chunks = false;  // launches dealt to the claim parts in 4,096-item chunks
@@ -253,6 +267,8 @@ uint64_t place(Request& r) {
   put(r.o_tasks, r.plan.td.size() * sizeof(TaskDesc));
   put(r.o_pref, r.plan.pref.size() * 8);
   put(r.o_jobs, r.jobs.size() * sizeof(DenseJob));
+  put(r.o_rcuts, r.rcuts.size() * 4);
+  put(r.o_lex2plan, r.lex2plan.size() * 4);
   const uint64_t tables_end = o;
   put(r.o_rp, L.rowptr.size() * 4);
   put(r.o_col, L.cols.size() * 4);
@@ -263,7 +279,44 @@ uint64_t place(Request& r) {
   if (r.image_end > kStageImageBytes) r.image_end = 0;  // tables beyond one staging buffer
   put(r.o_cnt, std::max<uint64_t>(r.ntasks, 1) * 8);
   put(r.o_dense, L.dense_words * 4);
+  put(r.o_rows, r.rows ? uint64_t(r.nrows) * r.rows_S * 4 : 0);
+  put(r.o_rowlen, r.rows ? uint64_t(r.nrows) * 4 : 0);
   return std::max(o, r.o_cnt + kSdmaMinBytes);
+}
+
+// Sets up the row form for request r of layout L where the plain path would
+// take it (BBTC_KERNEL_ROWS, or AUTO where rows_pay holds): every task
+// admits the lane form, no row is longer than kRowSlotMax, no task is listed
+// twice, and the table fits kRowTableMaxBytes.
+void plan_request_rows(Request& r, const Layout& L, const std::vector<TaskRef>& refs, bbtc_kernel kernel,
+                       const std::function<bool()>& lane_only) {
+  const uint32_t P = L.P;
+  if (refs.empty() || P == 0 || P > kRowMaxCuts || !L.longest_known || L.longest_row > kRowSlotMax) return;
+  if (!(kernel == BBTC_KERNEL_ROWS || (kernel == BBTC_KERNEL_AUTO && rows_pay(P, r.tiles, L.deg)))) return;
+  if (!lane_only()) return;
+  std::vector<uint32_t> cuts(size_t(P) + 1, 0);
+  for (uint32_t b = 0; b < P; ++b) {
+    const uint64_t e = uint64_t(cuts[b]) + r.tiles[size_t(b) * P + b].h;
+    if (e >= 0xFFFFFFFFull) return;  // ids must stay below kRowPad
+    cuts[b + 1] = uint32_t(e);
+  }
+  uint32_t S = 4;
+  while (S < L.longest_row && S < kRowSlotMax) S <<= 1;
+  if (cuts[P] == 0 || uint64_t(cuts[P]) * S * 4 > kRowTableMaxBytes) return;
+  const uint64_t ntask = lex_task(P, P - 1, P - 1, P - 1) + 1;
+  if (ntask >= 0xFFFFFFFFull) return;
+  std::vector<uint32_t> lex2plan(ntask, ~0u);
+  for (size_t q = 0; q < refs.size(); ++q) {
+    const uint32_t i = refs[q].ab / P, j = refs[q].ab % P, k = refs[q].bc % P;
+    uint32_t& slot = lex2plan[lex_task(P, i, j, k)];
+    if (slot != ~0u) return;  // listed twice
+    slot = uint32_t(q);
+  }
+  r.rows = true;
+  r.rows_S = S;
+  r.nrows = cuts[P];
+  r.rcuts = std::move(cuts);
+  r.lex2plan = std::move(lex2plan);
 }
 
 void push_result(bbtc_server* S, const Request& r) {
@@ -346,7 +399,9 @@ void admit_loop(bbtc_server* S) {
         std::vector<CopyPiece> img = {{r->o_tiles, r->tiles.data(), tile_b},
                                       {r->o_tasks, r->plan.td.data(), task_b},
                                       {r->o_pref, r->plan.pref.data(), pref_b},
-                                      {r->o_jobs, r->jobs.data(), job_b}};
+                                      {r->o_jobs, r->jobs.data(), job_b},
+                                      {r->o_rcuts, r->rcuts.data(), r->rcuts.size() * 4},
+                                      {r->o_lex2plan, r->lex2plan.data(), r->lex2plan.size() * 4}};
         if (r->image_graph) {
           img.push_back({r->o_rp, L.rowptr.data(), L.rowptr.size() * 4});
           img.push_back({r->o_col, L.cols.data(), L.cols.size() * 4});
@@ -358,6 +413,11 @@ void admit_loop(bbtc_server* S) {
         HIP_OK(hipMemcpyAsync(b + r->o_tasks, r->plan.td.data(), task_b, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(b + r->o_pref, r->plan.pref.data(), pref_b, hipMemcpyHostToDevice, s));
         if (job_b) HIP_OK(hipMemcpyAsync(b + r->o_jobs, r->jobs.data(), job_b, hipMemcpyHostToDevice, s));
+        if (r->rows) {
+          HIP_OK(hipMemcpyAsync(b + r->o_rcuts, r->rcuts.data(), r->rcuts.size() * 4, hipMemcpyHostToDevice, s));
+          HIP_OK(hipMemcpyAsync(b + r->o_lex2plan, r->lex2plan.data(), r->lex2plan.size() * 4,
+                                hipMemcpyHostToDevice, s));
+        }
       }
       if (!r->image_graph) {
         copy_to_device(S->ctx, s, b + r->o_rp, L.rowptr.data(), L.rowptr.size() * 4, L.rowptr.pinned());
@@ -375,6 +435,20 @@ void admit_loop(bbtc_server* S) {
       enqueue_zero(cnt, r->ntasks, bs);
       enqueue_dense_build(reinterpret_cast<const uint32_t*>(b + r->o_rp), reinterpret_cast<const uint32_t*>(b + r->o_col),
                           reinterpret_cast<uint32_t*>(b + r->o_dense), L.dense_words, jd, r->jobs.size(), r->job_rows, bs);
+      if (r->rows) {
+        RowArgs& ra = r->ra;
+        ra.rows = reinterpret_cast<const uint32_t*>(b + r->o_rows);
+        ra.rowlen = reinterpret_cast<const uint32_t*>(b + r->o_rowlen);
+        ra.cuts = reinterpret_cast<const uint32_t*>(b + r->o_rcuts);
+        ra.lex2plan = reinterpret_cast<const uint32_t*>(b + r->o_lex2plan);
+        ra.per_task = cnt;
+        ra.nrows = r->nrows;
+        ra.P = L.P;
+        ra.s_shift = uint32_t(__builtin_ctz(r->rows_S));
+        enqueue_build_rows(reinterpret_cast<const uint32_t*>(b + r->o_rp), reinterpret_cast<const uint32_t*>(b + r->o_col),
+                           reinterpret_cast<const TileDesc*>(b + r->o_tiles), ra.cuts, L.P, r->nrows, r->rows_S,
+                           reinterpret_cast<uint32_t*>(b + r->o_rows), reinterpret_cast<uint32_t*>(b + r->o_rowlen), bs);
+      }
       SegDesc& g = r->seg;
       g.rowptr = reinterpret_cast<const uint32_t*>(b + r->o_rp);
       g.cols = reinterpret_cast<const uint32_t*>(b + r->o_col);
@@ -496,8 +570,58 @@ void sched_loop(bbtc_server* S) {
         ring_free.push_back(l.ring);
         progress = true;
       }
+      // A request in the row form is counted by a count_rows launch of its
+      // own (the whole request, oldest first), taking turns with the shared
+      // launches while both have work.
+      Request* rr = nullptr;
+      bool shared_work = false;
+      for (auto& r : act)
+        if (r->next < r->nitems) {
+          if (!r->rows) shared_work = true;
+          else if (!rr) rr = r.get();
+        }
+      if (!ring_free.empty() && rr && (!shared_work || S->rows_turn)) {
+        S->rows_turn = false;
+        Launch l;
+        l.ring = ring_free.back();
+        ring_free.pop_back();
+        l.kind = 2;
+        hipStream_t s = S->compute;
+        RowArgs a = rr->ra;
+        a.ctl = S->ctl;
+        while (S->event_pool.size() < 2) {
+          hipEvent_t e;
+          HIP_OK(hipEventCreate(&e));
+          S->event_pool.push_back(e);
+        }
+        l.start = S->event_pool.back();
+        S->event_pool.pop_back();
+        l.done = S->event_pool.back();
+        S->event_pool.pop_back();
+        l.items = rr->nitems - rr->next;
+        HIP_OK(hipEventRecord(l.start, s));
+        enqueue_rows(S->ctx, a, s);
+        HIP_OK(hipEventRecord(l.done, s));
+        rr->next = rr->nitems;
+        rr->launches++;
+        if (rr->start_s < 0) rr->start_s = S->now();
+        for (auto& x : act)
+          if (x.get() == rr) l.finishing.push_back(x);
+        {
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->stats.launches++;
+          S->stats.segments++;
+          S->stats.items += l.items;
+          S->stats.row_form_requests++;
+          S->stats.peak_segments = std::max<uint32_t>(S->stats.peak_segments, 1u);
+        }
+        inflight.push_back(std::move(l));
+        progress = true;
+      } else if (rr) {
+        S->rows_turn = true;
+      }
       // queue a launch over every request with items left
-      if (!ring_free.empty()) {
+      if (!ring_free.empty() && shared_work) {
         const int ring = ring_free.back();
         SegDesc* sh = S->seg_host[ring];
         uint32_t nseg = 0;
@@ -511,7 +635,7 @@ void sched_loop(bbtc_server* S) {
         // slow a large request); the rest goes shortest remaining work first.
         std::vector<Request*> cand;  // admission order
         for (auto& r : act)
-          if (r->next < r->nitems) cand.push_back(r.get());
+          if (r->next < r->nitems && !r->rows) cand.push_back(r.get());
         // A launch holds requests of one kind: all in the lane form (run by
         // the lane kernel) or none; the two kinds take turns while both have
         // work.
@@ -796,8 +920,13 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
   r->L = &L->L;
   r->tiles = bbtc::tile_table(L->L);
   const std::vector<bbtc::TaskWork> work = bbtc::task_work(L->L, tasks, ntasks);
-  r->plan = bbtc::describe_plan(r->tiles, L->L.deg, bbtc::refs_of(P, tasks, ntasks), kernel, nullptr,
-                                S->lane_form, &work);
+  const std::vector<bbtc::TaskRef> refs = bbtc::refs_of(P, tasks, ntasks);
+  r->plan = bbtc::describe_plan(r->tiles, L->L.deg, refs, kernel, nullptr, S->lane_form, &work);
+  bbtc::plan_request_rows(*r, L->L, refs, kernel, [&] {
+    // every task in the lane form, as a lane-only plan of the plain path
+    return S->lane_form ? r->plan.lane_only
+                        : bbtc::describe_plan(r->tiles, L->L.deg, refs, kernel, nullptr, true, &work).lane_only;
+  });
   r->ntasks = ntasks;
   r->nitems = r->plan.pref[ntasks];
   r->wpref.assign(ntasks + 1, 0.0);

@@ -79,6 +79,10 @@ struct Layout {
   // false for a layout built on the device without a host copy of its arrays:
   // only the per-tile fields above are valid then
   bool host_arrays = true;
+  // entries in the graph's longest row (over all bands), when known (the
+  // layout was built from a CSR)
+  uint32_t longest_row = 0;
+  bool longest_known = false;
   Buf<uint32_t> rowptr;
   Buf<uint32_t> cols;
   Buf<uint32_t> nzrows;
@@ -104,6 +108,8 @@ void parallel_copy(void* dst, const void* src, size_t bytes);
 // One past the largest column of g (0 when it has no entries); a layout
 // needs it at most cuts[P].
 uint64_t column_end(const Csr& g);
+// Entries in g's longest row.
+uint32_t longest_row(const Csr& g);
 // Throws BBTC_ERR_INVALID naming `what` when L holds no host arrays.
 void require_host_arrays(const Layout& L, const char* what);
 

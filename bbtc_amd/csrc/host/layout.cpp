@@ -131,8 +131,17 @@ uint64_t column_end(const Csr& g) {
   return mx;
 }
 
+uint32_t longest_row(const Csr& g) {
+  uint64_t best = 0;
+#pragma omp parallel for schedule(static, 1 << 16) reduction(max : best)
+  for (int64_t u = 0; u < int64_t(g.n); ++u) best = std::max<uint64_t>(best, g.off[u + 1] - g.off[u]);
+  return uint32_t(std::min<uint64_t>(best, 0xFFFFFFFFull));
+}
+
 Layout build_layout(const Csr& g, const std::vector<uint32_t>& cuts) {
   Layout L = layout_frame(g.n, g.m, cuts);
+  L.longest_row = longest_row(g);
+  L.longest_known = true;
   const uint32_t P = L.P;
   if (column_end(g) > cuts[P]) fail(BBTC_ERR_INVALID, "a column lies at or beyond cuts[P]");
   const uint64_t rp_total = L.rp_words;

@@ -544,6 +544,7 @@ typedef struct bbtc_server_stats {
   uint32_t peak_active, peak_segments;
   double busy_s;    /* sum of the counting launches' device times (HIP events) */
   double upload_s;  /* host time the upload thread spent copying requests in */
+  uint64_t row_form_requests;  /* requests counted in the row form (a launch of their own) */
 } bbtc_server_stats;
 
 bbtc_status bbtc_server_create(bbtc_ctx* ctx, const bbtc_server_opts* opts, bbtc_server** out);
@@ -552,7 +553,11 @@ bbtc_status bbtc_server_reset_clock(bbtc_server* s);
 double bbtc_server_now(const bbtc_server* s);
 /* Queues a count of tasks[0..n) of L, arriving at arrival_s on the server
  * clock (not decreasing between calls; the request waits until then). L and
- * per_task_out (may be NULL) must stay valid until the result is polled. */
+ * per_task_out (may be NULL) must stay valid until the result is polled.
+ * A request that bbtc_count would count in the row form (BBTC_KERNEL_ROWS,
+ * or AUTO where it pays; L built from a CSR with rows of at most 64 entries)
+ * is counted in that form, in a launch of its own that takes turns with the
+ * shared launches. */
 bbtc_status bbtc_server_submit(bbtc_server* s, const bbtc_layout* L, const bbtc_task* tasks, uint64_t n,
                                bbtc_kernel kernel, double arrival_s, uint64_t* per_task_out,
                                uint64_t* id_out);

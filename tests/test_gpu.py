@@ -698,6 +698,40 @@ def test_pinned_layouts(dev):
     assert np.array_equal(pt, want)
 
 
+def test_server_rows_form(dev):
+    # requests the plain path's AUTO counts in the row form (sparse G(n, m)
+    # graphs) mixed with ones it does not (RMAT): the server counts the
+    # first in launches of their own, as many as the plain plans take the
+    # row form, taking turns with shared launches of the others; every
+    # count per task against the oracle, with the lane kernel asked for
+    # instead (no row form) as well
+    specs = [("gnm", 30000, 150000, 31, "pal", 4), ("rmat", 13, 16, 32, "pal", 4), ("gnm", 5000, 30000, 33, "uni", 2),
+             ("gnm", 40000, 200000, 34, "uni", 8), ("rmat", 12, 8, 35, "uni", 1), ("gnm", 8000, 60000, 36, "pal", 1)]
+    reqs, want_rows = [], 0
+    for kind, a, b, seed, part, p in specs:
+        g = B.Graph.rmat(a, b, seed=seed) if kind == "rmat" else B.Graph.gnm(a, b, seed)
+        cuts = g.partition(part, p)
+        L = B.Layout(g, cuts)
+        t = L.tasks("desc")
+        rper, _ = O.RefLayout(g.offsets(), g.cols(), cuts).count(np.array(t.tolist(), dtype=np.uint32))
+        dg = dev.upload(L)
+        pl = dg.plan(t, "auto")
+        want_rows += int((pl.kinds() == 6).any())
+        pl.destroy()
+        dg.release()
+        reqs.append((L, t, rper))
+    assert want_rows >= 3
+    for kernel, rows in (("auto", want_rows), ("lane", 0)):
+        srv = B.Server(dev, heap_bytes=1 << 30, launch_us=300)
+        ids = [srv.submit(L, t, 0.0, kernel=kernel) for L, t, _ in reqs]
+        got = {r.id: r for r in srv.drain()}
+        st = srv.stats()
+        srv.destroy()
+        assert st["row_form_requests"] == rows, (kernel, st)
+        for i, (_, _, cpu) in zip(ids, reqs):
+            assert got[i].status == 0 and np.array_equal(got[i].per_task, cpu), (kernel, i)
+
+
 def test_server_alone_vs_mix(dev):
     reqs = _serving_requests()
     # alone: one request in the server at a time
