@@ -176,6 +176,9 @@ uint64_t items_for_work(const Request& r, uint64_t from, double work) {
   return x - from;
 }
 
+// Waiting requests of equal arrival time compared for the admission order.
+constexpr int kAdmitScan = 1024;
+
 struct Launch {
   hipEvent_t start = nullptr, done = nullptr;  // around the launch on the compute stream
   uint64_t items = 0;
@@ -361,23 +364,35 @@ void admit_loop(bbtc_server* S) {
             S->cv_admit.wait(lk);
             continue;
           }
-          auto& head = S->pending.front();
           const double t = S->now();
-          if (head->arrival > t) {
-            S->cv_admit.wait_for(lk, std::chrono::duration<double>(head->arrival - t));
+          if (S->pending.front()->arrival > t) {
+            S->cv_admit.wait_for(lk, std::chrono::duration<double>(S->pending.front()->arrival - t));
             continue;
           }
           if (S->sequential && S->active > 0) {  // plain mode: one request at a time
             S->cv_admit.wait(lk);
             continue;
           }
-          const uint64_t off = S->heap.alloc(head->bytes);
+          // Requests keep their arrival order, but of those that arrived at
+          // the same time as the first one waiting, the one of most
+          // estimated work goes first (for a batch submitted at once:
+          // longest first, so the largest counts overlap the uploads of the
+          // rest instead of running alone at the end).
+          auto pick = S->pending.begin();
+          if (!S->sequential) {
+            int scanned = 0;
+            for (auto it = std::next(pick); it != S->pending.end() && scanned < kAdmitScan; ++it, ++scanned) {
+              if ((*it)->arrival != S->pending.front()->arrival) break;
+              if ((*it)->wpref.back() > (*pick)->wpref.back()) pick = it;
+            }
+          }
+          const uint64_t off = S->heap.alloc((*pick)->bytes);
           if (off == ~0ull) {  // wait for a finished request to free memory
             S->cv_admit.wait(lk);
             continue;
           }
-          r = head;
-          S->pending.pop_front();
+          r = *pick;
+          S->pending.erase(pick);
           r->heap_off = off;
           r->admit_s = t;
           S->active++;

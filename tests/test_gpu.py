@@ -799,6 +799,28 @@ def test_server_heap_admission(dev):
     assert sorted(r.id for r in got) == sorted(ids)
 
 
+def test_server_admits_simultaneous_arrivals_heaviest_first(dev):
+    # three requests arriving together (after submission): admitted by
+    # estimated work, largest first; a later arrival waits for its time
+    specs = [B.Graph.gnm(2000, 8000, 3), B.Graph.rmat(16, 16, seed=4), B.Graph.rmat(13, 8, seed=5)]
+    reqs = []
+    for g in specs:
+        L = B.Layout(g, g.partition("pal", 4))
+        t = L.tasks("desc")
+        cpu, _ = O.RefLayout(g.offsets(), g.cols(), L.cuts).count(np.array(t.tolist(), dtype=np.uint32))
+        reqs.append((L, t, cpu))
+    srv = B.Server(dev, heap_bytes=1 << 30)
+    srv.reset_clock()
+    ids = [srv.submit(L, t, 0.3) for L, t, _ in reqs] + [srv.submit(reqs[0][0], reqs[0][1], 0.35)]
+    got = {r.id: r for r in srv.drain()}
+    srv.destroy()
+    admit = [got[i].admit_s for i in ids]
+    assert admit[1] < admit[2] < admit[0] <= admit[3], admit
+    assert all(a >= 0.3 for a in admit) and admit[3] >= 0.35
+    for i, (_, _, cpu) in zip(ids, reqs + [reqs[0]]):
+        assert got[i].status == 0 and np.array_equal(got[i].per_task, cpu)
+
+
 def test_server_poisson_arrivals(dev):
     from bbtc_amd.workloads import poisson_arrivals
     reqs = _serving_requests() * 3
