@@ -62,6 +62,25 @@ std::vector<TileDesc> tile_table(const Layout& L);
 // Host side of a plan: the form and rows per item of each task (kernel rule,
 // or choice[q] = 1 merge, 2 bitmap, 3 dense, 4 rdense, 5 lane) and the item
 // prefix.
+// Which tasks make_plan may list in an order of its own (see mixed_order):
+// none (on false), the first `upto`, or with `contested` the first
+// max(upto, first task a whole-plan launch's last claim part holds).
+struct MixSpec {
+  bool on = false;
+  uint64_t upto = 0;
+  bool contested = false;
+};
+
+// Per-task values of plan p in its own order -> the caller's order.
+template <class T>
+void to_caller_order(const bbtc_plan* p, const T* in, T* out) {
+  if (p->order.empty()) {
+    std::copy(in, in + p->ntasks, out);
+    return;
+  }
+  for (uint64_t q = 0; q < p->ntasks; ++q) out[p->order[q]] = in[q];
+}
+
 struct PlanDesc {
   std::vector<TaskDesc> td;       // max(n, 1) entries
   std::vector<uint64_t> pref;     // n + 1 entries

@@ -82,6 +82,10 @@ struct bbtc_plan {
   bbtc::SegDesc* segs = nullptr;        // the whole plan: [0] plain, [1] with clock counters
   std::vector<uint64_t> hprefix;        // host copy of item_prefix
   std::vector<uint32_t> hkind;          // form per task (1 search, 2 bitmap, 3 dense, 4 rdense, 5 lane)
+  // The plan's own task order: its task q is the caller's task order[q]
+  // (empty: the caller's order). Tasks [mixed, ntasks) keep their places.
+  std::vector<uint64_t> order;
+  uint64_t mixed = 0;
   bool lane_only = false;               // every task with items uses the lane form
   // Row form (a lane-only plan of a graph with a row table): launch() counts
   // every task in one count_rows launch over the graph's rows; task ranges,

@@ -10,6 +10,14 @@
 namespace bbtc {
 namespace dev {
 
+// bbtc_plan_create_tuned for the --border split (throws on error): the plan
+// may list tasks [0, upto) in an order of its own (with `contested`, up to
+// the first task of a whole-plan launch's last claim part if that is
+// further); tasks from plan_mixed on keep the caller's indices. Per-task
+// outputs (plan_copy_counts, bbtc_plan_results) are in the caller's order.
+bbtc_plan* plan_create_split(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n, bbtc_kernel kernel,
+                             const uint8_t* choice, uint64_t upto, bool contested);
+uint64_t plan_mixed(const bbtc_plan* p);
 // Zeroes every per-task counter of the plan on slot 0 and waits.
 void plan_zero_sync(bbtc_plan* p);
 // Enqueues the count of tasks [t0, t1) of the plan on slot `slot`.
@@ -21,7 +29,7 @@ uint64_t plan_batch_items(const bbtc_plan* p);
 // True when every launch enqueued on the slot has finished.
 bool plan_slot_idle(bbtc_plan* p, int slot);
 void plan_slot_wait(bbtc_plan* p, int slot);
-// Waits for all slots, copies all per-task counters to host.
+// Waits for all slots, copies all per-task counters to host (caller's order).
 void plan_copy_counts(bbtc_plan* p, uint64_t* out);
 int max_slots();
 // Task queue shared by the device and host workers (CountArgs::queue):

@@ -174,19 +174,22 @@ HybridResult run_shared(bbtc_ctx* ctx, bbtc_plan* plan, const Layout& L, const b
 }
 
 HybridResult run_hybrid(bbtc_ctx* ctx, bbtc_dgraph* g, const Layout& L, const bbtc_task* tasks,
-                        uint64_t n, uint64_t sep, int nthreads, int nstreams, bbtc_kernel kernel,
+                        uint64_t n, uint64_t sep_in, int nthreads, int nstreams, bbtc_kernel kernel,
                         const uint8_t* choice, uint64_t* per_task) {
-  if (sep < n) require_host_arrays(L, "the host tail of bbtc_count_hybrid");
-  bbtc_plan* plan = nullptr;
-  const bbtc_status st = bbtc_plan_create_tuned(ctx, g, tasks, n, kernel, choice, &plan);
-  if (st != BBTC_OK) fail(st, bbtc_last_error());
+  if (sep_in < n) require_host_arrays(L, "the host tail of bbtc_count_hybrid");
+  const int T = std::max(1, nthreads > 0 ? nthreads : omp_get_max_threads());
+  // The plan may list in its own order only tasks the host workers never
+  // take: all of them without host workers, else those before the border
+  // (and, with the shared queue, before the launch's last claim part).
+  const bool hosts = T > 1 && sep_in < n;
+  bbtc_plan* plan =
+      dev::plan_create_split(ctx, g, tasks, n, kernel, choice, hosts ? sep_in : n, hosts && dev::host_atomics(ctx));
   struct Guard {
     bbtc_plan* p;
     ~Guard() { bbtc_plan_destroy(p); }
   } guard{plan};
   dev::plan_zero_sync(plan);
-
-  const int T = std::max(1, nthreads > 0 ? nthreads : omp_get_max_threads());
+  const uint64_t sep = std::max(sep_in, dev::plan_mixed(plan));
   // estimated work of each task (algorithmic bytes with 4-byte offsets) and
   // its prefix over the list
   std::vector<double> pre(n + 1, 0.0);

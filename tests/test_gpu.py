@@ -139,6 +139,34 @@ def test_lane_slots_wide_bands(dev, p):
     dg.release()
 
 
+def test_plan_own_order(dev):
+    # an AUTO plan over several forms lists its tasks in an order of its own
+    # (the forms interleaved); every per-task output comes back in the
+    # caller's order: counts against the oracle, forms and the tasks the
+    # clock profile finds empty against a plan of the reversed list
+    g = B.Graph.rmat(17, 16, seed=7)
+    cuts = g.partition("pal", 8)
+    L = B.Layout(g, cuts)
+    t = L.tasks("desc")
+    rper, _ = O.RefLayout(g.offsets(), g.cols(), cuts).count(np.array(t.tolist(), dtype=np.uint32))
+    dg = dev.upload(L)
+    out = []
+    for tl in (t, t[::-1].copy()):
+        p = dg.plan(tl, "auto")
+        p.launch()
+        per, tot = p.results()
+        out.append((per, tot, p.kinds(), p.profile(), p.time_tasks()))
+        p.destroy()
+    dg.release()
+    (per, tot, kinds, cyc, ms), (per2, tot2, kinds2, cyc2, ms2) = out
+    assert len(set(kinds.tolist())) >= 2, np.bincount(kinds)
+    assert np.array_equal(per, rper) and tot == int(rper.sum())
+    assert np.array_equal(per2, rper[::-1]) and tot2 == tot
+    assert np.array_equal(kinds2, kinds[::-1])
+    assert np.array_equal(cyc > 0, cyc2[::-1] > 0) and (cyc[rper > 0] > 0).all()
+    assert ms.shape == (len(t),) and (ms >= 0).all() and (ms2 >= 0).all()
+
+
 @pytest.mark.parametrize("case", range(8))
 def test_rows_form_vs_oracle(dev, case):
     # the row form: every task of a lane-only plan counted by one launch over
