@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -697,28 +698,44 @@ uint64_t contested_from(const std::vector<uint64_t>& pref, uint64_t n, uint64_t 
 }
 
 // The order in which a plan lists tasks [0, k) of D (tasks [k, n) keep their
-// places): the tasks of each form keep their relative order, and the forms
-// are interleaved in proportion to the AUTO rule's estimated clocks, each
-// task placed at the middle of its share of its form's total. Forms are
-// bound by different parts of the machine (the rdense probes by 128-byte
-// line fetches, the merge searches by instruction issue and LDS latency), so
-// a claim part that works through a mix keeps both busy: RMAT s26 with 32
-// pal cuts counts in 558.7 against 580.4 ms in descending weight order
-// (570.0 ms shuffled, 562.4 ms with two groups, 564.1 ms by measured task
-// times).
+// places). Within each form, tasks that share a bc tile (the tile whose rows
+// are read at random) come together, the tiles of most estimated work
+// first, each tile's tasks in the caller's order; the forms are then
+// interleaved in proportion to the AUTO rule's estimated clocks, each task
+// placed at the middle of its share of its form's total. Forms are bound by
+// different parts of the machine (the rdense probes by 128-byte line
+// fetches, the merge searches by instruction issue and LDS latency), so a
+// claim part that works through a mix keeps both busy, and the tasks of one
+// bc tile run close together, while its rows are in the L2 and MALL. RMAT
+// s26 with 32 pal cuts: 542.6 ms against 580.4 ms in descending weight order
+// (forms interleaved, each form in descending order: 558.7 ms; its bc tiles
+// in index order 555.3, or by (k, j) 551.1 ms; grouped by ab or ac tile
+// instead: 675-680 ms, from load imbalance).
 std::vector<uint64_t> mixed_order(const std::vector<TileDesc>& tiles, const PlanDesc& D,
-                                  const std::vector<TaskWork>& work, uint64_t k, uint64_t n) {
+                                  const std::vector<TaskWork>& work, const std::vector<TaskRef>& refs, uint64_t k,
+                                  uint64_t n) {
   std::vector<double> cost(k, 0.0);
+  std::map<uint32_t, double> tile_cost;  // bc tile -> estimated clocks of its tasks
   for (uint64_t q = 0; q < k; ++q) {
-    if (D.pref[q + 1] == D.pref[q]) continue;
-    const TaskDesc& d = D.td[q];
-    const FormCosts c = form_costs(work[q], tiles[d.ac].dwpr);
-    cost[q] = d.kind == kKindBitmap ? c.bitmap
-              : d.kind == kKindDense ? c.dense
-              : d.kind == kKindRdense ? c.rdense
-                                      : c.merge;
-    if (!(cost[q] > 0) || !std::isfinite(cost[q])) cost[q] = 0;
+    if (D.pref[q + 1] > D.pref[q]) {
+      const TaskDesc& d = D.td[q];
+      const FormCosts c = form_costs(work[q], tiles[d.ac].dwpr);
+      cost[q] = d.kind == kKindBitmap ? c.bitmap
+                : d.kind == kKindDense ? c.dense
+                : d.kind == kKindRdense ? c.rdense
+                                        : c.merge;
+      if (!(cost[q] > 0) || !std::isfinite(cost[q])) cost[q] = 0;
+    }
+    tile_cost[refs[q].bc] += cost[q];
   }
+  std::vector<uint64_t> by_tile(k);
+  for (uint64_t q = 0; q < k; ++q) by_tile[q] = q;
+  std::stable_sort(by_tile.begin(), by_tile.end(), [&](uint64_t a, uint64_t b) {
+    const uint32_t ta = refs[a].bc, tb = refs[b].bc;
+    if (ta == tb) return false;
+    const double ca = tile_cost[ta], cb = tile_cost[tb];
+    return ca != cb ? ca > cb : ta < tb;
+  });
   // per form (in form order), each task's position: the middle of its share
   std::vector<std::pair<double, uint64_t>> pos;
   pos.reserve(k);
@@ -727,7 +744,7 @@ std::vector<uint64_t> mixed_order(const std::vector<TileDesc>& tiles, const Plan
     for (uint64_t q = 0; q < k; ++q)
       if (D.hkind[q] == kind) total += cost[q];
     double run = 0;
-    for (uint64_t q = 0; q < k; ++q)
+    for (uint64_t q : by_tile)
       if (D.hkind[q] == kind) {
         pos.emplace_back(total > 0 ? (run + cost[q] / 2) / total : 0.0, q);
         run += cost[q];
@@ -768,7 +785,7 @@ bbtc_plan* make_plan(bbtc_ctx* ctx, bbtc_dgraph* g, const std::vector<TaskRef>& 
       if (mix.contested) k = std::max<uint64_t>(mix.upto, contested_from(D.pref, n, 0));
       k = std::min<uint64_t>(k, n);
       if (k > 1) {
-        p->order = mixed_order(g->htiles, D, *work, k, n);
+        p->order = mixed_order(g->htiles, D, *work, refs_in, k, n);
         p->mixed = k;
         mixed_refs.resize(n);
         mixed_work.resize(n);
