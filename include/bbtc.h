@@ -343,7 +343,11 @@ bbtc_status bbtc_graph_tasks_stats(bbtc_dgraph* g, const bbtc_task* tasks, uint6
 /* A plan binds a graph to a task list and a kernel choice: it uploads task
  * descriptors and the work-item table once, so repeated launches do no host
  * work. Replaces the per-task launch loop of BBTc::RunGpuKernels /
- * NoOfTrianglesInGPU (bb_tc.cpp:76-115, 142-181). */
+ * NoOfTrianglesInGPU (bb_tc.cpp:76-115, 142-181). With BBTC_KERNEL_AUTO and
+ * no choice array the plan runs the tasks in an order of its own (each
+ * form's tasks grouped by bc tile, the forms interleaved by estimated
+ * work); every per-task output (results, kinds, clocks, times) is in the
+ * order of `tasks`. */
 bbtc_status bbtc_plan_create(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* tasks, uint64_t n,
                              bbtc_kernel kernel, bbtc_plan** out);
 /* As bbtc_plan_create, with a per-task override choice[q]: in bits 0-2, 0
