@@ -697,6 +697,8 @@ uint64_t contested_from(const std::vector<uint64_t>& pref, uint64_t n, uint64_t 
   return t + 1;
 }
 
+}  // namespace
+
 // The order in which a plan lists tasks [0, k) of D (tasks [k, n) keep their
 // places). Within each form, tasks that share a bc tile (the tile whose rows
 // are read at random) come together, the tiles of most estimated work
@@ -759,6 +761,8 @@ std::vector<uint64_t> mixed_order(const std::vector<TileDesc>& tiles, const Plan
   for (uint64_t q = k; q < n; ++q) order[q] = q;
   return order;
 }
+
+namespace {
 
 // Builds a plan over tile-addressed tasks of a resident graph. work: the
 // tasks' statistics for the AUTO form rule (NULL: those of a single graph's

@@ -88,6 +88,15 @@ struct PlanDesc {
   uint64_t bitmap_tasks = 0;
   bool lane_only = true;          // every task with work items uses the lane form
 };
+
+// The order in which a plan runs tasks [0, k) of D (refs, work: the tasks in
+// D's order; tasks [k, n) keep their places): each form's tasks grouped by
+// bc tile, the tiles of most estimated work first, and the forms
+// interleaved by estimated work. Entry q of the result is D's task order[q].
+std::vector<uint64_t> mixed_order(const std::vector<TileDesc>& tiles, const PlanDesc& D,
+                                  const std::vector<TaskWork>& work, const std::vector<TaskRef>& refs, uint64_t k,
+                                  uint64_t n);
+
 // auto_lanes: BBTC_KERNEL_AUTO may take the lane form for every task.
 // work (may be NULL): each task's work statistics, aligned with refs; with
 // them BBTC_KERNEL_AUTO takes the form of least estimated cost

@@ -137,6 +137,8 @@ struct Request {
   std::vector<uint32_t> rcuts, lex2plan;
   uint64_t o_rows = 0, o_rowlen = 0, o_rcuts = 0, o_lex2plan = 0;
   RowArgs ra{};
+  // the plan's task q is the caller's task order[q] (empty: the caller's order)
+  std::vector<uint64_t> order;
   std::vector<DenseJob> jobs;  // bitmap rows built on the device after the upload
   uint64_t job_rows = 0;
   uint64_t bytes = 0;
@@ -506,7 +508,11 @@ void finish(bbtc_server* S, Request& r) {
   uint64_t tot = 0;
   for (uint64_t t = 0; t < r.ntasks; ++t) tot += per[t];
   r.total = tot;
-  if (r.per_task_out && r.ntasks) std::copy(per.begin(), per.begin() + r.ntasks, r.per_task_out);
+  if (r.per_task_out && r.ntasks) {
+    if (r.order.empty()) std::copy(per.begin(), per.begin() + r.ntasks, r.per_task_out);
+    else
+      for (uint64_t q = 0; q < r.ntasks; ++q) r.per_task_out[r.order[q]] = per[q];
+  }
   if (r.uploaded) (void)hipEventDestroy(r.uploaded);
   r.uploaded = nullptr;
   if (r.copied) (void)hipEventDestroy(r.copied);
@@ -942,6 +948,19 @@ bbtc_status bbtc_server_submit(bbtc_server* S, const bbtc_layout* L, const bbtc_
     return S->lane_form ? r->plan.lane_only
                         : bbtc::describe_plan(r->tiles, L->L.deg, refs, kernel, nullptr, true, &work).lane_only;
   });
+  if (!r->rows && kernel == BBTC_KERNEL_AUTO && !r->plan.lane_only && ntasks > 1) {
+    // the tasks in the order a one-graph plan runs them (each form's tasks
+    // grouped by bc tile, the forms interleaved); the counters go back to
+    // the caller's order
+    r->order = bbtc::mixed_order(r->tiles, r->plan, work, refs, ntasks, ntasks);
+    std::vector<bbtc::TaskRef> mrefs(ntasks);
+    std::vector<bbtc::TaskWork> mwork(ntasks);
+    for (uint64_t q = 0; q < ntasks; ++q) {
+      mrefs[q] = refs[r->order[q]];
+      mwork[q] = work[r->order[q]];
+    }
+    r->plan = bbtc::describe_plan(r->tiles, L->L.deg, mrefs, kernel, nullptr, S->lane_form, &mwork);
+  }
   r->ntasks = ntasks;
   r->nitems = r->plan.pref[ntasks];
   r->wpref.assign(ntasks + 1, 0.0);
