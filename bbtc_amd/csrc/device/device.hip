@@ -1522,8 +1522,21 @@ bbtc_status bbtc_count_noalloc(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* t
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   const std::vector<bbtc::TaskRef> refs = bbtc::refs_of(g->P, tasks, n);
   const std::vector<bbtc::TaskWork> work = bbtc::graph_work(g, refs);
-  const bbtc::PlanDesc D =
+  bbtc::PlanDesc D =
       bbtc::describe_plan(g->htiles, g->deg, refs, kernel, nullptr, true, work.empty() ? nullptr : &work);
+  // the tasks in the order bbtc_count's plan runs them; counts go back to
+  // the caller's order
+  std::vector<uint64_t> order;
+  if (kernel == BBTC_KERNEL_AUTO && !work.empty() && !D.lane_only && n > 1) {
+    order = bbtc::mixed_order(g->htiles, D, work, refs, n, n);
+    std::vector<bbtc::TaskRef> mrefs(n);
+    std::vector<bbtc::TaskWork> mwork(n);
+    for (uint64_t q = 0; q < n; ++q) {
+      mrefs[q] = refs[order[q]];
+      mwork[q] = work[order[q]];
+    }
+    D = bbtc::describe_plan(g->htiles, g->deg, mrefs, kernel, nullptr, true, &mwork);
+  }
   const uint64_t nitems = D.pref[n];
   unsigned char* base = ctx->scratch;
   auto* ctl = reinterpret_cast<bbtc::SlotCtl*>(base + bbtc::kScratchCtl);
@@ -1569,6 +1582,10 @@ bbtc_status bbtc_count_noalloc(bbtc_ctx* ctx, bbtc_dgraph* g, const bbtc_task* t
     if (n) HIP_OK(hipMemcpyAsync(per_task, counts, n * 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     for (uint64_t q = 0; q < n; ++q) t += per_task[q];
+    if (!order.empty()) {
+      const std::vector<uint64_t> mine(per_task, per_task + n);
+      for (uint64_t q = 0; q < n; ++q) per_task[order[q]] = mine[q];
+    }
   } else {
     // the total without a host array: counters copied back a block at a time
     uint64_t buf[512];
