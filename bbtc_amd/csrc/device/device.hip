@@ -1825,6 +1825,37 @@ bbtc_status bbtc_batch_create(bbtc_ctx* ctx, const bbtc_layout* const* layouts, 
       b->work[x] = work[order[x].q];
       for (int c = 0; c < 3; ++c) ijk[3 * x + c] = b->task_ijk[3 * order[x].q + c];
     }
+    if (order_kind == BBTC_BATCH_SMALL_GRAPHS_FIRST && kernel == BBTC_KERNEL_AUTO && !refs.empty()) {
+      // each graph's tasks (a contiguous block) in the order a one-graph
+      // plan runs them: each form grouped by bc tile, the forms interleaved
+      const bbtc::PlanDesc D = bbtc::describe_plan(A->htiles, A->deg, refs, kernel, nullptr, true, &b->work);
+      std::vector<bbtc::TaskRef> r2(refs.size());
+      std::vector<bbtc::TaskWork> w2(refs.size());
+      std::vector<uint32_t> ijk2(ijk.size());
+      size_t lo = 0;
+      while (lo < refs.size()) {
+        size_t hi = lo + 1;
+        while (hi < refs.size() && refs[hi].graph == refs[lo].graph) ++hi;
+        const size_t len = hi - lo;
+        bbtc::PlanDesc G;
+        G.td.assign(D.td.begin() + long(lo), D.td.begin() + long(hi));
+        G.hkind.assign(D.hkind.begin() + long(lo), D.hkind.begin() + long(hi));
+        G.pref.resize(len + 1);
+        for (size_t x = 0; x <= len; ++x) G.pref[x] = D.pref[lo + x] - D.pref[lo];
+        const std::vector<bbtc::TaskRef> gr(refs.begin() + long(lo), refs.begin() + long(hi));
+        const std::vector<bbtc::TaskWork> gw(b->work.begin() + long(lo), b->work.begin() + long(hi));
+        const std::vector<uint64_t> ord = bbtc::mixed_order(A->htiles, G, gw, gr, len, len);
+        for (size_t x = 0; x < len; ++x) {
+          r2[lo + x] = refs[lo + ord[x]];
+          w2[lo + x] = b->work[lo + ord[x]];
+          for (int c = 0; c < 3; ++c) ijk2[3 * (lo + x) + c] = ijk[3 * (lo + ord[x]) + c];
+        }
+        lo = hi;
+      }
+      refs.swap(r2);
+      b->work.swap(w2);
+      ijk.swap(ijk2);
+    }
     b->refs.swap(refs);
     b->task_ijk.swap(ijk);
     const uint64_t ng = std::max<uint64_t>(ngraphs, 1);
