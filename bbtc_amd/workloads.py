@@ -265,10 +265,13 @@ def run_server(dev: Device, layouts, tasks, arrivals=None, heap_bytes: int = 200
         srv = Server(dev, heap_bytes=heap_bytes, **opts)
     try:
         srv.reset_clock()
+        t_sub = time.perf_counter()
         ids = [srv.submit(layouts[x], tasks[x], 0.0 if arrivals is None else float(arrivals[x]))
                for x in range(n)]
+        t_sub = time.perf_counter() - t_sub
         res = {r.id: r for r in srv.drain()}
         stats = srv.stats()
+        stats["submit_s"] = t_sub  # host time of the submit calls (request set-up)
     finally:
         if own:
             srv.destroy()
@@ -337,7 +340,8 @@ def _serving_runs(dev, layouts, tasks, srv, n, p_tot, p_per, p_cap, rate, load, 
         server_p50_ms=percentile(s_lat, 50) * 1e3, server_p99_ms=percentile(s_lat, 99) * 1e3,
         server_launches=st2["launches"], server_peak_active=st2["peak_active"],
         server_peak_resident_gb=st2["peak_resident_bytes"] / 1e9,
-        server_busy_s=st["busy_s"], server_upload_s=st["upload_s"],
+        server_busy_s=st["busy_s"], server_upload_s=st["upload_s"], server_submit_s=st["submit_s"],
+        server_makespan_s=s_make,
         pinned_layouts=all(L.pinned for L in layouts),
     )
     log(f"serving at {lam:.2f} req/s: server p50/p99 {res['server_p50_ms']:.1f}/{res['server_p99_ms']:.1f} ms, "
