@@ -22,7 +22,7 @@ Prints one JSON line:
                 the CSR (with the task work statistics), autotuning
                 (--kernel tuned only), median count; beside it the host
                 layout build plus upload it replaces
-  cpu_baseline  the CPU oracle (restated reference CPU path) on a
+  cpu_baseline  (taken after every GPU workload) the CPU oracle (restated reference CPU path) on a
                 systematic 1/s sample of every task's rows, scaled by s
                 (--cpu-full: every task once, measured)
   c2            G(n=2^22, m=2^26) seed 2, pal 16 (816 tasks), AUTO forms
@@ -204,9 +204,10 @@ def cpu_baseline(g, cuts, tasks, per_task_gpu, m, budget_s, full=False):
     return out
 
 
-def count_c2(B, dev, steps, cpu=True):
+def count_c2(B, dev, steps, cpu=True, defer_cpu=False):
     """G(n=2^22, m=2^26) seed 2, pal 16, AUTO forms: one resident count, and
-    the oracle on every task beside it."""
+    the oracle on every task beside it. With defer_cpu, returns (result, the
+    oracle's arguments) for a CPU baseline taken later."""
     t0 = time.time()
     g = B.Graph.gnm_device(dev, 1 << 22, 1 << 26, 2)
     cuts = g.partition("pal", 16)
@@ -247,6 +248,8 @@ def count_c2(B, dev, steps, cpu=True):
         out["hbm_frac"] = traffic / mean_s / 1e9 / HBM_PEAK_GBS
         out["traffic_over_algorithmic"] = traffic / total_bytes
         out["traffic_source"] = src
+    if defer_cpu:
+        return out, ((g, cuts, tasks, per, g.m, 0) if cpu else None)
     if cpu:
         out["cpu_baseline"] = cpu_baseline(g, cuts, tasks, per, g.m, 0, full=True)
         log(f"c2 cpu baseline: {out['cpu_baseline']['seconds_per_count']} s per count")
@@ -416,9 +419,9 @@ def main():
     if int(per.sum()) != total:
         raise RuntimeError("per-task counts do not sum to the total")
 
+    # the CPU baselines run after every GPU workload: their host threads do
+    # not share the host with the timed GPU paths' own host threads
     cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(g, cuts, tasks, per, m, args.cpu_seconds, full=args.cpu_full)
 
     workload = f"rmat-s{args.scale}-ef{args.edge_factor} {args.partitioner} ncut{args.ncut}"
     traffic, traffic_src = measured_traffic(workload, args.kernel)
@@ -470,19 +473,18 @@ def main():
     if not args.no_shards:
         out["shards"] = shard_shares(dg, tasks, per, ev_median_ms)
     dg.release()
-    del plan, dg, L, g
+    del plan, dg, L
+    s26_cpu_args = None if args.no_cpu else (g, cuts, tasks, per, m, args.cpu_seconds)
 
     from bbtc_amd import workloads as W
 
     def rounded(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
 
+    c2_cpu_args = None
     if not args.no_c2:
-        c2 = count_c2(B, dev, steps=3, cpu=not args.no_cpu)
-        cpu2 = c2.pop("cpu_baseline", None)
+        c2, c2_cpu_args = count_c2(B, dev, steps=3, cpu=not args.no_cpu, defer_cpu=True)
         out["c2"] = rounded(c2)
-        if cpu2:
-            out["c2"]["cpu_baseline"] = cpu2
 
     if args.batched_graphs:
         res = W.run_batched(dev, W.batched_specs(args.batched_graphs), log=log, tune=not args.no_batch_tune)
@@ -492,6 +494,11 @@ def main():
         res = W.run_serving(dev, W.serving_specs(args.serving_requests), log=log)
         out["serving"] = rounded(dict(res, metric="graphs/s at capacity", unit="graphs/s",
                                       value=res["server_capacity_gps"], vs_plain=res["capacity_ratio"]))
+    if s26_cpu_args:
+        out["cpu_baseline"] = cpu_baseline(*s26_cpu_args, full=args.cpu_full)
+    if c2_cpu_args:
+        out["c2"]["cpu_baseline"] = cpu_baseline(*c2_cpu_args, full=True)
+        log(f"c2 cpu baseline: {out['c2']['cpu_baseline']['seconds_per_count']} s per count")
     print(json.dumps(out), flush=True)
 
 
